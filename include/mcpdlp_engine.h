@@ -1,0 +1,147 @@
+/*
+ * mcpdlp_engine.h -- thin C-ABI between the C++20 host library and the HIP
+ * PDHG engine (libmcpdlp_hip.so).
+ *
+ * The reference has no device layer: its solve_pdlp runs the PDHG loop on the
+ * host (src/lp/first_order/pdlp.cpp:139-276).  These entry points replace the
+ * pieces of that function one for one:
+ *   mcp_load        -- the model copy (pdlp.cpp:150) + host->device upload
+ *   mcp_setup       -- equilibrate_model (pdlp.cpp:152 -> src/scale/ruiz_scaling.cpp:152-247)
+ *                      and the norm/sentinel setup (pdlp.cpp:153-185)
+ *   mcp_solve       -- start point + the iteration loop and exits (pdlp.cpp:186-275)
+ *   mcp_fetch       -- the unscaled averaged iterate and residuals the result carries
+ *                      (pdlp.cpp:236-244, 267-274)
+ *   mcp_spmv        -- spmv / spmv_t (pdlp.cpp:22-44) on the loaded, scaled matrix
+ * A "problem" is B >= 1 independent LPs packed block-diagonally; B == 1 is
+ * the reference's single solve, B > 1 the batched extension.  Plain pointers
+ * and sizes only; every call returns 0 or an MC_ERR_* code (mcpdlp.h) and
+ * never throws; mcp_last_error() holds the message (thread-local).
+ * An engine owns one HIP stream and its device buffers; use one engine per
+ * host thread (calls on one engine are not synchronized).
+ */
+#ifndef MCPDLP_ENGINE_H_
+#define MCPDLP_ENGINE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcp_engine mcp_engine;
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int mcp_device_count(int32_t* count);
+int mcp_engine_create(int32_t device, mcp_engine** out);
+void mcp_engine_destroy(mcp_engine* e);
+const char* mcp_last_error(void);
+
+/* Packed problem in host memory (int32 indices; nnz < 2^31).  The matrix is
+ * given in both orientations: CSR of A (rows) and CSR of A^T (= CSC of A),
+ * entries sorted by index inside each row/column.  Bounds use IEEE +-inf for
+ * infinite kinds.  `objective` is the model's (unsigned) objective; `sense`
+ * holds +1 (minimize) / -1 (maximize) per instance. */
+typedef struct mcp_problem {
+    int32_t instances;
+    int32_t rows, cols;
+    int64_t nnz;
+    const int32_t* inst_row;   /* [B+1] */
+    const int32_t* inst_col;   /* [B+1] */
+    const int32_t* row_ptr;    /* [m+1] CSR(A) */
+    const int32_t* col_idx;    /* [nnz] */
+    const double* row_val;     /* [nnz] */
+    const int32_t* col_ptr;    /* [n+1] CSR(A^T) */
+    const int32_t* row_idx;    /* [nnz] */
+    const double* col_val;     /* [nnz] */
+    const double* row_lower;   /* [m] */
+    const double* row_upper;   /* [m] */
+    const double* var_lower;   /* [n] */
+    const double* var_upper;   /* [n] */
+    const double* objective;   /* [n] */
+    const double* sense;       /* [B] */
+    const double* offset;      /* [B] */
+} mcp_problem;
+
+/* Tile shape policy for the CSR-stream SpMV (autotuner knobs). */
+typedef struct mcp_tile_policy {
+    int32_t nnz_budget;     /* nonzeros staged in LDS per tile (<= 2048)        */
+    int32_t per_lane;       /* target nonzeros per lane when picking lanes/row  */
+} mcp_tile_policy;
+
+/* Upload + tiling.  Reuses device buffers when sizes allow. */
+int mcp_load(mcp_engine* e, const mcp_problem* p);
+
+typedef struct mcp_setup_opts {
+    int32_t ruiz;              /* PdlpOptions::ruiz_scaling */
+    int32_t ruiz_iterations;   /* PdlpOptions::ruiz_iterations */
+    int32_t exact;             /* MC_ORDER_REFERENCE tiling (1 lane per row) */
+    int32_t autotune;          /* time candidate tile policies on the device */
+} mcp_setup_opts;
+int mcp_setup(mcp_engine* e, const mcp_setup_opts* o);
+
+typedef struct mcp_solve_opts {
+    uint64_t max_iterations;
+    uint64_t restart_every;
+    double primal_tolerance, dual_tolerance, gap_tolerance;
+    double step_size_reduction;
+    int32_t restart_strategy;
+    int32_t adaptive_step_size;
+    double restart_reduction_factor;
+    int32_t adaptive_primal_weight;
+    int32_t precision;          /* MC_PRECISION_* */
+    double initial_primal_weight;
+    double primal_weight_smoothing;
+    double polish_threshold;
+    int32_t reduction_order;    /* MC_ORDER_* */
+    int32_t graph_chunk;        /* iterations per HIP graph; 0 => default; <0 => no graphs */
+    const double* warm_primal;  /* optional [n] host, original space */
+    const double* warm_dual;    /* optional [m] host, original space (reported convention) */
+} mcp_solve_opts;
+
+typedef struct mcp_solve_stats {
+    double kernel_ms;           /* device time of the iteration loop */
+    double setup_ms;            /* device time of mcp_setup           */
+    double h2d_ms;              /* time of the last mcp_load upload   */
+    uint64_t iterations;        /* global iterations executed          */
+    int32_t graphs_built;
+    int32_t chunks;
+} mcp_solve_stats;
+int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats);
+
+/* Per-instance outcome. */
+typedef struct mcp_inst_result {
+    int32_t status;             /* MC_PDLP_* */
+    int32_t mode;               /* 0 FP64, 1 FP32 (final iterate precision) */
+    uint64_t iterations;
+    uint64_t fp32_iterations;
+    double objective;
+    double primal_infeasibility, dual_infeasibility, duality_gap;
+    double omega, eta;
+} mcp_inst_result;
+
+/* Copy the unscaled averaged iterate (x[n], y[m]; either may be NULL) and the
+ * per-instance results ([B]; may be NULL) to the host.  *d2h_ms = copy time. */
+int mcp_fetch(mcp_engine* e, double* x, double* y, mcp_inst_result* res, double* d2h_ms);
+
+/* SpMV on the loaded problem's (scaled, after mcp_setup) matrix: transpose=0
+ * computes out[m] = A v[n], transpose=1 out[n] = A^T v[m].  precision selects
+ * the FP64 or FP32 value copy (vectors are converted at the boundary);
+ * order=MC_ORDER_REFERENCE sums each row serially.  reps > 1 repeats the
+ * launch; *avg_us receives the mean device time per launch. */
+int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
+             const double* v, double* out, int32_t reps, double* avg_us);
+
+/* Scaled matrix values/scales after mcp_setup (for parity tests of the
+ * device Ruiz): CSC(A)-ordered values [nnz] and row/col scales. */
+int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* col_scale,
+                    int32_t* passes, int32_t* converged);
+
+/* Current tile policies (after mcp_setup / autotune) and tile counts. */
+int mcp_get_tiling(mcp_engine* e, mcp_tile_policy* rows_policy, mcp_tile_policy* cols_policy,
+                   int32_t* row_tiles, int32_t* col_tiles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCPDLP_ENGINE_H_ */

@@ -1,0 +1,97 @@
+// dev.hpp -- device pointers of one packed problem + iterate state, passed
+// by value to every kernel, and the host-side launch helpers of kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace mcp {
+
+struct Dev {
+    int32_t B;      // instances
+    int32_t m, n;   // packed rows / columns
+    int32_t ntA, ntT;
+    // structure
+    const int32_t* rp;  // CSR(A) row pointers [m+1]
+    const int32_t* ci;  // CSR(A) column ids [nnz]
+    const int32_t* cp;  // CSR(A^T) pointers [n+1]
+    const int32_t* ri;  // CSR(A^T) row ids [nnz]
+    double* av64;       // CSR(A) values (scaled in place by the setup)
+    double* tv64;       // CSR(A^T) values
+    float* av32;
+    float* tv32;
+    const int32_t* inst_row;   // [B+1] packed row offsets
+    const int32_t* inst_col;   // [B+1]
+    const int32_t* row_inst;   // [m] instance of each row
+    const int32_t* col_inst;   // [n]
+    const Tile* tA;            // tiles over rows of A
+    const Tile* tT;            // tiles over rows of A^T
+    const int32_t* tA_begin;   // [B+1] tile range of each instance in tA
+    const int32_t* tT_begin;   // [B+1]
+    // original model (residuals are evaluated on it); infinite bounds = IEEE +-inf
+    double* orow_lo;
+    double* orow_hi;
+    double* ovar_lo;
+    double* ovar_hi;
+    const double* obj;     // unsigned original objective [n]
+    const double* sense;   // [B] +1 minimize / -1 maximize
+    const double* offset;  // [B] objective offset
+    // scaling
+    double* row_scale;  // [m]
+    double* col_scale;  // [n]
+    double* rtmp;       // [m] setup scratch (Ruiz row max / delta)
+    double* ctmp;       // [n]
+    // scaled iteration data
+    double* blo;
+    double* bhi;
+    double* rnorm;   // [m] row L1 norm of scaled A (floored)
+    double* c;
+    double* slo;
+    double* shi;
+    double* cnorm;   // [n]
+    float4* rowpk;   // [m] {blo, bhi, 1/rnorm, 0} in FP32 (+-inf for open bounds)
+    float4* colpk;   // [n] {c, lo, hi, 1/cnorm}
+    // iterates
+    double *x, *xbar, *xavg, *dx;
+    double *y, *yavg;
+    float *x32, *xbar32, *xavg32, *dx32;
+    float *y32, *yavg32;
+    double* ax;    // [m] reference-order mode: A*v results
+    double* aty;   // [n] reference-order mode: A^T*v results
+    // reduction partials (one slot group per tile)
+    double* part_dxn;   // [ntT]
+    double* part_adx;   // [ntA]
+    double* part_rows;  // [2*ntA]: max primal violation, dual-objective part
+    double* part_cols;  // [4*ntT]: max violation, dual residual^2, primal obj, dual obj part
+    InstState* st;      // [B]
+    int64_t* iter_base; // global iteration counter at the start of the current chunk
+    int32_t* counters;  // [0] running instances, [1] running instances in FP32
+};
+
+// ---- launch helpers (kernels.hip) ----
+// Iteration kernels.  `t` is the iteration's offset inside the chunk;
+// FP32 = iterate precision; EXACT = reference reduction order.
+void launch_primal(const Dev& d, int t, bool fp32, bool adapt, bool exact, hipStream_t s);
+void launch_dual(const Dev& d, int t, bool fp32, bool exact, hipStream_t s);
+void launch_adx(const Dev& d, int t, bool fp32, bool exact, hipStream_t s);
+void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s);
+void launch_resid(const Dev& d, bool fp32, bool exact, hipStream_t s);
+void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
+                           hipStream_t s);
+void launch_post_check(const Dev& d, hipStream_t s);
+void launch_advance(const Dev& d, int len, hipStream_t s);
+// Setup kernels.
+void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);
+void launch_setup_data(const Dev& d, hipStream_t s);
+void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm_y, hipStream_t s);
+void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, bool mixed,
+                        hipStream_t s);
+void launch_fill(double* p, double v, int64_t n, hipStream_t s);
+void launch_unscale(const Dev& d, double* x_out, double* y_out, hipStream_t s);
+// Plain SpMV over a tile set: out[r] = sum val * vec (tests and micro-benchmarks).
+void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const int32_t* idx,
+                       const void* val, const void* vec, void* out, bool fp32, bool exact,
+                       hipStream_t s);
+
+}  // namespace mcp

@@ -1,0 +1,731 @@
+// engine.hip -- host side of the HIP PDHG engine: device buffers, tiling,
+// the chunked iteration loop (HIP-graph captured) and the C-ABI of
+// include/mcpdlp_engine.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mcpdlp.h"
+#include "../../include/mcpdlp_engine.h"
+#include "dev.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct EngineError : std::runtime_error {
+    int code;
+    EngineError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        const hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                            \
+            throw EngineError(e_ == hipErrorOutOfMemory ? MC_ERR_OUT_OF_MEMORY : MC_ERR_DEVICE, \
+                              std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+void require(bool ok, const std::string& msg) {
+    if (!ok) throw EngineError(MC_ERR_INVALID_ARGUMENT, msg);
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return MC_OK;
+    } catch (const EngineError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return MC_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return MC_ERR_INTERNAL;
+    }
+}
+
+int lanes_log2(int len, int per_lane) {
+    int lg = 0;
+    while (lg < 6 && len > (per_lane << lg)) ++lg;
+    return lg;
+}
+
+// Cut rows [inst[b], inst[b+1]) of every instance into tiles: consecutive
+// rows whose nonzeros fit `budget`, at most kBlock >> lg rows (lg = log2 of
+// lanes per row, from the tile's longest row); a row longer than the budget
+// is a tile of its own.  exact => 1 lane per row.
+void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& inst, int B,
+                 mcp_tile_policy pol, bool exact, std::vector<mcp::Tile>& tiles,
+                 std::vector<int32_t>& begin) {
+    const int budget = std::max(1, std::min(pol.nnz_budget, mcp::kNnzTile));
+    const int per = std::max(1, pol.per_lane);
+    tiles.clear();
+    begin.assign(B + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        begin[b] = (int32_t)tiles.size();
+        int r = inst[b];
+        const int re = inst[b + 1];
+        while (r < re) {
+            const int len0 = ptr[r + 1] - ptr[r];
+            if (len0 > budget) {
+                tiles.push_back({r, r + 1, b, mcp::kShapeLong});
+                ++r;
+                continue;
+            }
+            const int r0 = r;
+            int nnz = 0, maxlen = 0, lg = 0;
+            while (r < re) {
+                const int len = ptr[r + 1] - ptr[r];
+                if (len > budget) break;
+                const int nm = std::max(maxlen, len);
+                const int lgn = exact ? 0 : lanes_log2(nm, per);
+                if (nnz + len > budget || (r - r0 + 1) > (mcp::kBlock >> lgn)) break;
+                nnz += len;
+                maxlen = nm;
+                lg = lgn;
+                ++r;
+            }
+            tiles.push_back({r0, r, b, lg});
+        }
+    }
+    begin[B] = (int32_t)tiles.size();
+    // invariants the kernels rely on (LDS capacity, one thread/group per row)
+    for (const auto& t : tiles) {
+        const int nnz = ptr[t.r1] - ptr[t.r0];
+        if (t.shape & mcp::kShapeLong) {
+            if (t.r1 != t.r0 + 1) throw EngineError(MC_ERR_INTERNAL, "bad long tile");
+        } else {
+            const int lg = t.shape & 7;
+            if (t.r1 <= t.r0 || nnz > mcp::kNnzTile || (t.r1 - t.r0) > (mcp::kBlock >> lg) || lg > 6)
+                throw EngineError(MC_ERR_INTERNAL, "tile invariant violated");
+        }
+    }
+}
+
+double ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    return (double)ms;
+}
+
+}  // namespace
+
+struct mcp_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Buf> bufs;
+    mcp::Dev d{};
+    int32_t B = 0, m = 0, n = 0;
+    int64_t nnz = 0;
+    std::vector<int32_t> inst_row, inst_col, rp, cp;
+    std::vector<mcp::Tile> tA, tT;
+    std::vector<int32_t> tA_begin, tT_begin;
+    mcp_tile_policy polA{2048, 4}, polT{2048, 4};
+    bool loaded = false, setup_done = false, exact_tiles = false;
+    int ruiz_passes_req = -1, ruiz_on_req = -1;
+    double h2d_ms = 0.0, setup_ms = 0.0;
+    std::map<std::tuple<int64_t, int, int, int, int>, hipGraphExec_t> graphs;
+    mcp::Dev graph_dev{};  // kernel arguments the cached graphs were captured with
+    int32_t* pinned = nullptr;  // [2 slots x 2 counters]
+    hipEvent_t ev[4]{};
+
+    template <class T>
+    T* buf(const std::string& name, size_t count) {
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        Buf& b = bufs[name];
+        if (b.bytes < bytes) {
+            if (b.p) HIPCHK(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+            HIPCHK(hipMalloc(&b.p, bytes));
+            b.bytes = bytes;
+        }
+        return static_cast<T*>(b.p);
+    }
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    ~mcp_engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (auto& kv : bufs)
+            if (kv.second.p) (void)hipFree(kv.second.p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (pinned) (void)hipHostFree(pinned);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void upload_tiles() {
+        auto* dA = buf<mcp::Tile>("tA", tA.size());
+        auto* dT = buf<mcp::Tile>("tT", tT.size());
+        auto* bA = buf<int32_t>("tA_begin", B + 1);
+        auto* bT = buf<int32_t>("tT_begin", B + 1);
+        if (!tA.empty())
+            HIPCHK(hipMemcpyAsync(dA, tA.data(), tA.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
+        if (!tT.empty())
+            HIPCHK(hipMemcpyAsync(dT, tT.data(), tT.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(bA, tA_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(bT, tT_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        // partial-sum slots follow the tile counts
+        d.part_dxn = buf<double>("part_dxn", tT.size());
+        d.part_adx = buf<double>("part_adx", tA.size());
+        d.part_rows = buf<double>("part_rows", 2 * tA.size());
+        d.part_cols = buf<double>("part_cols", 4 * tT.size());
+        d.tA = dA;
+        d.tT = dT;
+        d.tA_begin = bA;
+        d.tT_begin = bT;
+        d.ntA = (int32_t)tA.size();
+        d.ntT = (int32_t)tT.size();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void retile(bool exact) {
+        build_tiles(rp, inst_row, B, polA, exact, tA, tA_begin);
+        build_tiles(cp, inst_col, B, polT, exact, tT, tT_begin);
+        exact_tiles = exact;
+        upload_tiles();
+    }
+
+    // Time the plain SpMV of one orientation under each candidate policy and
+    // keep the fastest (FP32 values: the mixed-precision iteration's shape).
+    void autotune_one(bool rows) {
+        static const int budgets[] = {512, 1024, 2048};
+        static const int pers[] = {2, 4, 8, 16};
+        const auto& ptr = rows ? rp : cp;
+        const auto& inst = rows ? inst_row : inst_col;
+        const int nrows = rows ? m : n;
+        const int ncols = rows ? n : m;
+        if (nrows == 0) return;
+        float* vin = buf<float>("tune_in", ncols);
+        float* vout = buf<float>("tune_out", nrows);
+        HIPCHK(hipMemsetAsync(vin, 0, sizeof(float) * std::max(ncols, 1), stream));
+        const int32_t* dptr = rows ? d.rp : d.cp;
+        const int32_t* didx = rows ? d.ci : d.ri;
+        const float* dval = rows ? d.av32 : d.tv32;
+        std::vector<mcp::Tile> tl;
+        std::vector<int32_t> tb;
+        double best = std::numeric_limits<double>::infinity();
+        mcp_tile_policy bestp = rows ? polA : polT;
+        for (int bgt : budgets)
+            for (int per : pers) {
+                mcp_tile_policy p{bgt, per};
+                build_tiles(ptr, inst, B, p, false, tl, tb);
+                auto* dt = buf<mcp::Tile>("tune_tiles", tl.size());
+                HIPCHK(hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
+                mcp::launch_spmv_plain(dt, (int)tl.size(), dptr, didx, dval, vin, vout, true, false, stream);
+                HIPCHK(hipEventRecord(ev[2], stream));
+                const int reps = 5;
+                for (int r = 0; r < reps; ++r)
+                    mcp::launch_spmv_plain(dt, (int)tl.size(), dptr, didx, dval, vin, vout, true, false, stream);
+                HIPCHK(hipEventRecord(ev[3], stream));
+                HIPCHK(hipEventSynchronize(ev[3]));
+                const double t = ms_between(ev[2], ev[3]) / reps;
+                if (t < best) {
+                    best = t;
+                    bestp = p;
+                }
+            }
+        (rows ? polA : polT) = bestp;
+    }
+};
+
+namespace {
+
+void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
+    io.primal_tol = o->primal_tolerance;
+    io.dual_tol = o->dual_tolerance;
+    io.gap_tol = o->gap_tolerance;
+    io.restart_factor = o->restart_reduction_factor;
+    io.smoothing = o->primal_weight_smoothing;
+    io.restart_every = (int64_t)o->restart_every;
+    io.max_iterations = (int64_t)o->max_iterations;
+    io.restart_strategy = o->restart_strategy;
+    io.adaptive_weight = o->adaptive_primal_weight;
+    io.ruiz = 1;
+    io.mixed = o->precision == MC_PRECISION_MIXED;
+    const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
+    io.polish_threshold = o->polish_threshold > 0.0 ? o->polish_threshold : 10.0 * tol;
+}
+
+// Kernels of iterations [iter0, iter0 + len): the sequence depends only on
+// iter0 modulo lcm(10, restart_every), so a captured chunk is reusable.
+void enqueue_chunk(mcp_engine* e, const mcp::IterOpts& io, bool adaptive, bool exact, int64_t iter0,
+                   int len, bool use32, bool use64) {
+    const auto& d = e->d;
+    hipStream_t s = e->stream;
+    for (int t = 0; t < len; ++t) {
+        const int64_t it = iter0 + t;
+        const bool adapt = adaptive && (it % 10 == 0);
+        if (use32) mcp::launch_primal(d, t, true, adapt, false, s);
+        if (use64) mcp::launch_primal(d, t, false, adapt, exact, s);
+        if (adapt) {
+            if (use32) mcp::launch_adx(d, t, true, false, s);
+            if (use64) mcp::launch_adx(d, t, false, exact, s);
+            mcp::launch_adapt_finalize(d, exact, s);
+        }
+        if (use32) mcp::launch_dual(d, t, true, false, s);
+        if (use64) mcp::launch_dual(d, t, false, exact, s);
+        if ((it + 1) % io.restart_every == 0) {
+            if (use32) mcp::launch_resid(d, true, false, s);
+            if (use64) mcp::launch_resid(d, false, exact, s);
+            mcp::launch_check_finalize(d, io, t, exact, false, s);
+            mcp::launch_post_check(d, s);
+        }
+    }
+    mcp::launch_advance(d, len, s);
+}
+
+int64_t gcd64(int64_t a, int64_t b) {
+    while (b) {
+        const int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mcp_last_error(void) { return g_err.c_str(); }
+
+int mcp_device_count(int32_t* count) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return MC_OK;
+}
+
+int mcp_engine_create(int32_t device, mcp_engine** out) {
+    *out = nullptr;
+    return guarded([&] {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess || c <= 0)
+            throw EngineError(MC_ERR_DEVICE, "no HIP device available");
+        require(device >= 0 && device < c, "device ordinal out of range");
+        HIPCHK(hipSetDevice(device));
+        auto e = std::make_unique<mcp_engine>();
+        e->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
+        HIPCHK(hipHostMalloc((void**)&e->pinned, 8 * sizeof(int32_t), hipHostMallocDefault));
+        *out = e.release();
+    });
+}
+
+void mcp_engine_destroy(mcp_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    delete e;
+}
+
+int mcp_load(mcp_engine* e, const mcp_problem* p) {
+    return guarded([&] {
+        require(e && p, "null argument");
+        HIPCHK(hipSetDevice(e->device));
+        require(p->instances >= 1 && p->rows >= 0 && p->cols >= 0, "bad problem sizes");
+        require(p->nnz >= 0 && p->nnz < (int64_t)std::numeric_limits<int32_t>::max(), "nnz must be < 2^31");
+        const int B = p->instances, m = p->rows, n = p->cols;
+        const int64_t nnz = p->nnz;
+        require(p->inst_row[0] == 0 && p->inst_row[B] == m && p->inst_col[0] == 0 && p->inst_col[B] == n,
+                "instance tables must span the packed problem");
+        for (int b = 0; b < B; ++b)
+            require(p->inst_row[b] <= p->inst_row[b + 1] && p->inst_col[b] <= p->inst_col[b + 1],
+                    "instance tables must be monotone");
+        require(p->row_ptr[0] == 0 && p->row_ptr[m] == nnz && p->col_ptr[0] == 0 && p->col_ptr[n] == nnz,
+                "row/col pointers must span nnz");
+        for (int i = 0; i < m; ++i) require(p->row_ptr[i] <= p->row_ptr[i + 1], "row_ptr not monotone");
+        for (int j = 0; j < n; ++j) require(p->col_ptr[j] <= p->col_ptr[j + 1], "col_ptr not monotone");
+        for (int64_t k = 0; k < nnz; ++k)
+            require((uint32_t)p->col_idx[k] < (uint32_t)n && (uint32_t)p->row_idx[k] < (uint32_t)m,
+                    "index out of range");
+        e->B = B;
+        e->m = m;
+        e->n = n;
+        e->nnz = nnz;
+        e->inst_row.assign(p->inst_row, p->inst_row + B + 1);
+        e->inst_col.assign(p->inst_col, p->inst_col + B + 1);
+        e->rp.assign(p->row_ptr, p->row_ptr + m + 1);
+        e->cp.assign(p->col_ptr, p->col_ptr + n + 1);
+        std::vector<int32_t> row_inst(m), col_inst(n);
+        for (int b = 0; b < B; ++b) {
+            for (int i = p->inst_row[b]; i < p->inst_row[b + 1]; ++i) row_inst[i] = b;
+            for (int j = p->inst_col[b]; j < p->inst_col[b + 1]; ++j) col_inst[j] = b;
+        }
+        auto& d = e->d;
+        d = mcp::Dev{};
+        d.B = B;
+        d.m = m;
+        d.n = n;
+        hipStream_t s = e->stream;
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        auto up = [&](const std::string& name, const auto* src, size_t count) {
+            using T = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
+            T* dst = e->buf<T>(name, count);
+            if (count) HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+            return dst;
+        };
+        d.rp = up("rp", p->row_ptr, (size_t)m + 1);
+        d.ci = up("ci", p->col_idx, (size_t)nnz);
+        d.av64 = up("av64", p->row_val, (size_t)nnz);
+        d.cp = up("cp", p->col_ptr, (size_t)n + 1);
+        d.ri = up("ri", p->row_idx, (size_t)nnz);
+        d.tv64 = up("tv64", p->col_val, (size_t)nnz);
+        d.inst_row = up("inst_row", p->inst_row, (size_t)B + 1);
+        d.inst_col = up("inst_col", p->inst_col, (size_t)B + 1);
+        d.row_inst = up("row_inst", row_inst.data(), (size_t)m);
+        d.col_inst = up("col_inst", col_inst.data(), (size_t)n);
+        d.orow_lo = up("orow_lo", p->row_lower, (size_t)m);
+        d.orow_hi = up("orow_hi", p->row_upper, (size_t)m);
+        d.ovar_lo = up("ovar_lo", p->var_lower, (size_t)n);
+        d.ovar_hi = up("ovar_hi", p->var_upper, (size_t)n);
+        d.obj = up("obj", p->objective, (size_t)n);
+        d.sense = up("sense", p->sense, (size_t)B);
+        d.offset = up("offset", p->offset, (size_t)B);
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        d.av32 = e->buf<float>("av32", nnz);
+        d.tv32 = e->buf<float>("tv32", nnz);
+        d.row_scale = e->buf<double>("row_scale", m);
+        d.col_scale = e->buf<double>("col_scale", n);
+        d.rtmp = e->buf<double>("rtmp", m);
+        d.ctmp = e->buf<double>("ctmp", n);
+        d.blo = e->buf<double>("blo", m);
+        d.bhi = e->buf<double>("bhi", m);
+        d.rnorm = e->buf<double>("rnorm", m);
+        d.c = e->buf<double>("c", n);
+        d.slo = e->buf<double>("slo", n);
+        d.shi = e->buf<double>("shi", n);
+        d.cnorm = e->buf<double>("cnorm", n);
+        d.rowpk = e->buf<float4>("rowpk", m);
+        d.colpk = e->buf<float4>("colpk", n);
+        d.x = e->buf<double>("x", n);
+        d.xbar = e->buf<double>("xbar", n);
+        d.xavg = e->buf<double>("xavg", n);
+        d.dx = e->buf<double>("dx", n);
+        d.y = e->buf<double>("y", m);
+        d.yavg = e->buf<double>("yavg", m);
+        d.x32 = e->buf<float>("x32", n);
+        d.xbar32 = e->buf<float>("xbar32", n);
+        d.xavg32 = e->buf<float>("xavg32", n);
+        d.dx32 = e->buf<float>("dx32", n);
+        d.y32 = e->buf<float>("y32", m);
+        d.yavg32 = e->buf<float>("yavg32", m);
+        d.ax = e->buf<double>("ax", m);
+        d.aty = e->buf<double>("aty", n);
+        d.st = e->buf<mcp::InstState>("st", B);
+        d.iter_base = e->buf<int64_t>("iter_base", 1);
+        d.counters = e->buf<int32_t>("counters", 4);
+        HIPCHK(hipStreamSynchronize(s));
+        e->h2d_ms = ms_between(e->ev[0], e->ev[1]);
+        e->loaded = true;
+        e->setup_done = false;
+        e->tA.clear();
+        e->tT.clear();
+    });
+}
+
+int mcp_setup(mcp_engine* e, const mcp_setup_opts* o) {
+    return guarded([&] {
+        require(e && o, "null argument");
+        require(e->loaded, "mcp_setup before mcp_load");
+        HIPCHK(hipSetDevice(e->device));
+        if (e->setup_done) {
+            require(e->ruiz_on_req == o->ruiz && e->ruiz_passes_req == o->ruiz_iterations,
+                    "problem already equilibrated with other Ruiz options; reload it");
+            if ((o->exact != 0) != e->exact_tiles) e->retile(o->exact != 0);
+            return;
+        }
+        auto& d = e->d;
+        hipStream_t s = e->stream;
+        std::vector<mcp::InstState> st(e->B);
+        for (auto& x : st) {
+            std::memset(&x, 0, sizeof(x));
+            x.c_scale = 1.0;
+            x.b_norm_inf = 1.0;
+            x.c_norm_inf = 1.0;
+            x.ruiz_done = o->ruiz ? 0 : 1;
+            x.status = mcp::kRunning;
+        }
+        HIPCHK(hipMemcpyAsync(d.st, st.data(), st.size() * sizeof(mcp::InstState), hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        mcp::launch_fill(d.row_scale, 1.0, e->m, s);
+        mcp::launch_fill(d.col_scale, 1.0, e->n, s);
+        if (o->ruiz && e->nnz > 0)
+            for (int k = 0; k < o->ruiz_iterations; ++k) mcp::launch_ruiz_pass(d, 1e-3, s);
+        mcp::launch_setup_data(d, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        HIPCHK(hipStreamSynchronize(s));
+        e->setup_ms = ms_between(e->ev[0], e->ev[1]);
+        e->ruiz_on_req = o->ruiz;
+        e->ruiz_passes_req = o->ruiz_iterations;
+        if (o->autotune && !o->exact) {
+            e->autotune_one(true);
+            e->autotune_one(false);
+        }
+        e->retile(o->exact != 0);
+        e->setup_done = true;
+    });
+}
+
+int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
+    return guarded([&] {
+        require(e && o, "null argument");
+        require(e->setup_done, "mcp_solve before mcp_setup");
+        require(o->restart_every >= 1, "restart_every must be >= 1");
+        const bool exact = o->reduction_order == MC_ORDER_REFERENCE;
+        const bool mixed = o->precision == MC_PRECISION_MIXED;
+        require(!(exact && mixed), "reference reduction order requires FP64 precision");
+        require(!exact || e->exact_tiles, "reference order needs mcp_setup(exact=1)");
+        HIPCHK(hipSetDevice(e->device));
+        auto& d = e->d;
+        hipStream_t s = e->stream;
+        mcp::IterOpts io{};
+        make_opts(o, io);
+        // start point
+        const double* wx = nullptr;
+        const double* wy = nullptr;
+        if (o->warm_primal) {
+            double* t = e->buf<double>("warm_x", e->n);
+            HIPCHK(hipMemcpyAsync(t, o->warm_primal, sizeof(double) * e->n, hipMemcpyHostToDevice, s));
+            wx = t;
+        }
+        if (o->warm_dual) {
+            double* t = e->buf<double>("warm_y", e->m);
+            HIPCHK(hipMemcpyAsync(t, o->warm_dual, sizeof(double) * e->m, hipMemcpyHostToDevice, s));
+            wy = t;
+        }
+        mcp::launch_init_iterates(d, wx, wy, s);
+        mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, mixed, s);
+        int32_t* pin = e->pinned;
+        pin[4] = e->B;
+        pin[5] = mixed ? e->B : 0;
+        HIPCHK(hipMemcpyAsync(d.counters, pin + 4, 2 * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        int64_t zero = 0;
+        HIPCHK(hipMemcpyAsync(d.iter_base, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+
+        const int64_t max_it = (int64_t)o->max_iterations;
+        const int64_t R = io.restart_every;
+        const bool adaptive = o->adaptive_step_size != 0;
+        const int64_t period = adaptive ? (10 / gcd64(10, R)) * R : R;
+        int chunk = o->graph_chunk == 0 ? 64 : std::abs(o->graph_chunk);
+        const bool use_graphs = o->graph_chunk >= 0;
+        int graphs_built = 0, chunks = 0;
+        // Captured chunks bake the kernel arguments (buffer pointers, tile
+        // counts); they stay valid while those are unchanged.
+        if (std::memcmp(&e->graph_dev, &d, sizeof(d)) != 0) {
+            e->drop_graphs();
+            e->graph_dev = d;
+        }
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        int64_t iter = 0;
+        int32_t fp32_running = mixed ? e->B : 0;
+        bool finished = false;
+        while (iter < max_it) {
+            const int len = (int)std::min<int64_t>(chunk, max_it - iter);
+            const bool use32 = mixed && fp32_running > 0;
+            const bool use64 = true;
+            if (use_graphs && len == chunk) {
+                const auto key = std::make_tuple(iter % period, len, (int)use32, (int)use64, (int)exact);
+                auto it = e->graphs.find(key);
+                if (it == e->graphs.end()) {
+                    hipGraph_t g;
+                    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    enqueue_chunk(e, io, adaptive, exact, iter, len, use32, use64);
+                    HIPCHK(hipStreamEndCapture(s, &g));
+                    hipGraphExec_t ex;
+                    HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                    HIPCHK(hipGraphDestroy(g));
+                    it = e->graphs.emplace(key, ex).first;
+                    ++graphs_built;
+                }
+                HIPCHK(hipGraphLaunch(it->second, s));
+            } else {
+                enqueue_chunk(e, io, adaptive, exact, iter, len, use32, use64);
+            }
+            HIPCHK(hipGetLastError());
+            const int slot = chunks & 1;
+            HIPCHK(hipMemcpyAsync(pin + 2 * slot, d.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipEventRecord(e->ev[2 + slot], s));
+            iter += len;
+            ++chunks;
+            if (chunks >= 2) {
+                const int prev = (chunks - 2) & 1;
+                HIPCHK(hipEventSynchronize(e->ev[2 + prev]));
+                if (pin[2 * prev] == 0) {
+                    finished = true;
+                    break;
+                }
+                fp32_running = pin[2 * prev + 1];
+            }
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        const int last = (chunks - 1) & 1;
+        const int32_t still_running = chunks > 0 ? pin[2 * last] : e->B;
+        if (!finished && still_running > 0) {
+            // iteration limit: residuals of the final average (pdlp.cpp:263-265)
+            if (mixed) mcp::launch_resid(d, true, false, s);
+            mcp::launch_resid(d, false, exact, s);
+            mcp::launch_check_finalize(d, io, -1, exact, true, s);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (stats) {
+            stats->kernel_ms = ms_between(e->ev[0], e->ev[1]);
+            stats->setup_ms = e->setup_ms;
+            stats->h2d_ms = e->h2d_ms;
+            stats->iterations = (uint64_t)iter;
+            stats->graphs_built = graphs_built;
+            stats->chunks = chunks;
+        }
+    });
+}
+
+int mcp_fetch(mcp_engine* e, double* x, double* y, mcp_inst_result* res, double* d2h_ms) {
+    return guarded([&] {
+        require(e && e->setup_done, "mcp_fetch before a solve");
+        HIPCHK(hipSetDevice(e->device));
+        auto& d = e->d;
+        hipStream_t s = e->stream;
+        double* xo = e->buf<double>("x_out", e->n);
+        double* yo = e->buf<double>("y_out", e->m);
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        mcp::launch_unscale(d, xo, yo, s);
+        if (x && e->n) HIPCHK(hipMemcpyAsync(x, xo, sizeof(double) * e->n, hipMemcpyDeviceToHost, s));
+        if (y && e->m) HIPCHK(hipMemcpyAsync(y, yo, sizeof(double) * e->m, hipMemcpyDeviceToHost, s));
+        std::vector<mcp::InstState> st(e->B);
+        HIPCHK(hipMemcpyAsync(st.data(), d.st, sizeof(mcp::InstState) * e->B, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (d2h_ms) *d2h_ms = ms_between(e->ev[0], e->ev[1]);
+        if (res)
+            for (int b = 0; b < e->B; ++b) {
+                const auto& q = st[b];
+                auto& r = res[b];
+                r.status = q.status == mcp::kOptimal    ? MC_PDLP_OPTIMAL
+                           : q.status == mcp::kNumFail  ? MC_PDLP_NUMERICAL_FAILURE
+                                                        : MC_PDLP_ITERATION_LIMIT;
+                r.mode = q.mode;
+                r.iterations = (uint64_t)q.done_iter;
+                r.fp32_iterations = (uint64_t)q.fp32_iters;
+                r.objective = q.objective;
+                r.primal_infeasibility = q.primal;
+                r.dual_infeasibility = q.dual;
+                r.duality_gap = q.gap;
+                r.omega = q.omega;
+                r.eta = q.eta;
+            }
+    });
+}
+
+int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order, const double* v,
+             double* out, int32_t reps, double* avg_us) {
+    return guarded([&] {
+        require(e && e->setup_done, "mcp_spmv before mcp_setup");
+        HIPCHK(hipSetDevice(e->device));
+        auto& d = e->d;
+        hipStream_t s = e->stream;
+        const bool t = transpose != 0;
+        const bool f32 = precision == MC_PRECISION_MIXED;
+        const bool exact = order == MC_ORDER_REFERENCE;
+        require(!(exact && f32), "reference order is FP64 only");
+        const int nin = t ? e->m : e->n, nout = t ? e->n : e->m;
+        std::vector<mcp::Tile> tl;
+        std::vector<int32_t> tb;
+        build_tiles(t ? e->cp : e->rp, t ? e->inst_col : e->inst_row, e->B, t ? e->polT : e->polA,
+                    exact, tl, tb);
+        auto* dt = e->buf<mcp::Tile>("spmv_tiles", tl.size());
+        if (!tl.empty())
+            HIPCHK(hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, s));
+        void* din = nullptr;
+        void* dout = nullptr;
+        std::vector<float> vf, of;
+        if (f32) {
+            vf.assign(v, v + nin);
+            din = e->buf<float>("spmv_in32", nin);
+            dout = e->buf<float>("spmv_out32", nout);
+            if (nin) HIPCHK(hipMemcpyAsync(din, vf.data(), sizeof(float) * nin, hipMemcpyHostToDevice, s));
+        } else {
+            din = e->buf<double>("spmv_in", nin);
+            dout = e->buf<double>("spmv_out", nout);
+            if (nin) HIPCHK(hipMemcpyAsync(din, v, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+        }
+        const int32_t* ptr = t ? d.cp : d.rp;
+        const int32_t* idx = t ? d.ri : d.ci;
+        const void* val = f32 ? (const void*)(t ? d.tv32 : d.av32) : (const void*)(t ? d.tv64 : d.av64);
+        mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, exact, s);
+        const int nrep = std::max(1, reps);
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        for (int r = 0; r < nrep; ++r)
+            mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, exact, s);
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        HIPCHK(hipGetLastError());
+        if (f32) {
+            of.resize(nout);
+            if (nout) HIPCHK(hipMemcpyAsync(of.data(), dout, sizeof(float) * nout, hipMemcpyDeviceToHost, s));
+        } else if (nout) {
+            HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        if (f32)
+            for (int k = 0; k < nout; ++k) out[k] = of[k];
+        if (avg_us) *avg_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / nrep;
+    });
+}
+
+int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* col_scale,
+                    int32_t* passes, int32_t* converged) {
+    return guarded([&] {
+        require(e && e->setup_done, "mcp_get_scaling before mcp_setup");
+        HIPCHK(hipSetDevice(e->device));
+        auto& d = e->d;
+        hipStream_t s = e->stream;
+        if (col_val && e->nnz)
+            HIPCHK(hipMemcpyAsync(col_val, d.tv64, sizeof(double) * e->nnz, hipMemcpyDeviceToHost, s));
+        if (row_scale && e->m)
+            HIPCHK(hipMemcpyAsync(row_scale, d.row_scale, sizeof(double) * e->m, hipMemcpyDeviceToHost, s));
+        if (col_scale && e->n)
+            HIPCHK(hipMemcpyAsync(col_scale, d.col_scale, sizeof(double) * e->n, hipMemcpyDeviceToHost, s));
+        mcp::InstState st0;
+        HIPCHK(hipMemcpyAsync(&st0, d.st, sizeof(st0), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (passes) *passes = st0.ruiz_passes;
+        if (converged) *converged = st0.ruiz_converged;
+    });
+}
+
+int mcp_get_tiling(mcp_engine* e, mcp_tile_policy* rows_policy, mcp_tile_policy* cols_policy,
+                   int32_t* row_tiles, int32_t* col_tiles) {
+    return guarded([&] {
+        require(e != nullptr, "null engine");
+        if (rows_policy) *rows_policy = e->polA;
+        if (cols_policy) *cols_policy = e->polT;
+        if (row_tiles) *row_tiles = (int32_t)e->tA.size();
+        if (col_tiles) *col_tiles = (int32_t)e->tT.size();
+    });
+}
+
+}  // extern "C"

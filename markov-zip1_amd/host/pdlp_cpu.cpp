@@ -1,0 +1,269 @@
+// pdlp_cpu.cpp -- Backend::cpu of solve_pdlp: the host PDHG loop
+// (reference src/lp/first_order/pdlp.cpp:139-276) in the reference's
+// arithmetic order, so that both backends of this library and the reference
+// agree on the same inputs.  Also the cpu_fallback path when no GPU exists.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "markov_cero/pdlp.hpp"
+#include "markov_cero/scale.hpp"
+#include "pdlp_internal.hpp"
+
+namespace markov_cero::lp::first_order::detail {
+namespace {
+
+constexpr double kOpen = 1e300;  // bound sentinel of the reference (pdlp.cpp:47-48)
+
+double lower_of(const model::Bound& b) { return b.kind == model::BoundKind::negative_infinity ? -kOpen : b.value; }
+double upper_of(const model::Bound& b) { return b.kind == model::BoundKind::positive_infinity ? kOpen : b.value; }
+
+// Scaled problem data + iterate of one solve.
+struct Pdhg {
+    const model::SparseMatrixCSC& A;
+    std::size_t m, n;
+    std::vector<double> c, lo, hi, blo, bhi, rnorm, cnorm;
+    std::vector<double> x, y, xbar, xavg, yavg, dx, aty, ax;
+    double eta{0.9}, omega{1.0};
+
+    explicit Pdhg(const model::Model& s)
+        : A(s.matrix), m(s.matrix.row_count), n(s.matrix.column_count) {}
+
+    // z = A^T v, per column in increasing row order (pdlp.cpp:34-44).
+    void at_times(const std::vector<double>& v, std::vector<double>& z) const {
+        for (std::size_t j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (std::size_t k = A.column_start[j]; k < A.column_start[j + 1]; ++k) s += A.value[k] * v[A.row_index[k]];
+            z[j] = s;
+        }
+    }
+    // z = A v, column scatter skipping zero entries of v (pdlp.cpp:22-32).
+    void a_times(const std::vector<double>& v, std::vector<double>& z) const {
+        std::fill(z.begin(), z.end(), 0.0);
+        for (std::size_t j = 0; j < n; ++j) {
+            const double vj = v[j];
+            if (vj == 0.0) continue;
+            for (std::size_t k = A.column_start[j]; k < A.column_start[j + 1]; ++k) z[A.row_index[k]] += A.value[k] * vj;
+        }
+    }
+    double tau(std::size_t j) const { return (eta / omega) / cnorm[j]; }
+    double sigma(std::size_t i) const { return (eta * omega) / rnorm[i]; }
+};
+
+struct Residuals {
+    double primal{0}, dual{0}, gap{0}, score{0}, objective{0};
+};
+
+// compute_unscaled_residuals (pdlp.cpp:62-135) on the ORIGINAL model.
+Residuals evaluate(const model::Model& orig, const std::vector<double>& xs, const std::vector<double>& ys,
+                   const std::vector<double>& axs, const std::vector<double>& atys, const scale::RuizScalers& sc,
+                   bool ruiz, std::vector<double>& xu, std::vector<double>& yu) {
+    const std::size_t m = orig.matrix.row_count, n = orig.matrix.column_count;
+    const double sense = orig.objective_sense == model::ObjectiveSense::maximize ? -1.0 : 1.0;
+    xu.resize(n);
+    yu.resize(m);
+    for (std::size_t j = 0; j < n; ++j) xu[j] = ruiz ? xs[j] * sc.col_scale[j] : xs[j];
+    for (std::size_t i = 0; i < m; ++i) yu[i] = ruiz ? ys[i] * sc.row_scale[i] : ys[i];
+    auto rel_violation = [](double v, double lo, double hi) {
+        const double p = std::clamp(v, lo, hi);
+        return std::abs(v - p) / (1.0 + std::max(std::abs(v), std::abs(p) < 1e299 ? std::abs(p) : 0.0));
+    };
+    Residuals r;
+    double worst = 0.0;
+    for (std::size_t i = 0; i < m; ++i) {
+        const double a = ruiz ? axs[i] / sc.row_scale[i] : axs[i];
+        worst = std::max(worst, rel_violation(a, lower_of(orig.row_lower[i]), upper_of(orig.row_upper[i])));
+    }
+    for (std::size_t j = 0; j < n; ++j)
+        worst = std::max(worst, rel_violation(xu[j], lower_of(orig.variable_lower[j]), upper_of(orig.variable_upper[j])));
+    r.primal = worst;
+    std::vector<double> g(n);
+    double cmax = 1.0, sq = 0.0;
+    for (std::size_t j = 0; j < n; ++j) {
+        const double corig = sense * orig.objective[j];
+        cmax = std::max(cmax, std::abs(corig));
+        g[j] = corig + (ruiz ? atys[j] / sc.col_scale[j] : atys[j]);
+        const double d = xu[j] - std::clamp(xu[j] - g[j], lower_of(orig.variable_lower[j]), upper_of(orig.variable_upper[j]));
+        sq += d * d;
+    }
+    r.dual = std::sqrt(sq) / cmax;
+    double pobj = orig.objective_offset;
+    for (std::size_t j = 0; j < n; ++j) pobj += orig.objective[j] * xu[j];
+    r.objective = pobj;
+    double dobj = orig.objective_offset;
+    for (std::size_t i = 0; i < m; ++i) {
+        const double lo = lower_of(orig.row_lower[i]), hi = upper_of(orig.row_upper[i]);
+        if (yu[i] > 0.0 && hi < 1e299)
+            dobj -= yu[i] * hi;
+        else if (yu[i] < 0.0 && lo > -1e299)
+            dobj -= yu[i] * lo;
+    }
+    for (std::size_t j = 0; j < n; ++j) {
+        const double lo = lower_of(orig.variable_lower[j]), hi = upper_of(orig.variable_upper[j]);
+        if (g[j] > 0.0 && lo > -1e299)
+            dobj += g[j] * lo;
+        else if (g[j] < 0.0 && hi < 1e299)
+            dobj += g[j] * hi;
+    }
+    r.gap = std::abs(pobj - dobj) / (1.0 + std::abs(pobj) + std::abs(dobj));
+    r.score = std::max({r.primal, r.dual, r.gap});
+    return r;
+}
+
+}  // namespace
+
+PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const char* backend_label) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::size_t m = orig.matrix.row_count, n = orig.matrix.column_count;
+    PdlpResult res;
+    res.backend_actually_used = backend_label;
+    res.tolerance = opt.primal_tolerance;
+    if (n == 0 || m == 0) {
+        res.status = PdlpStatus::optimal;
+        res.message = "trivial";
+        return res;
+    }
+    model::Model s = orig;
+    scale::RuizScalers sc;
+    if (opt.ruiz_scaling) sc = scale::equilibrate_model(s, {opt.ruiz_iterations});
+    Pdhg p(s);
+    const double sense = s.objective_sense == model::ObjectiveSense::maximize ? -1.0 : 1.0;
+    p.c.resize(n);
+    p.lo.resize(n);
+    p.hi.resize(n);
+    double cinf = 1.0, binf = 1.0;
+    for (std::size_t j = 0; j < n; ++j) {
+        p.c[j] = sense * s.objective[j];
+        cinf = std::max(cinf, std::abs(p.c[j]));
+        p.lo[j] = lower_of(s.variable_lower[j]);
+        p.hi[j] = upper_of(s.variable_upper[j]);
+    }
+    p.blo.resize(m);
+    p.bhi.resize(m);
+    for (std::size_t i = 0; i < m; ++i) {
+        p.blo[i] = lower_of(s.row_lower[i]);
+        p.bhi[i] = upper_of(s.row_upper[i]);
+        if (std::abs(p.blo[i]) < 1e299) binf = std::max(binf, std::abs(p.blo[i]));
+        if (std::abs(p.bhi[i]) < 1e299) binf = std::max(binf, std::abs(p.bhi[i]));
+    }
+    p.rnorm.assign(m, 0.0);
+    p.cnorm.assign(n, 0.0);
+    for (std::size_t j = 0; j < n; ++j)
+        for (std::size_t k = s.matrix.column_start[j]; k < s.matrix.column_start[j + 1]; ++k) {
+            const double a = std::abs(s.matrix.value[k]);
+            p.cnorm[j] += a;
+            p.rnorm[s.matrix.row_index[k]] += a;
+        }
+    for (auto& v : p.cnorm)
+        if (v < 1e-12) v = 1.0;
+    for (auto& v : p.rnorm)
+        if (v < 1e-12) v = 1.0;
+    p.omega = opt.initial_primal_weight > 0.0 ? opt.initial_primal_weight
+                                              : std::clamp(std::sqrt(cinf / binf), 0.01, 100.0);
+    p.eta = std::clamp(opt.step_size_reduction, 0.1, 0.99);
+    p.x.resize(n);
+    for (std::size_t j = 0; j < n; ++j) {
+        const double start = opt.warm_primal.size() == n && opt.ruiz_scaling ? opt.warm_primal[j] / sc.col_scale[j]
+                             : opt.warm_primal.size() == n                 ? opt.warm_primal[j]
+                                                                           : 0.0;
+        p.x[j] = std::clamp(start, p.lo[j], p.hi[j]);
+    }
+    p.y.assign(m, 0.0);
+    if (opt.warm_dual.size() == m)
+        for (std::size_t i = 0; i < m; ++i) p.y[i] = opt.ruiz_scaling ? opt.warm_dual[i] / sc.row_scale[i] : opt.warm_dual[i];
+    p.xbar = p.x;
+    p.xavg = p.x;
+    p.yavg = p.y;
+    p.dx.assign(n, 0.0);
+    p.aty.assign(n, 0.0);
+    p.ax.assign(m, 0.0);
+    std::vector<double> xprev(n), xu, yu;
+
+    std::size_t iter = 0, k = 0, since = 0;
+    double last_score = 1e300;
+    Residuals cur;
+    bool converged = false;
+    while (iter < opt.max_iterations) {
+        p.at_times(p.y, p.aty);
+        xprev = p.x;
+        double dxn = 0.0;
+        for (std::size_t j = 0; j < n; ++j) {
+            const double xn = std::clamp(p.x[j] - p.tau(j) * (p.c[j] + p.aty[j]), p.lo[j], p.hi[j]);
+            const double d = xn - p.x[j];
+            p.dx[j] = d;
+            dxn += p.cnorm[j] * d * d;
+            p.x[j] = xn;
+        }
+        for (std::size_t j = 0; j < n; ++j) p.xbar[j] = 2.0 * p.x[j] - xprev[j];
+        if (opt.adaptive_step_size && iter % 10 == 0 && dxn > 1e-14) {
+            p.a_times(p.dx, p.ax);
+            double adxn = 0.0;
+            for (std::size_t i = 0; i < m; ++i) adxn += (p.ax[i] * p.ax[i]) / p.rnorm[i];
+            if (adxn > 1e-14) {
+                const double L = std::sqrt(adxn / dxn);
+                if (L > 1e-6) {
+                    const double target = 0.95 / L;
+                    if (target < p.eta)
+                        p.eta = std::max(0.1, std::max(target, p.eta * 0.8));
+                    else if (target > 1.05 * p.eta && p.eta < 0.99)
+                        p.eta = std::min(0.99, p.eta * 1.05);
+                }
+            }
+        }
+        p.a_times(p.xbar, p.ax);
+        for (std::size_t i = 0; i < m; ++i) {
+            const double sg = p.sigma(i);
+            const double v = p.y[i] + sg * p.ax[i];
+            p.y[i] = v - sg * std::clamp(v / sg, p.blo[i], p.bhi[i]);
+        }
+        ++k;
+        const double kd = static_cast<double>(k);
+        for (std::size_t j = 0; j < n; ++j) p.xavg[j] += (p.x[j] - p.xavg[j]) / kd;
+        for (std::size_t i = 0; i < m; ++i) p.yavg[i] += (p.y[i] - p.yavg[i]) / kd;
+        ++iter;
+        ++since;
+        if (iter % opt.restart_every != 0) continue;
+        p.a_times(p.xavg, p.ax);
+        p.at_times(p.yavg, p.aty);
+        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, opt.ruiz_scaling, xu, yu);
+        if (cur.primal <= opt.primal_tolerance && cur.dual <= opt.dual_tolerance && cur.gap <= opt.gap_tolerance) {
+            converged = true;
+            break;
+        }
+        const bool restart = opt.restart_strategy == RestartStrategy::fixed ||
+                             (opt.restart_strategy == RestartStrategy::adaptive &&
+                              (cur.score <= opt.restart_reduction_factor * last_score ||
+                               (since >= 5 * opt.restart_every && cur.score < last_score)));
+        if (!restart) continue;
+        if (opt.adaptive_primal_weight && cur.primal > 1e-12 && cur.dual > 1e-12) {
+            const double ratio = std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0);
+            p.omega = std::clamp(p.omega * std::pow(ratio, opt.primal_weight_smoothing), 1e-6, 1e6);
+        }
+        p.x = p.xavg;
+        p.y = p.yavg;
+        p.xbar = p.x;
+        k = 0;
+        since = 0;
+        last_score = cur.score;
+    }
+    if (!converged) {
+        p.a_times(p.xavg, p.ax);
+        p.at_times(p.yavg, p.aty);
+        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, opt.ruiz_scaling, xu, yu);
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    res.status = converged ? PdlpStatus::optimal : PdlpStatus::iteration_limit;
+    res.message = converged ? "PDLP converged" : "iteration limit reached";
+    res.primal = std::move(xu);
+    res.dual = std::move(yu);
+    res.objective = cur.objective;
+    res.primal_infeasibility = cur.primal;
+    res.dual_infeasibility = cur.dual;
+    res.duality_gap = cur.gap;
+    res.iterations = iter;
+    res.kernel_ms = ms;
+    res.total_ms = ms;
+    return res;
+}
+
+}  // namespace markov_cero::lp::first_order::detail

@@ -1,0 +1,272 @@
+// pdlp_gpu.cpp -- Backend::gpu of solve_pdlp: packs one or more LPs for the
+// HIP engine (include/mcpdlp_engine.h), runs it and maps the outcome back to
+// PdlpResult (reference pdlp.hpp:45-63 semantics, backend "hip").
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+
+#include "../../include/mcpdlp_engine.h"
+#include "markov_cero/pdlp.hpp"
+#include "pdlp_internal.hpp"
+
+namespace markov_cero::lp::first_order::detail {
+
+static_assert(sizeof(model::Bound) == sizeof(mc_bound), "Bound layout");
+static_assert(offsetof(model::Bound, value) == offsetof(mc_bound, value), "Bound layout");
+static_assert(sizeof(std::size_t) == sizeof(int64_t), "LP64 index width");
+
+mc_lp view_of(const model::Model& md) {
+    mc_lp v{};
+    v.rows = (int64_t)md.matrix.row_count;
+    v.cols = (int64_t)md.matrix.column_count;
+    v.column_start = reinterpret_cast<const int64_t*>(md.matrix.column_start.data());
+    v.row_index = reinterpret_cast<const int64_t*>(md.matrix.row_index.data());
+    v.value = md.matrix.value.data();
+    v.objective = md.objective.data();
+    v.row_lower = reinterpret_cast<const mc_bound*>(md.row_lower.data());
+    v.row_upper = reinterpret_cast<const mc_bound*>(md.row_upper.data());
+    v.var_lower = reinterpret_cast<const mc_bound*>(md.variable_lower.data());
+    v.var_upper = reinterpret_cast<const mc_bound*>(md.variable_upper.data());
+    v.sense = md.objective_sense == model::ObjectiveSense::maximize ? MC_MAXIMIZE : MC_MINIMIZE;
+    v.objective_offset = md.objective_offset;
+    return v;
+}
+
+void csc_to_csr(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, const double* val, int32_t* rp,
+                int32_t* ci, double* rv, int threads) {
+    const int64_t nnz = n > 0 ? cs[n] : 0;
+    threads = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(1, nnz / (1 << 20))));
+    // counts[t][i]: entries of row i in worker t's column range; worker t's
+    // entries of a row follow those of workers < t, so every row ends up in
+    // increasing column order.
+    std::vector<std::vector<int32_t>> counts(threads, std::vector<int32_t>(m, 0));
+    std::vector<int64_t> cut(threads + 1);
+    for (int t = 0; t <= threads; ++t) cut[t] = n * t / threads;
+    auto run = [&](auto&& fn) {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < threads; ++t) pool.emplace_back(fn, t);
+        fn(0);
+        for (auto& th : pool) th.join();
+    };
+    run([&](int t) {
+        auto& c = counts[t];
+        for (int64_t k = cs[cut[t]]; k < cs[cut[t + 1]]; ++k) ++c[ri[k]];
+    });
+    rp[0] = 0;
+    for (int64_t i = 0; i < m; ++i) {
+        int32_t acc = rp[i];
+        for (int t = 0; t < threads; ++t) {
+            const int32_t c = counts[t][i];
+            counts[t][i] = acc;  // becomes worker t's next slot in row i
+            acc += c;
+        }
+        rp[i + 1] = acc;
+    }
+    run([&](int t) {
+        auto& next = counts[t];
+        for (int64_t j = cut[t]; j < cut[t + 1]; ++j)
+            for (int64_t k = cs[j]; k < cs[j + 1]; ++k) {
+                const int32_t pos = next[ri[k]]++;
+                ci[pos] = (int32_t)j;
+                rv[pos] = val[k];
+            }
+    });
+}
+
+namespace {
+
+struct EngineDeleter {
+    void operator()(mcp_engine* e) const { mcp_engine_destroy(e); }
+};
+
+// One engine per (host thread, device): the engine owns a stream and device
+// buffers that are reused by the next solve of the same thread.
+mcp_engine* engine_for(int device) {
+    thread_local std::map<int, std::unique_ptr<mcp_engine, EngineDeleter>> engines;
+    auto it = engines.find(device);
+    if (it != engines.end()) return it->second.get();
+    int32_t count = 0;
+    mcp_device_count(&count);
+    if (count <= 0) throw DeviceUnavailable("no HIP device available");
+    mcp_engine* e = nullptr;
+    const int rc = mcp_engine_create(device, &e);
+    if (rc == MC_ERR_DEVICE) throw DeviceUnavailable(mcp_last_error());
+    if (rc != MC_OK) throw std::runtime_error(std::string("mcp_engine_create: ") + mcp_last_error());
+    engines.emplace(device, std::unique_ptr<mcp_engine, EngineDeleter>(e));
+    return e;
+}
+
+void check(int rc, const char* what) {
+    if (rc == MC_OK) return;
+    const std::string msg = std::string(what) + ": " + mcp_last_error();
+    if (rc == MC_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+double bound_value(const mc_bound& b, double open) { return b.kind == MC_BOUND_FINITE ? b.value : open; }
+
+}  // namespace
+
+std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptions& opt) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<PdlpResult> out(lps.size());
+    std::vector<int> live;  // instances that go to the device (m > 0 and n > 0)
+    for (std::size_t k = 0; k < lps.size(); ++k) {
+        auto& r = out[k];
+        r.tolerance = opt.primal_tolerance;
+        if (lps[k].rows == 0 || lps[k].cols == 0) {
+            r.status = PdlpStatus::optimal;
+            r.message = "trivial";
+            r.backend_actually_used = "cpu_fallback";  // nothing ran on the device
+        } else {
+            live.push_back((int)k);
+        }
+    }
+    if (live.empty()) return out;
+    mcp_engine* eng = engine_for(opt.device);
+    if (opt.reduction_order == ReductionOrder::reference && opt.precision != Precision::fp64)
+        throw std::invalid_argument("reference reduction order requires FP64 precision");
+
+    // ---- pack (block-diagonal) ----
+    const int B = (int)live.size();
+    int64_t m = 0, n = 0, nnz = 0;
+    for (int k : live) {
+        m += lps[k].rows;
+        n += lps[k].cols;
+        nnz += lps[k].column_start[lps[k].cols];
+    }
+    constexpr int64_t kMax = std::numeric_limits<int32_t>::max();
+    if (m >= kMax || n >= kMax || nnz >= kMax) throw std::invalid_argument("problem exceeds int32 indexing");
+    std::vector<int32_t> inst_row(B + 1), inst_col(B + 1), rp(m + 1), ci(nnz), cp(n + 1), ri(nnz);
+    std::vector<double> rv(nnz), cv(nnz), rlo(m), rhi(m), vlo(n), vhi(n), obj(n), sense(B), offset(B);
+    const double inf = std::numeric_limits<double>::infinity();
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    {
+        int64_t r0 = 0, c0 = 0, p0 = 0;
+        for (int b = 0; b < B; ++b) {
+            const mc_lp& lp = lps[live[b]];
+            inst_row[b] = (int32_t)r0;
+            inst_col[b] = (int32_t)c0;
+            const int64_t lm = lp.rows, ln = lp.cols, lnz = lp.column_start[ln];
+            for (int64_t j = 0; j <= ln; ++j) cp[c0 + j] = (int32_t)(p0 + lp.column_start[j]);
+            for (int64_t k = 0; k < lnz; ++k) ri[p0 + k] = (int32_t)(r0 + lp.row_index[k]);
+            std::memcpy(cv.data() + p0, lp.value, sizeof(double) * lnz);
+            csc_to_csr(lm, ln, lp.column_start, lp.row_index, lp.value, rp.data() + r0, ci.data() + p0,
+                       rv.data() + p0, B == 1 ? hw : 1);
+            for (int64_t i = 0; i <= lm; ++i) rp[r0 + i] += (int32_t)p0;
+            for (int64_t k = 0; k < lnz; ++k) ci[p0 + k] += (int32_t)c0;
+            for (int64_t i = 0; i < lm; ++i) {
+                rlo[r0 + i] = bound_value(lp.row_lower[i], -inf);
+                rhi[r0 + i] = bound_value(lp.row_upper[i], inf);
+            }
+            for (int64_t j = 0; j < ln; ++j) {
+                vlo[c0 + j] = bound_value(lp.var_lower[j], -inf);
+                vhi[c0 + j] = bound_value(lp.var_upper[j], inf);
+            }
+            std::memcpy(obj.data() + c0, lp.objective, sizeof(double) * ln);
+            sense[b] = lp.sense == MC_MAXIMIZE ? -1.0 : 1.0;
+            offset[b] = lp.objective_offset;
+            r0 += lm;
+            c0 += ln;
+            p0 += lnz;
+        }
+        inst_row[B] = (int32_t)r0;
+        inst_col[B] = (int32_t)c0;
+    }
+    mcp_problem pb{};
+    pb.instances = B;
+    pb.rows = (int32_t)m;
+    pb.cols = (int32_t)n;
+    pb.nnz = nnz;
+    pb.inst_row = inst_row.data();
+    pb.inst_col = inst_col.data();
+    pb.row_ptr = rp.data();
+    pb.col_idx = ci.data();
+    pb.row_val = rv.data();
+    pb.col_ptr = cp.data();
+    pb.row_idx = ri.data();
+    pb.col_val = cv.data();
+    pb.row_lower = rlo.data();
+    pb.row_upper = rhi.data();
+    pb.var_lower = vlo.data();
+    pb.var_upper = vhi.data();
+    pb.objective = obj.data();
+    pb.sense = sense.data();
+    pb.offset = offset.data();
+    check(mcp_load(eng, &pb), "mcp_load");
+
+    mcp_setup_opts so{};
+    so.ruiz = opt.ruiz_scaling ? 1 : 0;
+    so.ruiz_iterations = (int32_t)opt.ruiz_iterations;
+    so.exact = opt.reduction_order == ReductionOrder::reference ? 1 : 0;
+    so.autotune = opt.autotune ? 1 : 0;
+    check(mcp_setup(eng, &so), "mcp_setup");
+
+    // warm start (packed)
+    std::vector<double> wx, wy;
+    const bool warm_x = B == 1 && opt.warm_primal.size() == (std::size_t)n;
+    const bool warm_y = B == 1 && opt.warm_dual.size() == (std::size_t)m;
+    mcp_solve_opts o{};
+    o.max_iterations = opt.max_iterations;
+    o.restart_every = opt.restart_every;
+    o.primal_tolerance = opt.primal_tolerance;
+    o.dual_tolerance = opt.dual_tolerance;
+    o.gap_tolerance = opt.gap_tolerance;
+    o.step_size_reduction = opt.step_size_reduction;
+    o.restart_strategy = opt.restart_strategy == RestartStrategy::none    ? MC_RESTART_NONE
+                         : opt.restart_strategy == RestartStrategy::fixed ? MC_RESTART_FIXED
+                                                                          : MC_RESTART_ADAPTIVE;
+    o.adaptive_step_size = opt.adaptive_step_size ? 1 : 0;
+    o.restart_reduction_factor = opt.restart_reduction_factor;
+    o.adaptive_primal_weight = opt.adaptive_primal_weight ? 1 : 0;
+    o.precision = opt.precision == Precision::mixed ? MC_PRECISION_MIXED : MC_PRECISION_FP64;
+    o.initial_primal_weight = opt.initial_primal_weight;
+    o.primal_weight_smoothing = opt.primal_weight_smoothing;
+    o.polish_threshold = opt.polish_threshold;
+    o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
+    o.graph_chunk = opt.graph_chunk;
+    o.warm_primal = warm_x ? opt.warm_primal.data() : nullptr;
+    o.warm_dual = warm_y ? opt.warm_dual.data() : nullptr;
+    mcp_solve_stats st{};
+    check(mcp_solve(eng, &o, &st), "mcp_solve");
+    std::vector<double> x(n), y(m);
+    std::vector<mcp_inst_result> ir(B);
+    double d2h = 0.0;
+    check(mcp_fetch(eng, x.data(), y.data(), ir.data(), &d2h), "mcp_fetch");
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int b = 0; b < B; ++b) {
+        auto& r = out[live[b]];
+        const auto& q = ir[b];
+        r.backend_actually_used = "hip";
+        r.status = q.status == MC_PDLP_OPTIMAL            ? PdlpStatus::optimal
+                   : q.status == MC_PDLP_NUMERICAL_FAILURE ? PdlpStatus::numerical_failure
+                                                           : PdlpStatus::iteration_limit;
+        r.message = r.status == PdlpStatus::optimal           ? "PDLP converged"
+                    : r.status == PdlpStatus::numerical_failure ? "numerical failure (non-finite residual)"
+                                                                : "iteration limit reached";
+        r.primal.assign(x.begin() + inst_col[b], x.begin() + inst_col[b + 1]);
+        r.dual.assign(y.begin() + inst_row[b], y.begin() + inst_row[b + 1]);
+        r.objective = q.objective;
+        r.primal_infeasibility = q.primal_infeasibility;
+        r.dual_infeasibility = q.dual_infeasibility;
+        r.duality_gap = q.duality_gap;
+        r.iterations = (std::size_t)q.iterations;
+        r.fp32_iterations = (std::size_t)q.fp32_iterations;
+        r.h2d_ms = st.h2d_ms;
+        r.setup_ms = st.setup_ms;
+        r.kernel_ms = st.kernel_ms;
+        r.d2h_ms = d2h;
+        r.total_ms = total;
+    }
+    return out;
+}
+
+}  // namespace markov_cero::lp::first_order::detail

@@ -1,0 +1,493 @@
+/*
+ * pdlp_oracle.c -- CPU restatement of markov-cero's PDLP path, in C99.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * liboracle.so.  Nothing in the product links or calls it.
+ *
+ * Every function restates one piece of the reference in the SAME operation
+ * order (serial sums in index order, no FMA contraction: build with
+ * -ffp-contract=off), so that on x86-64 it reproduces the reference bit for
+ * bit.  Pinning (tests/test_oracle.py):
+ *   or_spmv / or_spmv_t      -- dense long-double products on seeded matrices,
+ *                               exact integer matrices, compiled reference
+ *   or_equilibrate_model     -- A_s = D_r A D_c identity, norms within 1e-3 of 1
+ *                               on convergence, one-pass diagonal case,
+ *                               compiled reference scalers (bit-exact)
+ *   or_residuals             -- zero at planted (x*, -y*), hand 2x2 cases,
+ *                               compiled reference
+ *   or_solve_pdlp            -- reference golden fixtures (iterations,
+ *                               objective, residuals, vector hashes), ADR-P3-05
+ *                               known answer, pdlp_test objectives, planted
+ *                               optimum c^T x*, Netlib optima
+ *   or_verify_primal         -- hand cases + compiled reference
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/mcpdlp.h"
+
+#define OR_BIG 1e300
+
+static double or_clamp(double v, double lo, double hi) { /* std::clamp */
+    return (v < lo) ? lo : ((hi < v) ? hi : v);
+}
+static double or_max(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static double or_min(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+static double lo_val(const mc_bound* b) { /* pdlp.cpp:47 */
+    return b->kind == MC_BOUND_NEGATIVE_INFINITY ? -OR_BIG : b->value;
+}
+static double hi_val(const mc_bound* b) { /* pdlp.cpp:48 */
+    return b->kind == MC_BOUND_POSITIVE_INFINITY ? +OR_BIG : b->value;
+}
+
+/* spmv, reference src/lp/first_order/pdlp.cpp:22-32: column scatter, skips
+ * x[col] == 0, accumulates y[row] in increasing column order. */
+void or_spmv(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, const double* v,
+             const double* x, double* y) {
+    for (int64_t i = 0; i < m; ++i) y[i] = 0.0;
+    for (int64_t col = 0; col < n; ++col) {
+        const double xc = x[col];
+        if (xc == 0.0) continue;
+        for (int64_t p = cs[col]; p < cs[col + 1]; ++p) y[ri[p]] += v[p] * xc;
+    }
+}
+
+/* spmv_t, reference pdlp.cpp:34-44: per-column dot in increasing row order. */
+void or_spmv_t(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, const double* v,
+               const double* y, double* z) {
+    (void)m;
+    for (int64_t col = 0; col < n; ++col) {
+        double s = 0.0;
+        for (int64_t p = cs[col]; p < cs[col + 1]; ++p) s += v[p] * y[ri[p]];
+        z[col] = s;
+    }
+}
+
+/* equilibrate_model, reference src/scale/ruiz_scaling.cpp:152-247.
+ * Scales `val` (nnz), the four bound vectors and the objective in place and
+ * writes row_scale[m], col_scale[n].  Returns passes executed; *converged
+ * set as RuizScalers::converged. */
+int64_t or_equilibrate_model(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri,
+                             double* val, double* objective, mc_bound* row_lower,
+                             mc_bound* row_upper, mc_bound* var_lower, mc_bound* var_upper,
+                             int64_t max_iterations, double tolerance, double* row_scale,
+                             double* col_scale, int32_t* converged) {
+    *converged = 0;
+    for (int64_t i = 0; i < m; ++i) row_scale[i] = 1.0;
+    for (int64_t j = 0; j < n; ++j) col_scale[j] = 1.0;
+    const int64_t nnz = (n > 0) ? cs[n] : 0;
+    if (m == 0 || n == 0 || nnz == 0) return 0;
+    double* r_norm = (double*)malloc(sizeof(double) * (size_t)m);
+    double* c_norm = (double*)malloc(sizeof(double) * (size_t)n);
+    double* d_r = (double*)malloc(sizeof(double) * (size_t)m);
+    double* d_c = (double*)malloc(sizeof(double) * (size_t)n);
+    int64_t iter = 0;
+    for (; iter < max_iterations; ++iter) {
+        for (int64_t i = 0; i < m; ++i) r_norm[i] = 0.0;
+        for (int64_t j = 0; j < n; ++j) c_norm[j] = 0.0;
+        for (int64_t col = 0; col < n; ++col)
+            for (int64_t p = cs[col]; p < cs[col + 1]; ++p) {
+                const double a = fabs(val[p]);
+                c_norm[col] = or_max(c_norm[col], a);
+                r_norm[ri[p]] = or_max(r_norm[ri[p]], a);
+            }
+        for (int64_t i = 0; i < m; ++i)
+            if (r_norm[i] == 0.0) r_norm[i] = 1.0;
+        for (int64_t j = 0; j < n; ++j)
+            if (c_norm[j] == 0.0) c_norm[j] = 1.0;
+        double max_r_err = 0.0, max_c_err = 0.0;
+        for (int64_t i = 0; i < m; ++i) max_r_err = or_max(max_r_err, fabs(1.0 - r_norm[i]));
+        for (int64_t j = 0; j < n; ++j) max_c_err = or_max(max_c_err, fabs(1.0 - c_norm[j]));
+        if (max_r_err < tolerance && max_c_err < tolerance) {
+            *converged = 1;
+            break;
+        }
+        for (int64_t i = 0; i < m; ++i) {
+            d_r[i] = 1.0 / sqrt(r_norm[i]);
+            row_scale[i] *= d_r[i];
+        }
+        for (int64_t j = 0; j < n; ++j) {
+            d_c[j] = 1.0 / sqrt(c_norm[j]);
+            col_scale[j] *= d_c[j];
+        }
+        for (int64_t col = 0; col < n; ++col) {
+            const double dc = d_c[col];
+            for (int64_t p = cs[col]; p < cs[col + 1]; ++p) val[p] *= d_r[ri[p]] * dc;
+        }
+    }
+    /* ruiz_scaling.cpp:221-244: bounds by D_r / D_c^{-1}, objective by D_c. */
+    for (int64_t i = 0; i < m; ++i) {
+        if (row_lower[i].kind == MC_BOUND_FINITE) row_lower[i].value *= row_scale[i];
+        if (row_upper[i].kind == MC_BOUND_FINITE) row_upper[i].value *= row_scale[i];
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        const double inv = 1.0 / col_scale[j];
+        if (var_lower[j].kind == MC_BOUND_FINITE) var_lower[j].value *= inv;
+        if (var_upper[j].kind == MC_BOUND_FINITE) var_upper[j].value *= inv;
+        objective[j] *= col_scale[j];
+    }
+    free(r_norm);
+    free(c_norm);
+    free(d_r);
+    free(d_c);
+    return iter;
+}
+
+typedef struct or_resid {
+    double primal, dual, gap, score, objective;
+} or_resid;
+
+/* compute_unscaled_residuals, reference pdlp.cpp:62-135, on the ORIGINAL
+ * model.  x_out[n], y_out[m] receive the unscaled average iterate. */
+void or_residuals(const mc_lp* lp, const double* x_avg, const double* y_avg,
+                  const double* Ax_avg, const double* At_y_avg, const double* row_scale,
+                  const double* col_scale, int32_t ruiz, double* x_out, double* y_out,
+                  or_resid* r) {
+    const int64_t m = lp->rows, n = lp->cols;
+    const double obj_sign = (lp->sense == MC_MAXIMIZE) ? -1.0 : 1.0;
+    for (int64_t j = 0; j < n; ++j) x_out[j] = ruiz ? (x_avg[j] * col_scale[j]) : x_avg[j];
+    for (int64_t i = 0; i < m; ++i) y_out[i] = ruiz ? (y_avg[i] * row_scale[i]) : y_avg[i];
+    double max_prim = 0.0;
+    for (int64_t i = 0; i < m; ++i) {
+        const double ax = ruiz ? (Ax_avg[i] / row_scale[i]) : Ax_avg[i];
+        const double lo = lo_val(&lp->row_lower[i]), hi = hi_val(&lp->row_upper[i]);
+        const double proj = or_clamp(ax, lo, hi);
+        const double res = fabs(ax - proj);
+        const double sc = 1.0 + or_max(fabs(ax), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
+        max_prim = or_max(max_prim, res / sc);
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        const double lo = lo_val(&lp->var_lower[j]), hi = hi_val(&lp->var_upper[j]);
+        const double proj = or_clamp(x_out[j], lo, hi);
+        const double res = fabs(x_out[j] - proj);
+        const double sc = 1.0 + or_max(fabs(x_out[j]), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
+        max_prim = or_max(max_prim, res / sc);
+    }
+    r->primal = max_prim;
+    double c_scale = 1.0, dual_sq = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double aty = ruiz ? (At_y_avg[j] / col_scale[j]) : At_y_avg[j];
+        const double c_orig = obj_sign * lp->objective[j];
+        c_scale = or_max(c_scale, fabs(c_orig));
+        const double g = c_orig + aty;
+        const double lo = lo_val(&lp->var_lower[j]), hi = hi_val(&lp->var_upper[j]);
+        const double xp = or_clamp(x_out[j] - g, lo, hi);
+        const double d = x_out[j] - xp;
+        dual_sq += d * d;
+    }
+    r->dual = sqrt(dual_sq) / c_scale;
+    double pobj = lp->objective_offset;
+    for (int64_t j = 0; j < n; ++j) pobj += lp->objective[j] * x_out[j];
+    r->objective = pobj;
+    double dobj = lp->objective_offset;
+    for (int64_t i = 0; i < m; ++i) {
+        const double lo = lo_val(&lp->row_lower[i]), hi = hi_val(&lp->row_upper[i]);
+        if (y_out[i] > 0.0 && hi < 1e299)
+            dobj -= y_out[i] * hi;
+        else if (y_out[i] < 0.0 && lo > -1e299)
+            dobj -= y_out[i] * lo;
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        const double aty = ruiz ? (At_y_avg[j] / col_scale[j]) : At_y_avg[j];
+        const double c_orig = obj_sign * lp->objective[j];
+        const double g = c_orig + aty;
+        const double lo = lo_val(&lp->var_lower[j]), hi = hi_val(&lp->var_upper[j]);
+        if (g > 0.0 && lo > -1e299)
+            dobj += g * lo;
+        else if (g < 0.0 && hi < 1e299)
+            dobj += g * hi;
+    }
+    r->gap = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+    /* std::max({p, d, g}) */
+    double s = r->primal;
+    if (s < r->dual) s = r->dual;
+    if (s < r->gap) s = r->gap;
+    r->score = s;
+}
+
+/* Trace of the scalar state at every check, for trajectory comparisons. */
+typedef struct or_trace {
+    int64_t capacity;
+    int64_t count;
+    double* score;   /* [capacity] score at each check            */
+    double* omega;   /* [capacity] omega after the check          */
+    double* eta;     /* [capacity] eta at the check               */
+    int32_t* restart;/* [capacity] 1 if the check restarted       */
+} or_trace;
+
+/* solve_pdlp, reference pdlp.cpp:139-276.  `res->primal`/`res->dual` must hold
+ * n / m doubles (or be NULL).  Timings are wall-clock of this restatement.
+ * Optional `trace` records the per-check scalar sequence. */
+int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res,
+                  or_trace* trace) {
+    const int64_t m = lp->rows, n = lp->cols;
+    const int requested_gpu = (o->backend == MC_BACKEND_GPU);
+    memset(res->message, 0, sizeof(res->message));
+    memset(res->backend_actually_used, 0, sizeof(res->backend_actually_used));
+    strcpy(res->backend_actually_used, requested_gpu ? "cpu_fallback" : "cpu");
+    res->h2d_ms = res->d2h_ms = 0.0;
+    res->kernel_ms = res->total_ms = 0.0;
+    res->tolerance = o->primal_tolerance;
+    if (n == 0 || m == 0) { /* pdlp.cpp:144-149 */
+        res->status = MC_PDLP_OPTIMAL;
+        res->objective = res->primal_infeasibility = res->dual_infeasibility = 0.0;
+        res->duality_gap = 0.0;
+        res->iterations = 0;
+        res->primal_len = res->dual_len = 0;
+        strcpy(res->message, "trivial");
+        return 0;
+    }
+    const int64_t nnz = lp->column_start[n];
+    /* model copy (pdlp.cpp:150) */
+    double* val = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    double* obj = (double*)malloc(sizeof(double) * (size_t)n);
+    mc_bound* rl = (mc_bound*)malloc(sizeof(mc_bound) * (size_t)m);
+    mc_bound* ru = (mc_bound*)malloc(sizeof(mc_bound) * (size_t)m);
+    mc_bound* vl = (mc_bound*)malloc(sizeof(mc_bound) * (size_t)n);
+    mc_bound* vu = (mc_bound*)malloc(sizeof(mc_bound) * (size_t)n);
+    memcpy(val, lp->value, sizeof(double) * (size_t)nnz);
+    memcpy(obj, lp->objective, sizeof(double) * (size_t)n);
+    memcpy(rl, lp->row_lower, sizeof(mc_bound) * (size_t)m);
+    memcpy(ru, lp->row_upper, sizeof(mc_bound) * (size_t)m);
+    memcpy(vl, lp->var_lower, sizeof(mc_bound) * (size_t)n);
+    memcpy(vu, lp->var_upper, sizeof(mc_bound) * (size_t)n);
+    double* row_scale = (double*)malloc(sizeof(double) * (size_t)m);
+    double* col_scale = (double*)malloc(sizeof(double) * (size_t)n);
+    for (int64_t i = 0; i < m; ++i) row_scale[i] = 1.0;
+    for (int64_t j = 0; j < n; ++j) col_scale[j] = 1.0;
+    int32_t conv = 0;
+    if (o->ruiz_scaling)
+        or_equilibrate_model(m, n, lp->column_start, lp->row_index, val, obj, rl, ru, vl, vu,
+                             (int64_t)o->ruiz_iterations, 1e-3, row_scale, col_scale, &conv);
+    const int64_t* cs = lp->column_start;
+    const int64_t* ri = lp->row_index;
+    const double obj_sign = (lp->sense == MC_MAXIMIZE) ? -1.0 : 1.0;
+    double* c = (double*)malloc(sizeof(double) * (size_t)n);
+    double c_norm_inf = 1.0;
+    for (int64_t j = 0; j < n; ++j) {
+        c[j] = obj_sign * obj[j];
+        c_norm_inf = or_max(c_norm_inf, fabs(c[j]));
+    }
+    double* b_lo = (double*)malloc(sizeof(double) * (size_t)m);
+    double* b_hi = (double*)malloc(sizeof(double) * (size_t)m);
+    double b_norm_inf = 1.0;
+    for (int64_t i = 0; i < m; ++i) {
+        b_lo[i] = lo_val(&rl[i]);
+        b_hi[i] = hi_val(&ru[i]);
+        if (fabs(b_lo[i]) < 1e299) b_norm_inf = or_max(b_norm_inf, fabs(b_lo[i]));
+        if (fabs(b_hi[i]) < 1e299) b_norm_inf = or_max(b_norm_inf, fabs(b_hi[i]));
+    }
+    double* row_norms = (double*)calloc((size_t)m, sizeof(double));
+    double* col_norms = (double*)calloc((size_t)n, sizeof(double));
+    for (int64_t col = 0; col < n; ++col)
+        for (int64_t p = cs[col]; p < cs[col + 1]; ++p) {
+            const double a = fabs(val[p]);
+            col_norms[col] += a;
+            row_norms[ri[p]] += a;
+        }
+    for (int64_t j = 0; j < n; ++j)
+        if (col_norms[j] < 1e-12) col_norms[j] = 1.0;
+    for (int64_t i = 0; i < m; ++i)
+        if (row_norms[i] < 1e-12) row_norms[i] = 1.0;
+    double omega = (o->initial_primal_weight > 0.0)
+                       ? o->initial_primal_weight
+                       : or_clamp(sqrt(c_norm_inf / b_norm_inf), 0.01, 100.0);
+    double eta = or_clamp(o->step_size_reduction, 0.1, 0.99);
+    double* tau = (double*)malloc(sizeof(double) * (size_t)n);
+    double* sigma = (double*)malloc(sizeof(double) * (size_t)m);
+#define OR_UPDATE_STEPS()                                                         \
+    do {                                                                          \
+        for (int64_t j_ = 0; j_ < n; ++j_) tau[j_] = (eta / omega) / col_norms[j_]; \
+        for (int64_t i_ = 0; i_ < m; ++i_) sigma[i_] = (eta * omega) / row_norms[i_]; \
+    } while (0)
+    OR_UPDATE_STEPS();
+    double* x = (double*)calloc((size_t)n, sizeof(double));
+    double* y = (double*)calloc((size_t)m, sizeof(double));
+    double* x_bar = (double*)calloc((size_t)n, sizeof(double));
+    double* x_avg = (double*)calloc((size_t)n, sizeof(double));
+    double* y_avg = (double*)calloc((size_t)m, sizeof(double));
+    double* dx = (double*)calloc((size_t)n, sizeof(double));
+    double* x_prev = (double*)malloc(sizeof(double) * (size_t)n);
+    double* At_y = (double*)malloc(sizeof(double) * (size_t)n);
+    double* Ax = (double*)malloc(sizeof(double) * (size_t)m);
+    double* xo = (double*)malloc(sizeof(double) * (size_t)n);
+    double* yo = (double*)malloc(sizeof(double) * (size_t)m);
+    for (int64_t j = 0; j < n; ++j) {
+        x[j] = or_clamp(0.0, lo_val(&vl[j]), hi_val(&vu[j]));
+        x_bar[j] = x[j];
+        x_avg[j] = x[j];
+    }
+    uint64_t iter = 0, avg_count = 0, since = 0;
+    double last_restart_score = 1e300;
+    or_resid cur;
+    int optimal = 0;
+    while (iter < o->max_iterations) {
+        or_spmv_t(m, n, cs, ri, val, y, At_y); /* :195 */
+        memcpy(x_prev, x, sizeof(double) * (size_t)n);
+        double dxn = 0.0;
+        for (int64_t j = 0; j < n; ++j) { /* :198-205 */
+            const double g = c[j] + At_y[j];
+            const double px = or_clamp(x[j] - tau[j] * g, lo_val(&vl[j]), hi_val(&vu[j]));
+            const double d = px - x[j];
+            dx[j] = d;
+            dxn += col_norms[j] * d * d;
+            x[j] = px;
+        }
+        for (int64_t j = 0; j < n; ++j) x_bar[j] = 2.0 * x[j] - x_prev[j];
+        if (o->adaptive_step_size && iter % 10 == 0 && dxn > 1e-14) { /* :207-219 */
+            or_spmv(m, n, cs, ri, val, dx, Ax);
+            double adxn = 0.0;
+            for (int64_t i = 0; i < m; ++i) adxn += (Ax[i] * Ax[i]) / row_norms[i];
+            if (adxn > 1e-14) {
+                const double L = sqrt(adxn / dxn);
+                if (L > 1e-6) {
+                    const double target = 0.95 / L;
+                    if (target < eta) {
+                        eta = or_max(0.1, or_max(target, eta * 0.8));
+                        OR_UPDATE_STEPS();
+                    } else if (target > 1.05 * eta && eta < 0.99) {
+                        eta = or_min(0.99, eta * 1.05);
+                        OR_UPDATE_STEPS();
+                    }
+                }
+            }
+        }
+        or_spmv(m, n, cs, ri, val, x_bar, Ax); /* :220-224 */
+        for (int64_t i = 0; i < m; ++i) {
+            const double v = y[i] + sigma[i] * Ax[i];
+            y[i] = v - sigma[i] * or_clamp(v / sigma[i], b_lo[i], b_hi[i]);
+        }
+        ++avg_count; /* :225-227 */
+        for (int64_t j = 0; j < n; ++j) x_avg[j] += (x[j] - x_avg[j]) / (double)avg_count;
+        for (int64_t i = 0; i < m; ++i) y_avg[i] += (y[i] - y_avg[i]) / (double)avg_count;
+        ++iter;
+        ++since;
+        if (iter % o->restart_every == 0) { /* :229-261 */
+            or_spmv(m, n, cs, ri, val, x_avg, Ax);
+            or_spmv_t(m, n, cs, ri, val, y_avg, At_y);
+            or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, o->ruiz_scaling, xo,
+                         yo, &cur);
+            int restarted = 0;
+            if (cur.primal <= o->primal_tolerance && cur.dual <= o->dual_tolerance &&
+                cur.gap <= o->gap_tolerance) {
+                optimal = 1;
+            } else {
+                const double score = cur.score;
+                const int do_restart =
+                    (o->restart_strategy == MC_RESTART_FIXED) ||
+                    (o->restart_strategy == MC_RESTART_ADAPTIVE &&
+                     (score <= o->restart_reduction_factor * last_restart_score ||
+                      (since >= 5 * o->restart_every && score < last_restart_score)));
+                if (do_restart) {
+                    if (o->adaptive_primal_weight && cur.primal > 1e-12 && cur.dual > 1e-12) {
+                        const double ratio = or_clamp(sqrt(cur.primal / cur.dual), 0.05, 20.0);
+                        omega = or_clamp(omega * pow(ratio, o->primal_weight_smoothing), 1e-6, 1e6);
+                        OR_UPDATE_STEPS();
+                    }
+                    memcpy(x, x_avg, sizeof(double) * (size_t)n);
+                    memcpy(y, y_avg, sizeof(double) * (size_t)m);
+                    for (int64_t j = 0; j < n; ++j) x_bar[j] = x[j];
+                    avg_count = 0;
+                    since = 0;
+                    last_restart_score = score;
+                    restarted = 1;
+                }
+            }
+            if (trace && trace->count < trace->capacity) {
+                trace->score[trace->count] = cur.score;
+                trace->omega[trace->count] = omega;
+                trace->eta[trace->count] = eta;
+                trace->restart[trace->count] = restarted;
+                trace->count++;
+            }
+            if (optimal) break;
+        }
+    }
+    if (!optimal) { /* :263-265 */
+        or_spmv(m, n, cs, ri, val, x_avg, Ax);
+        or_spmv_t(m, n, cs, ri, val, y_avg, At_y);
+        or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, o->ruiz_scaling, xo, yo,
+                     &cur);
+    }
+    res->status = optimal ? MC_PDLP_OPTIMAL : MC_PDLP_ITERATION_LIMIT;
+    strcpy(res->message, optimal ? "PDLP converged" : "iteration limit reached");
+    res->objective = cur.objective;
+    res->primal_infeasibility = cur.primal;
+    res->dual_infeasibility = cur.dual;
+    res->duality_gap = cur.gap;
+    res->iterations = iter;
+    res->primal_len = n;
+    res->dual_len = m;
+    if (res->primal) memcpy(res->primal, xo, sizeof(double) * (size_t)n);
+    if (res->dual) memcpy(res->dual, yo, sizeof(double) * (size_t)m);
+    free(val); free(obj); free(rl); free(ru); free(vl); free(vu);
+    free(row_scale); free(col_scale); free(c); free(b_lo); free(b_hi);
+    free(row_norms); free(col_norms); free(tau); free(sigma);
+    free(x); free(y); free(x_bar); free(x_avg); free(y_avg); free(dx); free(x_prev);
+    free(At_y); free(Ax); free(xo); free(yo);
+#undef OR_UPDATE_STEPS
+    return 0;
+}
+
+/* verify_primal for continuous variables, reference
+ * src/verify/primal_verifier.cpp:9-104 (allowance, check_lower/upper, long
+ * double row activity and objective). */
+static double or_allowance(double actual, double expected, double a, double r) {
+    double mx = 1.0;
+    if (mx < fabs(actual)) mx = fabs(actual);
+    if (mx < fabs(expected)) mx = fabs(expected);
+    return a + r * mx;
+}
+
+int or_verify_primal(const mc_lp* lp, const double* x, double claimed, double fa, double fr,
+                     double oa, double orel, mc_verify_report* rep) {
+    memset(rep, 0, sizeof(*rep));
+    const int64_t m = lp->rows, n = lp->cols;
+    int32_t viol = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        if (lp->var_lower[j].kind == MC_BOUND_FINITE) {
+            const double b = lp->var_lower[j].value;
+            const double mag = or_max(0.0, b - x[j]);
+            rep->maximum_variable_violation = or_max(rep->maximum_variable_violation, mag);
+            if (mag > or_allowance(x[j], b, fa, fr)) viol++;
+        }
+        if (lp->var_upper[j].kind == MC_BOUND_FINITE) {
+            const double b = lp->var_upper[j].value;
+            const double mag = or_max(0.0, x[j] - b);
+            rep->maximum_variable_violation = or_max(rep->maximum_variable_violation, mag);
+            if (mag > or_allowance(x[j], b, fa, fr)) viol++;
+        }
+    }
+    long double* work = (long double*)calloc((size_t)(m > 0 ? m : 1), sizeof(long double));
+    for (int64_t col = 0; col < n; ++col)
+        for (int64_t p = lp->column_start[col]; p < lp->column_start[col + 1]; ++p)
+            work[lp->row_index[p]] += (long double)lp->value[p] * x[col];
+    for (int64_t i = 0; i < m; ++i) {
+        const double act = (double)work[i];
+        if (lp->row_lower[i].kind == MC_BOUND_FINITE) {
+            const double b = lp->row_lower[i].value;
+            const double mag = or_max(0.0, b - act);
+            rep->maximum_row_violation = or_max(rep->maximum_row_violation, mag);
+            if (mag > or_allowance(act, b, fa, fr)) viol++;
+        }
+        if (lp->row_upper[i].kind == MC_BOUND_FINITE) {
+            const double b = lp->row_upper[i].value;
+            const double mag = or_max(0.0, act - b);
+            rep->maximum_row_violation = or_max(rep->maximum_row_violation, mag);
+            if (mag > or_allowance(act, b, fa, fr)) viol++;
+        }
+    }
+    free(work);
+    long double objv = lp->objective_offset;
+    for (int64_t j = 0; j < n; ++j) objv += (long double)lp->objective[j] * x[j];
+    rep->recomputed_objective = (double)objv;
+    rep->objective_difference = fabs(rep->recomputed_objective - claimed);
+    if (rep->objective_difference > or_allowance(rep->recomputed_objective, claimed, oa, orel)) viol++;
+    rep->violation_count = viol;
+    rep->passed = (viol == 0);
+    return 0;
+}

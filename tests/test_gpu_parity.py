@@ -1,0 +1,172 @@
+"""GPU parity of the HIP engine against the oracle (CPU restatement of the
+reference PDLP path).  Run on an MI355X: ``pytest -m gpu``.
+
+Bars:
+* reference reduction order (MC_ORDER_REFERENCE, FP64): bit-identical to the
+  oracle -- same iteration count, same objective bits, same primal/dual bits;
+* fast order (FP64): same status, objective within 1e-9 relative, residuals
+  within 1e-6 relative (only the summation order of the reductions differs);
+* SpMV: reference order bit-identical, fast order within 1e-13 relative
+  (FP64) / 2e-6 relative to the row's absolute sum (FP32).
+"""
+import numpy as np
+import pytest
+
+import mcpdlp as mp
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def tiny_simplex():  # examples/phase3/tiny_simplex.mps; optimum -9 at (1, 3)
+    inf = np.inf
+    return mp.LPData.from_dense([[1, 1], [2, 1]], [-3, -2], [-inf, -inf], [4, 5], [0, 0], [inf, inf])
+
+
+def blend():  # tests/pdlp_test.cpp:12-48; optimum -7 at (1, 3)
+    inf = np.inf
+    return mp.LPData.from_dense([[1, 1], [1, 0], [0, 1]], [-1, -2], [-inf] * 3, [4, 3, 3], [0, 0], [10, 10])
+
+
+def equality():  # tests/pdlp_test.cpp:82-109; optimum 5
+    return mp.LPData.from_dense([[1, 1]], [1, 1], [5], [5], [0, 0], [5, 5])
+
+
+def small_uniform(seed=11, m=200, n=500):
+    return mp.generate(mp.FAMILY_UNIFORM_EQ, rows=m, cols=n, seed=seed)
+
+
+def small_mixed(seed=12, m=300, n=700):
+    return mp.generate(mp.FAMILY_ZIPF_MIXED, rows=m, cols=n, nnz_max=40, seed=seed)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    assert mp.device_count() >= 1, "no HIP device visible"
+    return True
+
+
+def gpu_solve(lp, **kw):
+    kw.setdefault("backend", mp.BACKEND_GPU)
+    kw.setdefault("require_gpu", 1)
+    r = mp.solve(lp, **kw)
+    assert r["backend_actually_used"] == "hip", r["backend_actually_used"]
+    return r
+
+
+def test_engine_visible(gpu):
+    e = mp.Engine(0)
+    e.close()
+
+
+@pytest.mark.parametrize("maker", [small_uniform, small_mixed])
+def test_spmv_parity(gpu, maker):
+    lp = maker()
+    print("spmv", maker.__name__, lp.rows, lp.cols, lp.nnz, flush=True)
+    e = mp.Engine(0)
+    e.load(lp)
+    e.setup(ruiz=False)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(lp.cols)
+    x[rng.random(lp.cols) < 0.2] = 0.0
+    y = rng.standard_normal(lp.rows)
+    ax, at = orc.spmv(lp, x), orc.spmv_t(lp, y)
+    g, _ = e.spmv(x, exact=True)
+    gt, _ = e.spmv(y, transpose=True, exact=True)
+    assert np.array_equal(g, ax) and np.array_equal(gt, at)
+    f, _ = e.spmv(x)
+    ft, _ = e.spmv(y, transpose=True)
+    np.testing.assert_allclose(f, ax, rtol=1e-13, atol=1e-13 * np.abs(ax).max())
+    np.testing.assert_allclose(ft, at, rtol=1e-13, atol=1e-13 * np.abs(at).max())
+    A = np.abs(lp.dense())
+    bound = A @ np.abs(x) * 2e-6 + 1e-30
+    s32, _ = e.spmv(x, fp32=True)
+    assert np.all(np.abs(s32 - ax) <= bound + 1e-6 * np.abs(ax))
+    e.close()
+
+
+@pytest.mark.parametrize("maker", [small_uniform, small_mixed])
+def test_device_ruiz_bit_exact(gpu, maker):
+    lp = maker()
+    e = mp.Engine(0)
+    e.load(lp)
+    e.setup(ruiz=True, ruiz_iterations=10)
+    vals, rs, cs, passes, conv = e.scaling(lp.nnz)
+    ref = orc.equilibrate(lp, 10)
+    assert np.array_equal(vals, ref["lp"].value)
+    assert np.array_equal(rs, ref["row_scale"]) and np.array_equal(cs, ref["col_scale"])
+    assert passes == ref["passes"] and bool(conv) == ref["converged"]
+    # the SpMV of the scaled matrix equals the oracle's on the oracle-scaled matrix
+    x = np.linspace(-1, 1, lp.cols)
+    g, _ = e.spmv(x, exact=True)
+    assert np.array_equal(g, orc.spmv(ref["lp"], x))
+    e.close()
+
+
+CASES = [
+    ("tiny_simplex@1e-6", tiny_simplex, dict(tolerance=1e-6)),
+    ("blend@1e-4", blend, dict(tolerance=1e-4, max_iterations=200000)),
+    ("equality@1e-4", equality, dict(tolerance=1e-4, max_iterations=200000)),
+    ("uniform200x500@400it", small_uniform, dict(tolerance=1e-6, max_iterations=400)),
+    ("mixed300x700@600it", small_mixed, dict(tolerance=1e-6, max_iterations=600)),
+]
+
+
+@pytest.mark.parametrize("name,maker,kw", CASES, ids=[c[0] for c in CASES])
+def test_reference_order_bit_exact(gpu, name, maker, kw):
+    lp = maker()
+    print("case", name, flush=True)
+    want = orc.solve(lp, **kw)
+    got = gpu_solve(lp, reduction_order=mp.ORDER_REFERENCE, **kw)
+    assert got["status"] == want["status"]
+    assert got["iterations"] == want["iterations"]
+    assert got["objective"] == want["objective"]
+    assert np.array_equal(got["primal"], want["primal"])
+    assert np.array_equal(got["dual"], want["dual"])
+    assert got["primal_infeasibility"] == want["primal_infeasibility"]
+    assert got["dual_infeasibility"] == want["dual_infeasibility"]
+    assert got["duality_gap"] == want["duality_gap"]
+
+
+@pytest.mark.parametrize("name,maker,kw", CASES, ids=[c[0] for c in CASES])
+def test_fast_order_close(gpu, name, maker, kw):
+    lp = maker()
+    want = orc.solve(lp, **kw)
+    got = gpu_solve(lp, **kw)
+    assert got["status"] == want["status"]
+    assert abs(got["iterations"] - want["iterations"]) <= 80
+    assert abs(got["objective"] - want["objective"]) <= 1e-9 * (1 + abs(want["objective"]))
+    if want["status"] == "optimal":
+        assert max(got["primal_infeasibility"], got["dual_infeasibility"], got["duality_gap"]) <= kw["tolerance"]
+
+
+def test_cfg1_full_shape_reference_order(gpu):
+    """m=2000, n=5000, 5 nnz/col equality LP, 2000 iterations: bit-identical trajectory."""
+    lp = mp.generate(mp.FAMILY_UNIFORM_EQ)
+    want = orc.solve(lp, tolerance=1e-6, max_iterations=2000)
+    got = gpu_solve(lp, tolerance=1e-6, max_iterations=2000, reduction_order=mp.ORDER_REFERENCE)
+    assert got["iterations"] == want["iterations"] == 2000
+    assert got["objective"] == want["objective"]
+    assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
+
+
+def test_batch_reference_order_matches_single_solves(gpu):
+    lps = [small_uniform(seed=100 + k, m=100 + 37 * k, n=250 + 90 * k) for k in range(4)] + [tiny_simplex()]
+    kw = dict(tolerance=1e-6, max_iterations=1200)
+    got = mp.solve_batch(lps, backend=mp.BACKEND_GPU, require_gpu=1, reduction_order=mp.ORDER_REFERENCE, **kw)
+    for lp, g in zip(lps, got):
+        w = orc.solve(lp, **kw)
+        assert g["backend_actually_used"] == "hip"
+        assert g["iterations"] == w["iterations"] and g["objective"] == w["objective"]
+        assert np.array_equal(g["primal"], w["primal"]) and np.array_equal(g["dual"], w["dual"])
+
+
+def test_graphs_and_eager_agree(gpu):
+    lp = small_mixed()
+    kw = dict(tolerance=1e-6, max_iterations=500, reduction_order=mp.ORDER_REFERENCE)
+    a = gpu_solve(lp, graph_chunk=0, **kw)
+    b = gpu_solve(lp, graph_chunk=-1, **kw)
+    c = gpu_solve(lp, graph_chunk=40, **kw)
+    for r in (b, c):
+        assert r["iterations"] == a["iterations"] and r["objective"] == a["objective"]
+        assert np.array_equal(r["primal"], a["primal"])
