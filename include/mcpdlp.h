@@ -77,6 +77,19 @@ typedef struct mc_lp {
 /* Arithmetic of the GPU engine (extension; no reference counterpart). */
 #define MC_PRECISION_FP64 0  /* every iteration in FP64                                   */
 #define MC_PRECISION_MIXED 1 /* FP32 SpMV/updates, FP64 checks, FP64 polish on stall     */
+/* Restart scheme (extension; both backends).
+ * REFERENCE: the reference's rule (pdlp.cpp:246-260): restart to the average when
+ *   its score fell to 0.368 x the last restart's, primal weight from sqrt(primal/dual).
+ * KKT: the adaptive restarts of Applegate et al. (2021, "Practical large-scale linear
+ *   programming using primal-dual hybrid gradient", sec. 3): candidate = the better of
+ *   the current iterate and the average by the same relative KKT score; restart on
+ *   sufficient decay (<= 0.2 x the score at the last restart), on necessary decay
+ *   (<= 0.8 x) without progress since the previous check, or when the epoch reaches
+ *   0.36 x all iterations; primal weight from the movement since the last restart,
+ *   omega <- exp(theta log(|dy|/|dx|) + (1-theta) log omega), theta = primal_weight_smoothing.
+ * Termination, tolerances and the reported score are the reference's in both. */
+#define MC_SCHEME_REFERENCE 0
+#define MC_SCHEME_KKT 1
 /* Reduction order of the GPU engine (extension). */
 #define MC_ORDER_FAST 0      /* tile-parallel sums (deterministic, not reference order)   */
 #define MC_ORDER_REFERENCE 1 /* every sum in the reference's serial order: bit-exact FP64 */
@@ -108,7 +121,7 @@ typedef struct mc_pdlp_options {
     int32_t graph_chunk;              /* iterations per captured HIP graph (0 => 64; < 0 => eager launches) */
     int32_t device;                   /* HIP device ordinal */
     int32_t require_gpu;              /* 1: a GPU request fails with MC_ERR_DEVICE instead of the cpu_fallback path */
-    int32_t pad1_;
+    int32_t restart_scheme;           /* MC_SCHEME_REFERENCE (default) or MC_SCHEME_KKT */
     const double* warm_primal;        /* optional [n] start point in the ORIGINAL space (NULL => cold x=proj(0)) */
     const double* warm_dual;          /* optional [m] start dual in the ORIGINAL space (reported sign convention) */
 } mc_pdlp_options;

@@ -95,6 +95,8 @@ typedef struct mcp_solve_opts {
     double polish_threshold;
     int32_t reduction_order;    /* MC_ORDER_* */
     int32_t graph_chunk;        /* iterations per HIP graph; 0 => default; <0 => no graphs */
+    int32_t restart_scheme;     /* MC_SCHEME_* */
+    int32_t pad_;
     const double* warm_primal;  /* optional [n] host, original space */
     const double* warm_dual;    /* optional [m] host, original space (reported convention) */
 } mcp_solve_opts;
@@ -136,6 +138,16 @@ int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
  * device Ruiz): CSC(A)-ordered values [nnz] and row/col scales. */
 int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* col_scale,
                     int32_t* passes, int32_t* converged);
+
+/* Micro-benchmark of one fused iteration kernel on the loaded, set-up problem,
+ * launched `reps` times on the engine's stream between two HIP events.
+ * kind: 0 primal (A^T y + primal update) FP32, 1 dual (A x_bar + dual update)
+ * FP32, 2 primal FP64, 3 dual FP64, 4 plain SpMV A x FP32, 5 plain A^T y FP32.
+ * *avg_us = mean device time per launch; *bytes = the kernel's algorithmic
+ * HBM bytes per launch (matrix and pointer streams once, gathered vector once,
+ * per-row/column operands once).  Leaves the iterate in an unspecified state;
+ * the next mcp_solve re-initializes it. */
+int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes);
 
 /* Current tile policies (after mcp_setup / autotune) and tile counts. */
 int mcp_get_tiling(mcp_engine* e, mcp_tile_policy* rows_policy, mcp_tile_policy* cols_policy,

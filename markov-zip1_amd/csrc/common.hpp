@@ -53,6 +53,9 @@ struct InstState {
     int32_t switch_flag;     // set by a check: convert FP32 iterate to FP64
     int32_t mode_at_check;   // mode in which the last check's iterate lives
     int32_t stall;           // checks without a 10% score improvement (FP32)
+    int32_t cand_cur;        // KKT scheme: this check's candidate is the current iterate
+    int32_t answer_cur;      // the reported iterate is the current one (else the average)
+    double prev_cand;        // KKT scheme: candidate score at the previous check of the epoch
     double ruiz_err_r, ruiz_err_c;  // setup: max |1 - norm| of the current pass
     int32_t ruiz_done, ruiz_passes, ruiz_converged, pad_;
     double b_norm_inf, c_norm_inf;  // setup: norms of the scaled b / c (floored at 1)
@@ -70,6 +73,8 @@ struct IterOpts {
     int32_t adaptive_weight;
     int32_t ruiz;
     int32_t mixed;
+    int32_t kkt;                 // restart scheme: 0 reference, 1 KKT (Applegate et al.)
+    int32_t pad_;
 };
 
 // Map a workgroup id to a tile so that each XCD works on a contiguous range

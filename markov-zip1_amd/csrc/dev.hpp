@@ -57,13 +57,19 @@ struct Dev {
     double *y, *yavg;
     float *x32, *xbar32, *xavg32, *dx32;
     float *y32, *yavg32;
-    double* ax;    // [m] reference-order mode: A*v results
+    double *xlast, *ylast;      // KKT scheme: iterate at the last restart
+    float *xlast32, *ylast32;
+    double* ax;    // [m] reference-order mode: A*v results (average candidate)
     double* aty;   // [n] reference-order mode: A^T*v results
+    double* ax2;   // [m] reference-order mode, current-iterate candidate
+    double* aty2;  // [n]
     // reduction partials (one slot group per tile)
     double* part_dxn;   // [ntT]
     double* part_adx;   // [ntA]
-    double* part_rows;  // [2*ntA]: max primal violation, dual-objective part
-    double* part_cols;  // [4*ntT]: max violation, dual residual^2, primal obj, dual obj part
+    // check partials per candidate c (0 average, 1 current iterate):
+    double* part_rows;  // [(c*ntA + tile)*3 + k]: max violation, dual-objective part, |y - ylast|^2
+    double* part_cols;  // [(c*ntT + tile)*5 + k]: max violation, dual residual^2, primal obj,
+                        //                          dual-objective part, |x - xlast|^2
     InstState* st;      // [B]
     int64_t* iter_base; // global iteration counter at the start of the current chunk
     int32_t* counters;  // [0] running instances, [1] running instances in FP32
@@ -76,10 +82,10 @@ void launch_primal(const Dev& d, int t, bool fp32, bool adapt, bool exact, hipSt
 void launch_dual(const Dev& d, int t, bool fp32, bool exact, hipStream_t s);
 void launch_adx(const Dev& d, int t, bool fp32, bool exact, hipStream_t s);
 void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s);
-void launch_resid(const Dev& d, bool fp32, bool exact, hipStream_t s);
+void launch_resid(const Dev& d, bool fp32, bool exact, bool current, hipStream_t s);
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
                            hipStream_t s);
-void launch_post_check(const Dev& d, hipStream_t s);
+void launch_post_check(const Dev& d, bool kkt, hipStream_t s);
 void launch_advance(const Dev& d, int len, hipStream_t s);
 // Setup kernels.
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -188,8 +190,8 @@ struct mcp_engine {
         // partial-sum slots follow the tile counts
         d.part_dxn = buf<double>("part_dxn", tT.size());
         d.part_adx = buf<double>("part_adx", tA.size());
-        d.part_rows = buf<double>("part_rows", 2 * tA.size());
-        d.part_cols = buf<double>("part_cols", 4 * tT.size());
+        d.part_rows = buf<double>("part_rows", 6 * tA.size());
+        d.part_cols = buf<double>("part_cols", 10 * tT.size());
         d.tA = dA;
         d.tT = dT;
         d.tA_begin = bA;
@@ -263,6 +265,7 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.adaptive_weight = o->adaptive_primal_weight;
     io.ruiz = 1;
     io.mixed = o->precision == MC_PRECISION_MIXED;
+    io.kkt = o->restart_scheme == MC_SCHEME_KKT ? 1 : 0;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
     io.polish_threshold = o->polish_threshold > 0.0 ? o->polish_threshold : 10.0 * tol;
 }
@@ -286,10 +289,12 @@ void enqueue_chunk(mcp_engine* e, const mcp::IterOpts& io, bool adaptive, bool e
         if (use32) mcp::launch_dual(d, t, true, false, s);
         if (use64) mcp::launch_dual(d, t, false, exact, s);
         if ((it + 1) % io.restart_every == 0) {
-            if (use32) mcp::launch_resid(d, true, false, s);
-            if (use64) mcp::launch_resid(d, false, exact, s);
+            for (int cur = 0; cur <= io.kkt; ++cur) {
+                if (use32) mcp::launch_resid(d, true, false, cur != 0, s);
+                if (use64) mcp::launch_resid(d, false, exact, cur != 0, s);
+            }
             mcp::launch_check_finalize(d, io, t, exact, false, s);
-            mcp::launch_post_check(d, s);
+            mcp::launch_post_check(d, io.kkt != 0, s);
         }
     }
     mcp::launch_advance(d, len, s);
@@ -431,8 +436,14 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         d.dx32 = e->buf<float>("dx32", n);
         d.y32 = e->buf<float>("y32", m);
         d.yavg32 = e->buf<float>("yavg32", m);
+        d.xlast = e->buf<double>("xlast", n);
+        d.ylast = e->buf<double>("ylast", m);
+        d.xlast32 = e->buf<float>("xlast32", n);
+        d.ylast32 = e->buf<float>("ylast32", m);
         d.ax = e->buf<double>("ax", m);
         d.aty = e->buf<double>("aty", n);
+        d.ax2 = e->buf<double>("ax2", m);
+        d.aty2 = e->buf<double>("aty2", n);
         d.st = e->buf<mcp::InstState>("st", B);
         d.iter_base = e->buf<int64_t>("iter_base", 1);
         d.counters = e->buf<int32_t>("counters", 4);
@@ -540,6 +551,10 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
             e->graph_dev = d;
         }
         HIPCHK(hipEventRecord(e->ev[0], s));
+        // MCPDLP_PROGRESS=<seconds>: report the solve's own progress on stderr.
+        const char* prog_env = std::getenv("MCPDLP_PROGRESS");
+        const double prog_every = prog_env ? std::atof(prog_env) : 0.0;
+        auto prog_last = std::chrono::steady_clock::now();
         int64_t iter = 0;
         int32_t fp32_running = mixed ? e->B : 0;
         bool finished = false;
@@ -579,6 +594,15 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                     break;
                 }
                 fp32_running = pin[2 * prev + 1];
+                if (prog_every > 0.0) {
+                    const auto now = std::chrono::steady_clock::now();
+                    if (std::chrono::duration<double>(now - prog_last).count() >= prog_every) {
+                        prog_last = now;
+                        std::fprintf(stderr, "[mcpdlp] iterations=%lld running=%d fp32=%d\n",
+                                     (long long)iter, pin[2 * prev], pin[2 * prev + 1]);
+                        std::fflush(stderr);
+                    }
+                }
             }
         }
         HIPCHK(hipStreamSynchronize(s));
@@ -586,8 +610,10 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         const int32_t still_running = chunks > 0 ? pin[2 * last] : e->B;
         if (!finished && still_running > 0) {
             // iteration limit: residuals of the final average (pdlp.cpp:263-265)
-            if (mixed) mcp::launch_resid(d, true, false, s);
-            mcp::launch_resid(d, false, exact, s);
+            for (int cur = 0; cur <= io.kkt; ++cur) {
+                if (mixed) mcp::launch_resid(d, true, false, cur != 0, s);
+                mcp::launch_resid(d, false, exact, cur != 0, s);
+            }
             mcp::launch_check_finalize(d, io, -1, exact, true, s);
             HIPCHK(hipGetLastError());
         }
@@ -714,6 +740,68 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
         HIPCHK(hipStreamSynchronize(s));
         if (passes) *passes = st0.ruiz_passes;
         if (converged) *converged = st0.ruiz_converged;
+    });
+}
+
+int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes) {
+    return guarded([&] {
+        require(e && e->setup_done, "mcp_time_kernel before mcp_setup");
+        require(kind >= 0 && kind <= 5, "unknown kernel kind");
+        HIPCHK(hipSetDevice(e->device));
+        auto& d = e->d;
+        hipStream_t s = e->stream;
+        const double m = e->m, n = e->n, nnz = (double)e->nnz;
+        const bool f32 = kind == 0 || kind == 1 || kind >= 4;
+        const double w = f32 ? 4.0 : 8.0;  // bytes per value
+        // Per-element operands: primal reads c/lo/hi/colnorm (packed float4 in
+        // FP32, four doubles in FP64), x, x_avg and writes x, x_bar, x_avg;
+        // dual reads blo/bhi/rownorm (float4 / three doubles), y, y_avg and
+        // writes y, y_avg.
+        double b = 0.0;
+        if (kind == 0 || kind == 2)
+            b = (4.0 + w) * nnz + 4.0 * (n + 1) + w * m + (f32 ? 16.0 : 32.0) * n + 2 * w * n + 3 * w * n;
+        else if (kind == 1 || kind == 3)
+            b = (4.0 + w) * nnz + 4.0 * (m + 1) + w * n + (f32 ? 16.0 : 24.0) * m + 2 * w * m + 2 * w * m;
+        else if (kind == 4)
+            b = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 4.0 * m;
+        else
+            b = 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 4.0 * n;
+        // every instance running in the kernel's precision, average count 1
+        std::vector<mcp::InstState> st(e->B);
+        HIPCHK(hipMemcpy(st.data(), d.st, sizeof(mcp::InstState) * e->B, hipMemcpyDeviceToHost));
+        for (auto& q : st) {
+            q.status = mcp::kRunning;
+            q.mode = f32 ? mcp::kModeFp32 : mcp::kModeFp64;
+            q.restart_iter = 0;
+            if (!(q.eta > 0.0)) q.eta = 0.9;
+            if (!(q.omega > 0.0)) q.omega = 1.0;
+        }
+        HIPCHK(hipMemcpy(d.st, st.data(), sizeof(mcp::InstState) * e->B, hipMemcpyHostToDevice));
+        int64_t zero = 0;
+        HIPCHK(hipMemcpy(d.iter_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        auto launch = [&] {
+            switch (kind) {
+                case 0: mcp::launch_primal(d, 1, true, false, false, s); break;
+                case 1: mcp::launch_dual(d, 1, true, false, s); break;
+                case 2: mcp::launch_primal(d, 1, false, false, false, s); break;
+                case 3: mcp::launch_dual(d, 1, false, false, s); break;
+                case 4:
+                    mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.x32, d.y32, true, false, s);
+                    break;
+                default:
+                    mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.x32, true, false, s);
+                    break;
+            }
+        };
+        launch();
+        const int nrep = std::max(1, reps);
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        for (int r = 0; r < nrep; ++r) launch();
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(e->ev[1]));
+        *avg_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / nrep;
+        *bytes = b;
     });
 }
 

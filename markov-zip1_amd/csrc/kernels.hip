@@ -213,9 +213,11 @@ __global__ void k_adapt_finalize(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 residual terms of the averaged iterate on the unscaled model
-// (pdlp.cpp:76-131).  Rows: A x_avg, row violation, dual-objective row part.
-template <typename TV, bool EXACT>
+// FP64 residual terms of a candidate iterate on the unscaled model
+// (pdlp.cpp:76-131).  CUR = false: the running average (the reference's only
+// candidate); CUR = true: the current iterate (KKT restart scheme).  Rows:
+// A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
+template <typename TV, bool EXACT, bool CUR>
 __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
     const int tile = xcd_tile(d.ntA);
     if (tile >= d.ntA) return;
@@ -225,15 +227,16 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
     __shared__ double prod[kNnzTile];
     __shared__ double rsum[kBlock];
     __shared__ double red[kWaves];
-    const TV* xa = pick<TV>(d.xavg, d.xavg32);
-    const TV* ya = pick<TV>(d.yavg, d.yavg32);
-    const int nr = tile_rowsums<EXACT, double>(tl, d.rp, d.ci, d.av64, xa, prod, rsum, red);
+    const TV* xc = CUR ? pick<TV>(d.x, d.x32) : pick<TV>(d.xavg, d.xavg32);
+    const TV* yc = CUR ? pick<TV>(d.y, d.y32) : pick<TV>(d.yavg, d.yavg32);
+    const TV* yl = pick<TV>(d.ylast, d.ylast32);
+    const int nr = tile_rowsums<EXACT, double>(tl, d.rp, d.ci, d.av64, xc, prod, rsum, red);
     const int tid = threadIdx.x;
-    double viol = 0.0, dob = 0.0;
+    double viol = 0.0, dob = 0.0, mv = 0.0;
     if (tid < nr) {
         const int i = tl.r0 + tid;
         if (EXACT) {
-            d.ax[i] = rsum[tid];
+            (CUR ? d.ax2 : d.ax)[i] = rsum[tid];
         } else {
             const double rs = d.row_scale[i];
             const double axu = rsum[tid] / rs;
@@ -242,23 +245,29 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
             const double r = fabs(axu - proj);
             const double sc = 1.0 + maxv(fabs(axu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
             viol = r / sc;
-            const double yu = (double)ya[i] * rs;
+            const double yu = (double)yc[i] * rs;
             if (yu > 0.0 && hi < 1e299)
                 dob = -(yu * hi);
             else if (yu < 0.0 && lo > -1e299)
                 dob = -(yu * lo);
+            const double dy = (double)yc[i] - (double)yl[i];
+            mv = dy * dy;
         }
     }
     if (!EXACT) {
+        double* out = d.part_rows + 3 * ((CUR ? d.ntA : 0) + tile);
         viol = block_max(viol, red);
-        if (tid == 0) d.part_rows[2 * tile] = viol;
+        if (tid == 0) out[0] = viol;
         dob = block_sum(dob, red);
-        if (tid == 0) d.part_rows[2 * tile + 1] = dob;
+        if (tid == 0) out[1] = dob;
+        mv = block_sum(mv, red);
+        if (tid == 0) out[2] = mv;
     }
 }
 
-// Columns: A^T y_avg, bound violation, dual residual, primal/dual objective parts.
-template <typename TV, bool EXACT>
+// Columns: A^T y_c, bound violation, dual residual, primal/dual objective
+// parts, |x_c - x_last|^2.
+template <typename TV, bool EXACT, bool CUR>
 __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
     const int tile = xcd_tile(d.ntT);
     if (tile >= d.ntT) return;
@@ -268,18 +277,19 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
     __shared__ double prod[kNnzTile];
     __shared__ double rsum[kBlock];
     __shared__ double red[kWaves];
-    const TV* xa = pick<TV>(d.xavg, d.xavg32);
-    const TV* ya = pick<TV>(d.yavg, d.yavg32);
-    const int nr = tile_rowsums<EXACT, double>(tl, d.cp, d.ri, d.tv64, ya, prod, rsum, red);
+    const TV* xc = CUR ? pick<TV>(d.x, d.x32) : pick<TV>(d.xavg, d.xavg32);
+    const TV* yc = CUR ? pick<TV>(d.y, d.y32) : pick<TV>(d.yavg, d.yavg32);
+    const TV* xl = pick<TV>(d.xlast, d.xlast32);
+    const int nr = tile_rowsums<EXACT, double>(tl, d.cp, d.ri, d.tv64, yc, prod, rsum, red);
     const int tid = threadIdx.x;
-    double viol = 0.0, dres = 0.0, pob = 0.0, dob = 0.0;
+    double viol = 0.0, dres = 0.0, pob = 0.0, dob = 0.0, mv = 0.0;
     if (tid < nr) {
         const int j = tl.r0 + tid;
         if (EXACT) {
-            d.aty[j] = rsum[tid];
+            (CUR ? d.aty2 : d.aty)[j] = rsum[tid];
         } else {
             const double cs = d.col_scale[j];
-            const double xu = (double)xa[j] * cs;
+            const double xu = (double)xc[j] * cs;
             const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
             const double proj = clampv(xu, lo, hi);
             const double r = fabs(xu - proj);
@@ -296,30 +306,41 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
                 dob = g * lo;
             else if (g < 0.0 && hi < 1e299)
                 dob = g * hi;
+            const double dxm = (double)xc[j] - (double)xl[j];
+            mv = dxm * dxm;
         }
     }
     if (!EXACT) {
+        double* out = d.part_cols + 5 * ((CUR ? d.ntT : 0) + tile);
         viol = block_max(viol, red);
-        if (tid == 0) d.part_cols[4 * tile] = viol;
+        if (tid == 0) out[0] = viol;
         dres = block_sum(dres, red);
-        if (tid == 0) d.part_cols[4 * tile + 1] = dres;
+        if (tid == 0) out[1] = dres;
         pob = block_sum(pob, red);
-        if (tid == 0) d.part_cols[4 * tile + 2] = pob;
+        if (tid == 0) out[2] = pob;
         dob = block_sum(dob, red);
-        if (tid == 0) d.part_cols[4 * tile + 3] = dob;
+        if (tid == 0) out[3] = dob;
+        mv = block_sum(mv, red);
+        if (tid == 0) out[4] = mv;
     }
 }
 
-// Reference-order residuals (pdlp.cpp:80-133), serial, from the stored
-// products d.ax = A_s x_avg and d.aty = A_s^T y_avg of instance b.
-__device__ void serial_residuals(const Dev& d, int b, double& primal, double& dual,
-                                 double& pobj, double& dobj, double c_scale) {
+struct CandResid {
+    double primal, dual, gap, score, objective, xmove2, ymove2;
+};
+
+// Reference-order residuals (pdlp.cpp:80-133), serial, of candidate
+// (xs, ys) of instance b from the stored products ax = A_s xs, aty = A_s^T ys;
+// movement sums |xs - x_last|^2, |ys - y_last|^2 in index order.
+__device__ void serial_residuals(const Dev& d, int b, const double* xs, const double* ys,
+                                 const double* ax, const double* aty, double c_scale,
+                                 CandResid& out) {
     const int r0 = d.inst_row[b], r1 = d.inst_row[b + 1];
     const int c0 = d.inst_col[b], c1 = d.inst_col[b + 1];
     const double sense = d.sense[b], off = d.offset[b];
     double mp = 0.0;
     for (int i = r0; i < r1; ++i) {
-        const double axu = d.ax[i] / d.row_scale[i];
+        const double axu = ax[i] / d.row_scale[i];
         const double lo = d.orow_lo[i], hi = d.orow_hi[i];
         const double proj = clampv(axu, lo, hi);
         const double r = fabs(axu - proj);
@@ -327,29 +348,29 @@ __device__ void serial_residuals(const Dev& d, int b, double& primal, double& du
         mp = maxv(mp, r / sc);
     }
     for (int j = c0; j < c1; ++j) {
-        const double xu = d.xavg[j] * d.col_scale[j];
+        const double xu = xs[j] * d.col_scale[j];
         const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
         const double proj = clampv(xu, lo, hi);
         const double r = fabs(xu - proj);
         const double sc = 1.0 + maxv(fabs(xu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
         mp = maxv(mp, r / sc);
     }
-    primal = mp;
+    out.primal = mp;
     double dsq = 0.0;
     for (int j = c0; j < c1; ++j) {
-        const double xu = d.xavg[j] * d.col_scale[j];
-        const double g = sense * d.obj[j] + d.aty[j] / d.col_scale[j];
+        const double xu = xs[j] * d.col_scale[j];
+        const double g = sense * d.obj[j] + aty[j] / d.col_scale[j];
         const double xp = clampv(xu - g, d.ovar_lo[j], d.ovar_hi[j]);
         const double df = xu - xp;
         dsq += df * df;
     }
-    dual = sqrt(dsq) / c_scale;
+    out.dual = sqrt(dsq) / c_scale;
     double po = off;
-    for (int j = c0; j < c1; ++j) po += d.obj[j] * (d.xavg[j] * d.col_scale[j]);
-    pobj = po;
+    for (int j = c0; j < c1; ++j) po += d.obj[j] * (xs[j] * d.col_scale[j]);
+    out.objective = po;
     double dob = off;
     for (int i = r0; i < r1; ++i) {
-        const double yu = d.yavg[i] * d.row_scale[i];
+        const double yu = ys[i] * d.row_scale[i];
         const double lo = d.orow_lo[i], hi = d.orow_hi[i];
         if (yu > 0.0 && hi < 1e299)
             dob -= yu * hi;
@@ -357,18 +378,65 @@ __device__ void serial_residuals(const Dev& d, int b, double& primal, double& du
             dob -= yu * lo;
     }
     for (int j = c0; j < c1; ++j) {
-        const double g = sense * d.obj[j] + d.aty[j] / d.col_scale[j];
+        const double g = sense * d.obj[j] + aty[j] / d.col_scale[j];
         const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
         if (g > 0.0 && lo > -1e299)
             dob += g * lo;
         else if (g < 0.0 && hi < 1e299)
             dob += g * hi;
     }
-    dobj = dob;
+    out.gap = fabs(po - dob) / (1.0 + fabs(po) + fabs(dob));
+    double sc = out.primal;
+    if (sc < out.dual) sc = out.dual;
+    if (sc < out.gap) sc = out.gap;
+    out.score = sc;
+    double xm = 0.0, ym = 0.0;
+    for (int j = c0; j < c1; ++j) xm += (xs[j] - d.xlast[j]) * (xs[j] - d.xlast[j]);
+    for (int i = r0; i < r1; ++i) ym += (ys[i] - d.ylast[i]) * (ys[i] - d.ylast[i]);
+    out.xmove2 = xm;
+    out.ymove2 = ym;
 }
 
-// Termination, adaptive restart and primal-weight update (pdlp.cpp:229-260).
-// FINAL: the iteration limit was reached -- record residuals, stop.
+// Candidate residuals from the tile partials (fast order).
+__device__ void partial_residuals(const Dev& d, int b, int cand, double c_scale, CandResid& out) {
+    double mp = 0.0, dsq = 0.0, po = 0.0, dr = 0.0, dc = 0.0, xm = 0.0, ym = 0.0;
+    const double* pr = d.part_rows + 3 * (cand ? d.ntA : 0);
+    const double* pc = d.part_cols + 5 * (cand ? d.ntT : 0);
+    for (int k = d.tA_begin[b]; k < d.tA_begin[b + 1]; ++k) {
+        mp = maxv(mp, pr[3 * k]);
+        dr += pr[3 * k + 1];
+        ym += pr[3 * k + 2];
+    }
+    for (int k = d.tT_begin[b]; k < d.tT_begin[b + 1]; ++k) {
+        mp = maxv(mp, pc[5 * k]);
+        dsq += pc[5 * k + 1];
+        po += pc[5 * k + 2];
+        dc += pc[5 * k + 3];
+        xm += pc[5 * k + 4];
+    }
+    out.primal = mp;
+    out.dual = sqrt(dsq) / c_scale;
+    out.objective = d.offset[b] + po;
+    const double dobj = d.offset[b] + dr + dc;
+    out.gap = fabs(out.objective - dobj) / (1.0 + fabs(out.objective) + fabs(dobj));
+    double sc = out.primal;
+    if (sc < out.dual) sc = out.dual;
+    if (sc < out.gap) sc = out.gap;
+    out.score = sc;
+    out.xmove2 = xm;
+    out.ymove2 = ym;
+}
+
+__device__ __forceinline__ void stop_instance(const Dev& d, InstState& s, int32_t status, int64_t it) {
+    s.status = status;
+    s.done_iter = it;
+    atomicSub(&d.counters[0], 1);
+    if (s.mode == kModeFp32) atomicSub(&d.counters[1], 1);
+}
+
+// Termination and restart decision of one instance (pdlp.cpp:229-260; the
+// KKT scheme as documented for MC_SCHEME_KKT).  FINAL: the iteration limit
+// was reached -- record the better candidate's residuals and stop.
 template <bool EXACT, bool FINAL>
 __global__ void k_check_finalize(Dev d, IterOpts o, int t) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,68 +446,76 @@ __global__ void k_check_finalize(Dev d, IterOpts o, int t) {
     s.switch_flag = 0;
     if (s.status != kRunning) return;
     const int64_t iter_after = *d.iter_base + t + 1;
-    double primal, dual, pobj, dobj;
-    if (EXACT) {
-        serial_residuals(d, b, primal, dual, pobj, dobj, s.c_scale);
-    } else {
-        double mp = 0.0, dsq = 0.0, po = 0.0, dr = 0.0, dc = 0.0;
-        for (int k = d.tA_begin[b]; k < d.tA_begin[b + 1]; ++k) {
-            mp = maxv(mp, d.part_rows[2 * k]);
-            dr += d.part_rows[2 * k + 1];
+    CandResid r, rc;
+    if (EXACT)
+        serial_residuals(d, b, d.xavg, d.yavg, d.ax, d.aty, s.c_scale, r);
+    else
+        partial_residuals(d, b, 0, s.c_scale, r);
+    int use_cur = 0;
+    if (o.kkt) {
+        if (EXACT)
+            serial_residuals(d, b, d.x, d.y, d.ax2, d.aty2, s.c_scale, rc);
+        else
+            partial_residuals(d, b, 1, s.c_scale, rc);
+        if (rc.score < r.score) {
+            r = rc;
+            use_cur = 1;
         }
-        for (int k = d.tT_begin[b]; k < d.tT_begin[b + 1]; ++k) {
-            mp = maxv(mp, d.part_cols[4 * k]);
-            dsq += d.part_cols[4 * k + 1];
-            po += d.part_cols[4 * k + 2];
-            dc += d.part_cols[4 * k + 3];
-        }
-        primal = mp;
-        dual = sqrt(dsq) / s.c_scale;
-        pobj = d.offset[b] + po;
-        dobj = d.offset[b] + dr + dc;
     }
-    const double gap = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
-    double score = primal;
-    if (score < dual) score = dual;
-    if (score < gap) score = gap;
-    s.primal = primal;
-    s.dual = dual;
-    s.gap = gap;
-    s.score = score;
-    s.objective = pobj;
+    s.primal = r.primal;
+    s.dual = r.dual;
+    s.gap = r.gap;
+    s.score = r.score;
+    s.objective = r.objective;
+    s.answer_cur = use_cur;
+    s.cand_cur = use_cur;
     s.mode_at_check = s.mode;
     if (FINAL) {
-        s.status = kIterLimit;
-        s.done_iter = iter_after;
-        atomicSub(&d.counters[0], 1);
-        if (s.mode == kModeFp32) atomicSub(&d.counters[1], 1);
+        stop_instance(d, s, kIterLimit, iter_after);
         return;
     }
-    if (primal <= o.primal_tol && dual <= o.dual_tol && gap <= o.gap_tol) {
-        s.status = kOptimal;
-        s.done_iter = iter_after;
-        atomicSub(&d.counters[0], 1);
-        if (s.mode == kModeFp32) atomicSub(&d.counters[1], 1);
+    if (r.primal <= o.primal_tol && r.dual <= o.dual_tol && r.gap <= o.gap_tol) {
+        stop_instance(d, s, kOptimal, iter_after);
         return;
     }
-    if (!(score == score)) {  // NaN: the iterate diverged
-        s.status = kNumFail;
-        s.done_iter = iter_after;
-        atomicSub(&d.counters[0], 1);
-        if (s.mode == kModeFp32) atomicSub(&d.counters[1], 1);
+    if (!(r.score == r.score)) {  // NaN: the iterate diverged
+        stop_instance(d, s, kNumFail, iter_after);
         return;
     }
+    const double score = r.score;
     const int64_t since = iter_after - s.restart_iter;
-    const bool do_restart =
-        (o.restart_strategy == 2) ||
-        (o.restart_strategy == 1 &&
-         (score <= o.restart_factor * s.last_restart_score ||
-          (since >= 5 * o.restart_every && score < s.last_restart_score)));
+    bool do_restart;
+    if (!o.kkt) {
+        do_restart = (o.restart_strategy == 2) ||
+                     (o.restart_strategy == 1 &&
+                      (score <= o.restart_factor * s.last_restart_score ||
+                       (since >= 5 * o.restart_every && score < s.last_restart_score)));
+    } else {
+        do_restart = (o.restart_strategy == 2) ||
+                     (o.restart_strategy == 1 &&
+                      (score <= 0.2 * s.last_restart_score ||
+                       (score <= 0.8 * s.last_restart_score && score > s.prev_cand) ||
+                       (double)since >= 0.36 * (double)iter_after));
+        s.prev_cand = score;
+    }
     if (do_restart) {
-        if (o.adaptive_weight && primal > 1e-12 && dual > 1e-12) {
-            const double ratio = clampv(sqrt(primal / dual), 0.05, 20.0);
-            const double f = (o.smoothing == 0.5) ? sqrt(ratio) : pow(ratio, o.smoothing);
-            s.omega = clampv(s.omega * f, 1e-6, 1e6);
+        if (!o.kkt) {
+            if (o.adaptive_weight && r.primal > 1e-12 && r.dual > 1e-12) {
+                const double ratio = clampv(sqrt(r.primal / r.dual), 0.05, 20.0);
+                const double f = (o.smoothing == 0.5) ? sqrt(ratio) : pow(ratio, o.smoothing);
+                s.omega = clampv(s.omega * f, 1e-6, 1e6);
+            }
+        } else {
+            if (o.adaptive_weight) {
+                const double dX = sqrt(r.xmove2), dY = sqrt(r.ymove2);
+                if (dX > 1e-10 && dY > 1e-10) {
+                    const double th = o.smoothing, ratio = dY / dX;
+                    const double w = (th == 0.5) ? sqrt(s.omega * ratio)
+                                                 : pow(ratio, th) * pow(s.omega, 1.0 - th);
+                    s.omega = clampv(w, 1e-6, 1e6);
+                }
+            }
+            s.prev_cand = 1e300;
         }
         s.restart_flag = 1;
         s.restart_iter = iter_after;
@@ -463,38 +539,37 @@ __global__ void k_check_finalize(Dev d, IterOpts o, int t) {
     }
 }
 
-// Restart to the average (pdlp.cpp:257) and FP32 -> FP64 conversion of an
-// instance's iterate.  Grid covers the column tiles then the row tiles.
-__global__ __launch_bounds__(kBlock) void k_post_check(Dev d) {
+// Restart to the candidate (pdlp.cpp:257; KKT scheme: the current iterate or
+// the average, which also becomes the new x_last/y_last) and FP32 -> FP64
+// conversion of an instance's iterate.  Grid: column tiles, then row tiles.
+__global__ __launch_bounds__(kBlock) void k_post_check(Dev d, int kkt) {
     const int g = blockIdx.x;
     const bool cols = g < d.ntT;
     const Tile tl = cols ? d.tT[g] : d.tA[g - d.ntT];
     const InstState& s = d.st[tl.inst];
     if (!(s.restart_flag || s.switch_flag)) return;
     const bool r32 = s.mode_at_check == kModeFp32;
+    const bool to_avg = !s.cand_cur;
+    double* v64 = cols ? d.x : d.y;
+    double* a64 = cols ? d.xavg : d.yavg;
+    double* l64 = cols ? d.xlast : d.ylast;
+    float* v32 = cols ? d.x32 : d.y32;
+    float* a32 = cols ? d.xavg32 : d.yavg32;
+    float* l32 = cols ? d.xlast32 : d.ylast32;
     for (int k = tl.r0 + threadIdx.x; k < tl.r1; k += kBlock) {
-        if (cols) {
-            if (s.restart_flag) {
-                if (r32)
-                    d.x32[k] = d.xavg32[k];
-                else
-                    d.x[k] = d.xavg[k];
+        if (s.restart_flag) {
+            if (r32) {
+                if (to_avg) v32[k] = a32[k];
+                if (kkt) l32[k] = v32[k];
+            } else {
+                if (to_avg) v64[k] = a64[k];
+                if (kkt) l64[k] = v64[k];
             }
-            if (s.switch_flag) {
-                d.x[k] = (double)d.x32[k];
-                d.xavg[k] = (double)d.xavg32[k];
-            }
-        } else {
-            if (s.restart_flag) {
-                if (r32)
-                    d.y32[k] = d.yavg32[k];
-                else
-                    d.y[k] = d.yavg[k];
-            }
-            if (s.switch_flag) {
-                d.y[k] = (double)d.y32[k];
-                d.yavg[k] = (double)d.yavg32[k];
-            }
+        }
+        if (s.switch_flag) {
+            v64[k] = (double)v32[k];
+            a64[k] = (double)a32[k];
+            l64[k] = (double)l32[k];
         }
     }
 }
@@ -656,6 +731,8 @@ __global__ void k_init_iterates(Dev d, const double* warm_x, const double* warm_
         d.x32[k] = (float)x0;
         d.xbar32[k] = (float)x0;
         d.xavg32[k] = (float)x0;
+        d.xlast[k] = x0;
+        d.xlast32[k] = (float)x0;
     } else if (k < d.n + d.m) {
         const int i = k - d.n;
         const double y0 = warm_y ? warm_y[i] / d.row_scale[i] : 0.0;
@@ -663,6 +740,8 @@ __global__ void k_init_iterates(Dev d, const double* warm_x, const double* warm_
         d.yavg[i] = y0;
         d.y32[i] = (float)y0;
         d.yavg32[i] = (float)y0;
+        d.ylast[i] = y0;
+        d.ylast32[i] = (float)y0;
     }
 }
 
@@ -684,20 +763,30 @@ __global__ void k_setup_state(Dev d, double initial_weight, double step_reductio
     s.restart_flag = 0;
     s.switch_flag = 0;
     s.stall = 0;
+    s.cand_cur = 0;
+    s.answer_cur = 0;
+    s.prev_cand = 1e300;
     s.primal = s.dual = s.gap = s.score = 1e300;
     s.objective = 0.0;
 }
 
-// Unscaled averaged iterate (pdlp.cpp:76-79): x = x_avg * col_scale, y = y_avg * row_scale.
+// Unscaled reported iterate (pdlp.cpp:76-79): x = x_c * col_scale, y = y_c * row_scale
+// for the candidate of the last check (the average under the reference scheme).
 __global__ void k_unscale(Dev d, double* xo, double* yo) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < d.n) {
-        const bool f32 = d.st[d.col_inst[k]].mode == kModeFp32;
-        xo[k] = (f32 ? (double)d.xavg32[k] : d.xavg[k]) * d.col_scale[k];
+        const InstState& s = d.st[d.col_inst[k]];
+        const bool f32 = s.mode == kModeFp32;
+        const double v = s.answer_cur ? (f32 ? (double)d.x32[k] : d.x[k])
+                                      : (f32 ? (double)d.xavg32[k] : d.xavg[k]);
+        xo[k] = v * d.col_scale[k];
     } else if (k < d.n + d.m) {
         const int i = k - d.n;
-        const bool f32 = d.st[d.row_inst[i]].mode == kModeFp32;
-        yo[i] = (f32 ? (double)d.yavg32[i] : d.yavg[i]) * d.row_scale[i];
+        const InstState& s = d.st[d.row_inst[i]];
+        const bool f32 = s.mode == kModeFp32;
+        const double v = s.answer_cur ? (f32 ? (double)d.y32[i] : d.y[i])
+                                      : (f32 ? (double)d.yavg32[i] : d.yavg[i]);
+        yo[i] = v * d.row_scale[i];
     }
 }
 
@@ -776,26 +865,34 @@ void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s) {
         hipLaunchKernelGGL((k_adapt_finalize<false>), g, b, 0, s, d);
 }
 
-void launch_resid(const Dev& d, bool fp32, bool exact, hipStream_t s) {
+template <bool CUR>
+static void resid_pair(const Dev& d, bool fp32, bool exact, hipStream_t s) {
     const dim3 b(kBlock);
     if (d.ntA > 0) {
         const dim3 g(tile_grid(d.ntA));
         if (fp32)
-            hipLaunchKernelGGL((k_resid_rows<float, false>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_rows<float, false, CUR>), g, b, 0, s, d);
         else if (exact)
-            hipLaunchKernelGGL((k_resid_rows<double, true>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_rows<double, true, CUR>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_rows<double, false>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_rows<double, false, CUR>), g, b, 0, s, d);
     }
     if (d.ntT > 0) {
         const dim3 g(tile_grid(d.ntT));
         if (fp32)
-            hipLaunchKernelGGL((k_resid_cols<float, false>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols<float, false, CUR>), g, b, 0, s, d);
         else if (exact)
-            hipLaunchKernelGGL((k_resid_cols<double, true>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols<double, true, CUR>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_cols<double, false>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols<double, false, CUR>), g, b, 0, s, d);
     }
+}
+
+void launch_resid(const Dev& d, bool fp32, bool exact, bool current, hipStream_t s) {
+    if (current)
+        resid_pair<true>(d, fp32, exact, s);
+    else
+        resid_pair<false>(d, fp32, exact, s);
 }
 
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
@@ -814,9 +911,9 @@ void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, b
     }
 }
 
-void launch_post_check(const Dev& d, hipStream_t s) {
+void launch_post_check(const Dev& d, bool kkt, hipStream_t s) {
     if (d.ntT + d.ntA == 0) return;
-    hipLaunchKernelGGL(k_post_check, dim3(d.ntT + d.ntA), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_post_check, dim3(d.ntT + d.ntA), dim3(kBlock), 0, s, d, kkt ? 1 : 0);
 }
 
 void launch_advance(const Dev& d, int len, hipStream_t s) {

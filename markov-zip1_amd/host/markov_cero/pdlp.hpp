@@ -18,6 +18,13 @@ enum class Backend { cpu, gpu };
 enum class RestartStrategy { none, adaptive, fixed };
 enum class Precision { fp64, mixed };               // extension
 enum class ReductionOrder { fast, reference };      // extension
+// extension: `reference` keeps the reference's restart rule (pdlp.cpp:246-260);
+// `kkt` uses the adaptive restarts of Applegate et al. (2021): candidate = the
+// better of current iterate and average by the reference's score, restart on
+// sufficient (0.2x) / necessary-without-progress (0.8x) decay or an epoch of
+// 0.36 x all iterations, primal weight from the movement since the last
+// restart (omega <- (|dy|/|dx|)^theta omega^(1-theta), theta = smoothing).
+enum class RestartScheme { reference, kkt };
 
 struct PdlpOptions {
     std::size_t max_iterations{100000};
@@ -38,6 +45,7 @@ struct PdlpOptions {
     std::size_t ruiz_iterations{10};
 
     // --- extensions (Backend::gpu) ---
+    RestartScheme restart_scheme{RestartScheme::reference};
     Precision precision{Precision::fp64};
     ReductionOrder reduction_order{ReductionOrder::fast};
     double polish_threshold{0.0};   // mixed: FP64 from this checked score (0 => 10 x tolerance)

@@ -179,10 +179,30 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     p.ax.assign(m, 0.0);
     std::vector<double> xprev(n), xu, yu;
 
+    const bool kkt = opt.restart_scheme == RestartScheme::kkt;
+    std::vector<double> xlast = p.x, ylast = p.y, ax2(m), aty2(n), xu2, yu2;
     std::size_t iter = 0, k = 0, since = 0;
-    double last_score = 1e300;
+    double last_score = 1e300, prev_cand = 1e300;
     Residuals cur;
-    bool converged = false;
+    bool converged = false, use_cur = false;
+    // Average's residuals; for the KKT scheme also the current iterate's, and
+    // the better of the two becomes the candidate (cur, xu, yu).
+    auto evaluate_candidates = [&] {
+        p.a_times(p.xavg, p.ax);
+        p.at_times(p.yavg, p.aty);
+        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, opt.ruiz_scaling, xu, yu);
+        use_cur = false;
+        if (!kkt) return;
+        p.a_times(p.x, ax2);
+        p.at_times(p.y, aty2);
+        const Residuals alt = evaluate(orig, p.x, p.y, ax2, aty2, sc, opt.ruiz_scaling, xu2, yu2);
+        if (alt.score < cur.score) {
+            use_cur = true;
+            cur = alt;
+            std::swap(xu, xu2);
+            std::swap(yu, yu2);
+        }
+    };
     while (iter < opt.max_iterations) {
         p.at_times(p.y, p.aty);
         xprev = p.x;
@@ -223,34 +243,61 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
         ++iter;
         ++since;
         if (iter % opt.restart_every != 0) continue;
-        p.a_times(p.xavg, p.ax);
-        p.at_times(p.yavg, p.aty);
-        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, opt.ruiz_scaling, xu, yu);
+        evaluate_candidates();
         if (cur.primal <= opt.primal_tolerance && cur.dual <= opt.dual_tolerance && cur.gap <= opt.gap_tolerance) {
             converged = true;
             break;
         }
-        const bool restart = opt.restart_strategy == RestartStrategy::fixed ||
-                             (opt.restart_strategy == RestartStrategy::adaptive &&
-                              (cur.score <= opt.restart_reduction_factor * last_score ||
-                               (since >= 5 * opt.restart_every && cur.score < last_score)));
-        if (!restart) continue;
-        if (opt.adaptive_primal_weight && cur.primal > 1e-12 && cur.dual > 1e-12) {
-            const double ratio = std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0);
-            p.omega = std::clamp(p.omega * std::pow(ratio, opt.primal_weight_smoothing), 1e-6, 1e6);
+        bool restart;
+        if (!kkt) {
+            restart = opt.restart_strategy == RestartStrategy::fixed ||
+                      (opt.restart_strategy == RestartStrategy::adaptive &&
+                       (cur.score <= opt.restart_reduction_factor * last_score ||
+                        (since >= 5 * opt.restart_every && cur.score < last_score)));
+        } else {
+            restart = opt.restart_strategy == RestartStrategy::fixed ||
+                      (opt.restart_strategy == RestartStrategy::adaptive &&
+                       (cur.score <= 0.2 * last_score || (cur.score <= 0.8 * last_score && cur.score > prev_cand) ||
+                        static_cast<double>(since) >= 0.36 * static_cast<double>(iter)));
+            prev_cand = cur.score;
         }
-        p.x = p.xavg;
-        p.y = p.yavg;
+        if (!restart) continue;
+        if (!kkt) {
+            if (opt.adaptive_primal_weight && cur.primal > 1e-12 && cur.dual > 1e-12) {
+                const double ratio = std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0);
+                p.omega = std::clamp(p.omega * std::pow(ratio, opt.primal_weight_smoothing), 1e-6, 1e6);
+            }
+            p.x = p.xavg;
+            p.y = p.yavg;
+        } else {
+            const auto& xc = use_cur ? p.x : p.xavg;
+            const auto& yc = use_cur ? p.y : p.yavg;
+            if (opt.adaptive_primal_weight) {
+                double dxs = 0.0, dys = 0.0;
+                for (std::size_t j = 0; j < n; ++j) dxs += (xc[j] - xlast[j]) * (xc[j] - xlast[j]);
+                for (std::size_t i = 0; i < m; ++i) dys += (yc[i] - ylast[i]) * (yc[i] - ylast[i]);
+                const double dX = std::sqrt(dxs), dY = std::sqrt(dys);
+                if (dX > 1e-10 && dY > 1e-10) {
+                    const double th = opt.primal_weight_smoothing, ratio = dY / dX;
+                    const double w = th == 0.5 ? std::sqrt(p.omega * ratio)
+                                               : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
+                    p.omega = std::clamp(w, 1e-6, 1e6);
+                }
+            }
+            if (!use_cur) {
+                p.x = p.xavg;
+                p.y = p.yavg;
+            }
+            xlast = p.x;
+            ylast = p.y;
+            prev_cand = 1e300;
+        }
         p.xbar = p.x;
         k = 0;
         since = 0;
         last_score = cur.score;
     }
-    if (!converged) {
-        p.a_times(p.xavg, p.ax);
-        p.at_times(p.yavg, p.aty);
-        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, opt.ruiz_scaling, xu, yu);
-    }
+    if (!converged) evaluate_candidates();
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     res.status = converged ? PdlpStatus::optimal : PdlpStatus::iteration_limit;
     res.message = converged ? "PDLP converged" : "iteration limit reached";

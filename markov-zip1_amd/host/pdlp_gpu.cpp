@@ -233,6 +233,7 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.polish_threshold = opt.polish_threshold;
     o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
     o.graph_chunk = opt.graph_chunk;
+    o.restart_scheme = opt.restart_scheme == RestartScheme::kkt ? MC_SCHEME_KKT : MC_SCHEME_REFERENCE;
     o.warm_primal = warm_x ? opt.warm_primal.data() : nullptr;
     o.warm_dual = warm_y ? opt.warm_dual.data() : nullptr;
     mcp_solve_stats st{};

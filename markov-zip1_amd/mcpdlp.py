@@ -27,6 +27,7 @@ BACKEND_CPU, BACKEND_GPU = 0, 1
 RESTART_NONE, RESTART_ADAPTIVE, RESTART_FIXED = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 ORDER_FAST, ORDER_REFERENCE = 0, 1
+SCHEME_REFERENCE, SCHEME_KKT = 0, 1
 STATUS = {0: "optimal", 1: "iteration_limit", 2: "infeasible_or_unbounded", 3: "numerical_failure"}
 FAMILY_UNIFORM_EQ, FAMILY_ZIPF_MIXED, FAMILY_BLOCK_ANGULAR = 1, 2, 5
 
@@ -63,7 +64,7 @@ class Options(C.Structure):
         ("ruiz_scaling", C.c_int32), ("pad0_", C.c_int32), ("ruiz_iterations", C.c_uint64),
         ("precision", C.c_int32), ("reduction_order", C.c_int32),
         ("polish_threshold", C.c_double),
-        ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("pad1_", C.c_int32),
+        ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("restart_scheme", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
     ]
 
@@ -127,6 +128,7 @@ class EngSolve(C.Structure):
         ("initial_primal_weight", C.c_double), ("primal_weight_smoothing", C.c_double),
         ("polish_threshold", C.c_double),
         ("reduction_order", C.c_int32), ("graph_chunk", C.c_int32),
+        ("restart_scheme", C.c_int32), ("pad_", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
     ]
 
@@ -199,6 +201,7 @@ def engine_lib():
         lib.mcp_spmv.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P_f64, P_f64, C.c_int32, P_f64]
         lib.mcp_get_scaling.argtypes = [C.c_void_p, P_f64, P_f64, P_f64, P_i32, P_i32]
         lib.mcp_get_tiling.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy), P_i32, P_i32]
+        lib.mcp_time_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P_f64, P_f64]
         _engine = lib
     return _engine
 
@@ -534,6 +537,15 @@ class Engine:
         na, nt = C.c_int32(), C.c_int32()
         _check_eng(self.lib.mcp_get_tiling(self.h, C.byref(a), C.byref(t), C.byref(na), C.byref(nt)))
         return (a.nnz_budget, a.per_lane, na.value), (t.nnz_budget, t.per_lane, nt.value)
+
+    KERNELS = {"primal_fp32": 0, "dual_fp32": 1, "primal_fp64": 2, "dual_fp64": 3,
+               "spmv_fp32": 4, "spmv_t_fp32": 5}
+
+    def time_kernel(self, kind: str, reps: int = 20):
+        """(mean device us per launch, algorithmic bytes per launch) of one fused kernel."""
+        us, b = C.c_double(), C.c_double()
+        _check_eng(self.lib.mcp_time_kernel(self.h, self.KERNELS[kind], int(reps), C.byref(us), C.byref(b)))
+        return us.value, b.value
 
     def solve(self, tolerance=1e-4, max_iterations=100000, precision=PRECISION_FP64, order=ORDER_FAST,
               graph_chunk=0, **kw):

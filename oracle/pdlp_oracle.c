@@ -16,7 +16,8 @@
  *                               compiled reference scalers (bit-exact)
  *   or_residuals             -- zero at planted (x*, -y*), hand 2x2 cases,
  *                               compiled reference
- *   or_solve_pdlp            -- reference golden fixtures (iterations,
+ *   or_solve_pdlp            -- compiled reference (bit-exact), golden
+ *                               fixtures (iterations,
  *                               objective, residuals, vector hashes), ADR-P3-05
  *                               known answer, pdlp_test objectives, planted
  *                               optimum c^T x*, Netlib optima
@@ -219,7 +220,10 @@ typedef struct or_trace {
     int32_t* restart;/* [capacity] 1 if the check restarted       */
 } or_trace;
 
-/* solve_pdlp, reference pdlp.cpp:139-276.  `res->primal`/`res->dual` must hold
+/* solve_pdlp, reference pdlp.cpp:139-276.  With o->restart_scheme ==
+ * MC_SCHEME_KKT the restart decision follows the extension documented in
+ * include/mcpdlp.h (Applegate et al. 2021 adaptive restarts); everything else
+ * is unchanged.  `res->primal`/`res->dual` must hold
  * n / m doubles (or be NULL).  Timings are wall-clock of this restatement.
  * Optional `trace` records the per-check scalar sequence. */
 int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res,
@@ -321,10 +325,41 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
         x_bar[j] = x[j];
         x_avg[j] = x[j];
     }
+    const int kkt = (o->restart_scheme == MC_SCHEME_KKT);
+    double* x_last = (double*)malloc(sizeof(double) * (size_t)n);
+    double* y_last = (double*)calloc((size_t)m, sizeof(double));
+    double* Ax2 = (double*)malloc(sizeof(double) * (size_t)m);
+    double* At_y2 = (double*)malloc(sizeof(double) * (size_t)n);
+    double* xo2 = (double*)malloc(sizeof(double) * (size_t)n);
+    double* yo2 = (double*)malloc(sizeof(double) * (size_t)m);
+    memcpy(x_last, x, sizeof(double) * (size_t)n);
     uint64_t iter = 0, avg_count = 0, since = 0;
-    double last_restart_score = 1e300;
-    or_resid cur;
-    int optimal = 0;
+    double last_restart_score = 1e300, prev_cand = 1e300;
+    or_resid cur, cur2;
+    int optimal = 0, use_cur = 0;
+    /* Residuals of the average (and, for the KKT scheme, of the current
+     * iterate); `cur`/xo/yo end up holding the chosen candidate. */
+#define OR_EVALUATE()                                                                             \
+    do {                                                                                          \
+        or_spmv(m, n, cs, ri, val, x_avg, Ax);                                                    \
+        or_spmv_t(m, n, cs, ri, val, y_avg, At_y);                                                \
+        or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, o->ruiz_scaling, xo, yo,   \
+                     &cur);                                                                       \
+        use_cur = 0;                                                                              \
+        if (kkt) {                                                                                \
+            or_spmv(m, n, cs, ri, val, x, Ax2);                                                   \
+            or_spmv_t(m, n, cs, ri, val, y, At_y2);                                               \
+            or_residuals(lp, x, y, Ax2, At_y2, row_scale, col_scale, o->ruiz_scaling, xo2, yo2,   \
+                         &cur2);                                                                  \
+            if (cur2.score < cur.score) {                                                         \
+                double* t_;                                                                       \
+                use_cur = 1;                                                                      \
+                cur = cur2;                                                                       \
+                t_ = xo; xo = xo2; xo2 = t_;                                                      \
+                t_ = yo; yo = yo2; yo2 = t_;                                                      \
+            }                                                                                     \
+        }                                                                                         \
+    } while (0)
     while (iter < o->max_iterations) {
         or_spmv_t(m, n, cs, ri, val, y, At_y); /* :195 */
         memcpy(x_prev, x, sizeof(double) * (size_t)n);
@@ -367,15 +402,12 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
         ++iter;
         ++since;
         if (iter % o->restart_every == 0) { /* :229-261 */
-            or_spmv(m, n, cs, ri, val, x_avg, Ax);
-            or_spmv_t(m, n, cs, ri, val, y_avg, At_y);
-            or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, o->ruiz_scaling, xo,
-                         yo, &cur);
+            OR_EVALUATE();
             int restarted = 0;
             if (cur.primal <= o->primal_tolerance && cur.dual <= o->dual_tolerance &&
                 cur.gap <= o->gap_tolerance) {
                 optimal = 1;
-            } else {
+            } else if (!kkt) {
                 const double score = cur.score;
                 const int do_restart =
                     (o->restart_strategy == MC_RESTART_FIXED) ||
@@ -396,23 +428,61 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                     last_restart_score = score;
                     restarted = 1;
                 }
+            } else {
+                /* KKT scheme (extension, see MC_SCHEME_KKT in include/mcpdlp.h) */
+                const double score = cur.score;
+                const int do_restart =
+                    (o->restart_strategy == MC_RESTART_FIXED) ||
+                    (o->restart_strategy == MC_RESTART_ADAPTIVE &&
+                     (score <= 0.2 * last_restart_score ||
+                      (score <= 0.8 * last_restart_score && score > prev_cand) ||
+                      (double)since >= 0.36 * (double)iter));
+                prev_cand = score;
+                if (do_restart) {
+                    const double* xc = use_cur ? x : x_avg;
+                    const double* yc = use_cur ? y : y_avg;
+                    if (o->adaptive_primal_weight) {
+                        double dxs = 0.0, dys = 0.0;
+                        for (int64_t j = 0; j < n; ++j) dxs += (xc[j] - x_last[j]) * (xc[j] - x_last[j]);
+                        for (int64_t i = 0; i < m; ++i) dys += (yc[i] - y_last[i]) * (yc[i] - y_last[i]);
+                        const double dX = sqrt(dxs), dY = sqrt(dys);
+                        if (dX > 1e-10 && dY > 1e-10) {
+                            const double th = o->primal_weight_smoothing;
+                            const double ratio = dY / dX;
+                            const double w = (th == 0.5) ? sqrt(omega * ratio)
+                                                         : pow(ratio, th) * pow(omega, 1.0 - th);
+                            omega = or_clamp(w, 1e-6, 1e6);
+                            OR_UPDATE_STEPS();
+                        }
+                    }
+                    if (!use_cur) {
+                        memcpy(x, x_avg, sizeof(double) * (size_t)n);
+                        memcpy(y, y_avg, sizeof(double) * (size_t)m);
+                    }
+                    memcpy(x_last, x, sizeof(double) * (size_t)n);
+                    memcpy(y_last, y, sizeof(double) * (size_t)m);
+                    for (int64_t j = 0; j < n; ++j) x_bar[j] = x[j];
+                    avg_count = 0;
+                    since = 0;
+                    last_restart_score = score;
+                    prev_cand = 1e300;
+                    restarted = 1;
+                }
             }
             if (trace && trace->count < trace->capacity) {
                 trace->score[trace->count] = cur.score;
                 trace->omega[trace->count] = omega;
                 trace->eta[trace->count] = eta;
-                trace->restart[trace->count] = restarted;
+                trace->restart[trace->count] = restarted + 2 * use_cur;
                 trace->count++;
             }
             if (optimal) break;
         }
     }
     if (!optimal) { /* :263-265 */
-        or_spmv(m, n, cs, ri, val, x_avg, Ax);
-        or_spmv_t(m, n, cs, ri, val, y_avg, At_y);
-        or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, o->ruiz_scaling, xo, yo,
-                     &cur);
+        OR_EVALUATE();
     }
+#undef OR_EVALUATE
     res->status = optimal ? MC_PDLP_OPTIMAL : MC_PDLP_ITERATION_LIMIT;
     strcpy(res->message, optimal ? "PDLP converged" : "iteration limit reached");
     res->objective = cur.objective;
@@ -429,6 +499,7 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
     free(row_norms); free(col_norms); free(tau); free(sigma);
     free(x); free(y); free(x_bar); free(x_avg); free(y_avg); free(dx); free(x_prev);
     free(At_y); free(Ax); free(xo); free(yo);
+    free(x_last); free(y_last); free(Ax2); free(At_y2); free(xo2); free(yo2);
 #undef OR_UPDATE_STEPS
     return 0;
 }
@@ -489,5 +560,31 @@ int or_verify_primal(const mc_lp* lp, const double* x, double claimed, double fa
     if (rep->objective_difference > or_allowance(rep->recomputed_objective, claimed, oa, orel)) viol++;
     rep->violation_count = viol;
     rep->passed = (viol == 0);
+    return 0;
+}
+
+/* Relative KKT score of an unscaled primal/dual pair (x[n], y[m] in the
+ * reported sign convention) on the original model: the residuals of
+ * pdlp.cpp:80-133 with A x and A^T y formed here.  out = {primal, dual, gap,
+ * score, objective}.  Used to check any returned solution independently. */
+int or_kkt(const mc_lp* lp, const double* x, const double* y, double* out) {
+    const int64_t m = lp->rows, n = lp->cols;
+    double* ax = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* aty = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* xo = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* yo = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    or_spmv(m, n, lp->column_start, lp->row_index, lp->value, x, ax);
+    or_spmv_t(m, n, lp->column_start, lp->row_index, lp->value, y, aty);
+    or_resid r;
+    or_residuals(lp, x, y, ax, aty, NULL, NULL, 0, xo, yo, &r);
+    out[0] = r.primal;
+    out[1] = r.dual;
+    out[2] = r.gap;
+    out[3] = r.score;
+    out[4] = r.objective;
+    free(ax);
+    free(aty);
+    free(xo);
+    free(yo);
     return 0;
 }

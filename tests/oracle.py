@@ -51,6 +51,7 @@ def oracle_lib():
                                       C.POINTER(Trace)]
         lib.or_verify_primal.argtypes = [C.POINTER(mp.LP), P_f64, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_double, C.POINTER(mp.VerifyReport)]
+        lib.or_kkt.argtypes = [C.POINTER(mp.LP), P_f64, P_f64, P_f64]
         _or = lib
     return _or
 
@@ -165,3 +166,13 @@ def verify(lp: mp.LPData, x, objective, tol=1e-4, use_ref=False) -> dict:
             "recomputed_objective": rep.recomputed_objective,
             "max_row_violation": rep.maximum_row_violation,
             "max_variable_violation": rep.maximum_variable_violation}
+
+
+def kkt(lp: mp.LPData, x, y) -> dict:
+    """Reference relative-KKT residuals of an unscaled (x, y) on the original model."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(5)
+    s = lp.struct()
+    oracle_lib().or_kkt(C.byref(s), _p(x, C.c_double), _p(y, C.c_double), _p(out, C.c_double))
+    return {"primal": out[0], "dual": out[1], "gap": out[2], "score": out[3], "objective": out[4]}

@@ -170,3 +170,40 @@ def test_graphs_and_eager_agree(gpu):
     for r in (b, c):
         assert r["iterations"] == a["iterations"] and r["objective"] == a["objective"]
         assert np.array_equal(r["primal"], a["primal"])
+
+
+KKT_CASES = [
+    ("tiny_simplex@1e-6", tiny_simplex, dict(tolerance=1e-6)),
+    ("blend@1e-6", blend, dict(tolerance=1e-6)),
+    ("uniform200x500@1e-6", small_uniform, dict(tolerance=1e-6, max_iterations=3000)),
+    ("mixed300x700@1e-6", small_mixed, dict(tolerance=1e-6, max_iterations=3000)),
+]
+
+
+@pytest.mark.parametrize("name,maker,kw", KKT_CASES, ids=[c[0] for c in KKT_CASES])
+def test_kkt_scheme_reference_order_bit_exact(gpu, name, maker, kw):
+    """KKT restart scheme (extension): GPU reference order == oracle, bit for bit."""
+    lp = maker()
+    print("kkt case", name, flush=True)
+    want = orc.solve(lp, restart_scheme=mp.SCHEME_KKT, **kw)
+    got = gpu_solve(lp, restart_scheme=mp.SCHEME_KKT, reduction_order=mp.ORDER_REFERENCE, **kw)
+    assert got["status"] == want["status"] and got["iterations"] == want["iterations"]
+    assert got["objective"] == want["objective"]
+    assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
+
+
+@pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED])
+def test_kkt_scheme_converges_to_1e6(gpu, precision):
+    """Fast order, FP64 or mixed: reaches 1e-6 on a planted LP; the returned pair
+    is re-scored independently by the oracle's FP64 residuals on the host, and its
+    objective matches the planted optimum c^T x* to 1e-5 relative (the gap
+    tolerance bounds |p - d| / (1 + |p| + |d|), not |p - p*| itself)."""
+    lp = small_uniform()  # the oracle converges on it in 8720 iterations (FP64, KKT scheme)
+    got = gpu_solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, precision=precision,
+                    max_iterations=200000)
+    assert got["status"] == "optimal", got["message"]
+    k = orc.kkt(lp, got["primal"], got["dual"])
+    assert k["score"] <= 1.5e-6, k
+    assert abs(got["objective"] - lp.planted_objective) <= 1e-5 * (1 + abs(lp.planted_objective))
+    if precision == mp.PRECISION_MIXED:
+        assert got["fp32_iterations"] > 0

@@ -1,0 +1,226 @@
+"""Host library, C-ABI and CLI -- CPU only (no compute on a GPU).
+
+* every function declared in include/*.h is exported by the built libraries;
+* Backend::cpu of the host library equals the oracle bit for bit (both restart
+  schemes), and a GPU request without a device reports "cpu_fallback"
+  (tests/regression_backend_actually_used.cpp:26-43) or fails with require_gpu;
+* the CLI keeps the reference's pdlp branch: JSON keys, exit codes,
+  10000 -> 100000 iteration-limit quirk, verification, stderr verdict;
+* MPS / binary LP round trips; generator determinism and planted optimality.
+"""
+import json
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import mcpdlp as mp
+import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(m[c]p?_\w+)\s*\(", text, flags=re.M)))
+
+
+def exported(lib_path):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True, text=True, check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+@pytest.mark.parametrize("header,lib", [("mcpdlp.h", mp.HOST_LIB), ("mcpdlp_engine.h", mp.ENGINE_LIB)])
+def test_every_declared_symbol_is_exported(header, lib):
+    names = declared_functions(header)
+    assert len(names) >= 8
+    missing = [n for n in names if n not in exported(lib)]
+    assert not missing, missing
+
+
+def test_libraries_load_without_gpu():
+    mp.host_lib()
+    mp.engine_lib()
+    assert mp.device_count() >= 0
+
+
+def tiny():
+    inf = np.inf
+    return mp.LPData.from_dense([[1, 1], [2, 1]], [-3, -2], [-inf, -inf], [4, 5], [0, 0], [inf, inf])
+
+
+CASES = [
+    (lambda: tiny(), dict(tolerance=1e-6)),
+    (lambda: mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11), dict(tolerance=1e-6, max_iterations=400)),
+    (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
+     dict(tolerance=1e-6, max_iterations=600)),
+    (lambda: mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, max_iterations=20000)),
+    (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, max_iterations=3000)),
+]
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_host_cpu_backend_equals_oracle(k):
+    make, kw = CASES[k]
+    lp = make()
+    a, b = mp.solve(lp, **kw), orc.solve(lp, **kw)
+    assert a["backend_actually_used"] == "cpu"
+    assert a["iterations"] == b["iterations"] and a["objective"] == b["objective"]
+    assert np.array_equal(a["primal"], b["primal"]) and np.array_equal(a["dual"], b["dual"])
+    assert (a["primal_infeasibility"], a["dual_infeasibility"], a["duality_gap"]) == \
+           (b["primal_infeasibility"], b["dual_infeasibility"], b["duality_gap"])
+
+
+def test_backend_labels_and_trivial():
+    lp = tiny()
+    assert mp.solve(lp, tolerance=1e-4)["backend_actually_used"] == "cpu"
+    if mp.device_count() == 0:
+        r = mp.solve(lp, tolerance=1e-4, backend=mp.BACKEND_GPU)
+        assert r["backend_actually_used"] == "cpu_fallback" and r["status"] == "optimal"
+        with pytest.raises(RuntimeError, match="GPU required"):
+            mp.solve(lp, tolerance=1e-4, backend=mp.BACKEND_GPU, require_gpu=1)
+    empty = mp.LPData(0, 0, np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0), np.zeros(0),
+                      np.zeros(0, mp.BOUND_DTYPE), np.zeros(0, mp.BOUND_DTYPE), np.zeros(0, mp.BOUND_DTYPE),
+                      np.zeros(0, mp.BOUND_DTYPE))
+    r = mp.solve(empty)
+    assert r["status"] == "optimal" and r["message"] == "trivial" and r["primal"].size == 0  # pdlp.cpp:144-149
+
+
+def test_iteration_limit_zero_reports_start_point():
+    lp = mp.generate(mp.FAMILY_UNIFORM_EQ, rows=50, cols=120, seed=2)
+    a, b = mp.solve(lp, max_iterations=0), orc.solve(lp, max_iterations=0)
+    assert a["status"] == b["status"] == "iteration_limit" and a["iterations"] == 0
+    assert a["objective"] == b["objective"]
+
+
+def test_invalid_model_is_an_error_code():
+    lp = tiny()
+    bad = lp.copy()
+    bad.row_index = bad.row_index.copy()
+    bad.row_index[0] = 99
+    with pytest.raises(RuntimeError, match="row index out of range"):
+        mp.solve(bad)
+
+
+def test_verify_matches_oracle_and_reference_semantics():
+    lp = tiny()
+    r = orc.solve(lp, tolerance=1e-6)
+    a = mp.verify(lp, r["primal"], r["objective"], 1e-6)
+    b = orc.verify(lp, r["primal"], r["objective"], 1e-6)
+    assert a["passed"] == b["passed"] and a["recomputed_objective"] == b["recomputed_objective"]
+    assert not mp.verify(lp, [3.0, 3.0], -15.0, 1e-6)["passed"]
+    with pytest.raises(RuntimeError, match="tolerances must be finite"):
+        mp.verify(lp, r["primal"], r["objective"], 1e-3)  # primal_verifier.cpp:39-43 rejects > 1e-4
+
+
+# ---- CLI ------------------------------------------------------------------------
+def run_cli(args):
+    p = subprocess.run([mp.CLI, *args], capture_output=True, text=True, timeout=120)
+    return p.returncode, json.loads(p.stdout.strip().splitlines()[-1]), p.stderr
+
+
+def test_cli_tiny_simplex_adr():
+    code, js, err = run_cli([os.path.join(GOLDEN, "tiny_simplex.mps"), "--engine", "pdlp", "--tolerance", "1e-6"])
+    assert code == 0 and js["status"] == "Optimal" and js["verified"]
+    assert round(js["objective"], 7) == -9.0000013 and js["backend"] == "cpu"
+    for key in ("relative_primal_residual", "relative_dual_residual", "relative_duality_gap", "h2d_ms", "kernel_ms",
+                "d2h_ms", "total_ms", "lp_iterations", "pdlp_tolerance", "primal"):
+        assert key in js
+    # json_number prints 17 significant digits (apps/json_output.hpp:69-78)
+    assert "Optimal [tol=9.9999999999999995e-07] [cpu total=" in err and "VERIFIED" in err
+
+
+def test_cli_exit_codes():
+    mps = os.path.join(GOLDEN, "tiny_simplex.mps")
+    code, js, _ = run_cli([mps, "--engine", "pdlp", "--tolerance", "1e-9", "--iteration-limit", "80"])
+    assert code == 6 and js["status"] == "IterationLimit" and js["lp_iterations"] == 80
+    with tempfile.NamedTemporaryFile("w", suffix=".mps", delete=False) as f:
+        f.write("NAME BAD\nROWS\n N COST\nCOLUMNS\n X1 NOPE 1.0\nENDATA\n")
+    code, js, _ = run_cli([f.name, "--engine", "pdlp"])
+    os.unlink(f.name)
+    assert code == 3 and js["status"] == "InvalidModel"
+    if mp.device_count() == 0:
+        code, js, err = run_cli([mps, "--engine", "pdlp", "--backend", "gpu", "--tolerance", "1e-6"])
+        assert code == 0 and js["backend_actually_used"] == "cpu_fallback" and "[gpu H2D=" in err
+
+
+def test_cli_iteration_limit_quirk():
+    """--iteration-limit 10000 (the simplex default) means 100000 (markov_cero_solve.cpp:141-144)."""
+    lp_path = os.path.join(GOLDEN, "tiny_simplex.mps")
+    code, js, _ = run_cli([lp_path, "--engine", "pdlp", "--tolerance", "1e-6", "--iteration-limit", "10000"])
+    assert code == 0 and js["lp_iterations"] == 200
+
+
+# ---- ingest -----------------------------------------------------------------------
+def test_mps_and_binary_round_trip(tmp_path):
+    lp = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=60, cols=150, nnz_max=12, seed=3)
+    ref = orc.solve(lp, tolerance=1e-6, max_iterations=400)
+    # the CLI reads the binary LP format and free MPS; both must give the oracle's run
+    path = tmp_path / "lp.mclp"
+    write_binary(lp, str(path))
+    code, js, _ = run_cli([str(path), "--engine", "pdlp", "--tolerance", "1e-6", "--iteration-limit", "400"])
+    assert js["lp_iterations"] == 400 and js["objective"] == ref["objective"]
+    mps_path = tmp_path / "lp.mps"
+    write_mps(lp, str(mps_path))
+    code, js2, _ = run_cli([str(mps_path), "--engine", "pdlp", "--tolerance", "1e-6", "--iteration-limit", "400"])
+    assert js2["objective"] == pytest.approx(ref["objective"], rel=1e-12)
+
+
+def write_binary(lp, path):
+    with open(path, "wb") as f:
+        f.write(b"MCLP0001")
+        np.array([lp.rows, lp.cols, lp.nnz], np.int64).tofile(f)
+        np.array([lp.sense, 0], np.int32).tofile(f)
+        np.array([lp.offset]).tofile(f)
+        for a in (lp.column_start.astype(np.int64), lp.row_index.astype(np.int64), lp.value, lp.objective,
+                  lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper):
+            a.tofile(f)
+
+
+def write_mps(lp, path):
+    A = lp.dense()
+    with open(path, "w") as f:
+        f.write("NAME RT\nROWS\n N COST\n")
+        for i in range(lp.rows):
+            lo, hi = lp.row_lower[i], lp.row_upper[i]
+            t = "E" if lo["kind"] == mp.FINITE and hi["kind"] == mp.FINITE and lo["value"] == hi["value"] else (
+                "L" if hi["kind"] == mp.FINITE else "G")
+            f.write(f" {t} R{i}\n")
+        f.write("COLUMNS\n")
+        for j in range(lp.cols):
+            f.write(f"    C{j} COST {float(lp.objective[j])!r}\n")
+            for i in np.nonzero(A[:, j])[0]:
+                f.write(f"    C{j} R{i} {float(A[i, j])!r}\n")
+        f.write("RHS\n")
+        for i in range(lp.rows):
+            hi = lp.row_upper[i]
+            v = hi["value"] if hi["kind"] == mp.FINITE else lp.row_lower[i]["value"]
+            f.write(f"    RHS R{i} {float(v)!r}\n")
+        f.write("BOUNDS\n")
+        for j in range(lp.cols):
+            f.write(f" UP BND C{j} {float(lp.var_upper[j]['value'])!r}\n")
+        f.write("ENDATA\n")
+
+
+def test_generators_deterministic_and_planted():
+    a = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=400, cols=900, nnz_max=50, seed=77)
+    b = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=400, cols=900, nnz_max=50, seed=77)
+    assert np.array_equal(a.value, b.value) and np.array_equal(a.row_index, b.row_index)
+    assert orc.kkt(a, a.x_star, -a.y_star)["score"] < 1e-11
+    eq = (a.row_lower["kind"] == mp.FINITE).mean()
+    assert 0.6 < eq < 0.8  # 70% equality rows
+    blk = mp.generate(mp.FAMILY_BLOCK_ANGULAR, rows=40, cols=100, periods=4, link_rows=30, seed=5)
+    assert blk.rows == 4 * 40 + 30 and blk.cols == 400 and blk.nnz == 400 * (8 + 2)
+    assert orc.kkt(blk, blk.x_star, -blk.y_star)["score"] < 1e-11
+    base, fam = mp.generate_family(dict(rows=100, cols=250, nnz_min=6, nnz_max=6), seeds=[1, 2, 3])
+    for q in fam:
+        assert np.array_equal(q.row_index, base.row_index) and np.array_equal(q.value, base.value)
+        assert not np.array_equal(q.objective, base.objective)
+        assert orc.kkt(q, q.x_star, -q.y_star)["score"] < 1e-11

@@ -1,0 +1,205 @@
+"""Pins the oracle (oracle/pdlp_oracle.c) -- CPU only.
+
+Each check is chosen so a plausible restatement mistake (dropped term, wrong
+sign/index, transposed operand, changed operation order) fails it:
+* golden fixtures written by the compiled reference (tests/golden): iteration
+  counts, objective/residual bits, primal/dual bits, Ruiz scaler hashes;
+* the compiled reference itself, live, where oracle/_ref is built;
+* closed forms: dense long-double products, exact integer SpMV, the Ruiz
+  identity A_s = D_r A D_c and its convergence property, a diagonal matrix
+  equilibrated in one pass, zero KKT at the planted pair, hand residuals;
+* known answers held by the reference: ADR-P3-05 (tiny_simplex @1e-6),
+  tests/pdlp_test.cpp objectives, verify_primal verdicts.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import mcpdlp as mp
+import oracle as orc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_inputs():
+    z = np.load(os.path.join(GOLDEN, "pdlp_inputs.npz"))
+    names = sorted({k.split(".")[0] for k in z.files})
+    out = {}
+    for nm in names:
+        m, n, sense = (int(v) for v in z[f"{nm}.meta"])
+        b = {}
+        for k in ("row_lower", "row_upper", "var_lower", "var_upper"):
+            b[k] = mp.LPData.bounds(z[f"{nm}.{k}.kind"], z[f"{nm}.{k}.value"])
+        out[nm] = mp.LPData(m, n, z[f"{nm}.column_start"].astype(np.int64), z[f"{nm}.row_index"].astype(np.int64),
+                            z[f"{nm}.value"], z[f"{nm}.objective"], b["row_lower"], b["row_upper"],
+                            b["var_lower"], b["var_upper"], sense, float(z[f"{nm}.offset"][0]))
+    return out
+
+
+INPUTS = load_inputs()
+with open(os.path.join(GOLDEN, "pdlp_golden.json")) as _f:
+    GOLD = json.load(_f)
+
+
+def unhex(xs):
+    return np.array([float.fromhex(v) for v in xs])
+
+
+@pytest.mark.parametrize("run", GOLD["runs"], ids=[f"{r['case']}-{i}" for i, r in enumerate(GOLD["runs"])])
+def test_oracle_matches_reference_golden(run):
+    lp = INPUTS[run["case"]]
+    got = orc.solve(lp, **run["options"])
+    assert got["status"] == run["status"]
+    assert got["iterations"] == run["iterations"]
+    assert got["objective"] == float.fromhex(run["objective"])
+    assert got["primal_infeasibility"] == float.fromhex(run["primal_infeasibility"])
+    assert got["dual_infeasibility"] == float.fromhex(run["dual_infeasibility"])
+    assert got["duality_gap"] == float.fromhex(run["duality_gap"])
+    assert np.array_equal(got["primal"], unhex(run["primal"]))
+    assert np.array_equal(got["dual"], unhex(run["dual"]))
+    assert got["message"] == run["message"] and got["backend_actually_used"] == run["backend_actually_used"]
+    if run["verify_1e-4"] is not None:
+        v = orc.verify(lp, got["primal"], got["objective"], tol=1e-4)
+        assert v["passed"] == run["verify_1e-4"]["passed"]
+        assert v["recomputed_objective"] == float.fromhex(run["verify_1e-4"]["recomputed_objective"])
+
+
+@pytest.mark.parametrize("name", sorted(GOLD["ruiz"]))
+def test_oracle_ruiz_matches_reference_golden(name):
+    import hashlib
+
+    q = orc.equilibrate(INPUTS[name], 10)
+    g = GOLD["ruiz"][name]
+    h = lambda a: hashlib.sha256(np.ascontiguousarray(a, np.float64).tobytes()).hexdigest()
+    assert q["passes"] == g["passes"] and q["converged"] == g["converged"]
+    assert h(q["lp"].value) == g["values_sha256"]
+    assert h(q["row_scale"]) == g["row_scale_sha256"] and h(q["col_scale"]) == g["col_scale_sha256"]
+
+
+@pytest.mark.skipif(not orc.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_matches_compiled_reference_live(seed):
+    lp = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=150 + 20 * seed, cols=400, nnz_max=30, seed=seed)
+    for kw in (dict(tolerance=1e-6, max_iterations=300), dict(tolerance=1e-3, max_iterations=20000)):
+        a, b = orc.solve(lp, **kw), orc.ref_solve(lp, **kw)
+        assert a["iterations"] == b["iterations"] and a["objective"] == b["objective"]
+        assert np.array_equal(a["primal"], b["primal"]) and np.array_equal(a["dual"], b["dual"])
+    qa, qb = orc.equilibrate(lp, 10), orc.equilibrate(lp, 10, use_ref=True)
+    assert np.array_equal(qa["lp"].value, qb["lp"].value)
+    assert np.array_equal(qa["row_scale"], qb["row_scale"]) and np.array_equal(qa["col_scale"], qb["col_scale"])
+
+
+# ---- closed forms -----------------------------------------------------------
+def random_lp(seed, m=7, n=5, integer=False):
+    rng = np.random.default_rng(seed)
+    A = np.where(rng.random((m, n)) < 0.45, rng.integers(-4, 5, (m, n)) if integer else rng.normal(size=(m, n)), 0.0)
+    inf = np.inf
+    return mp.LPData.from_dense(A, rng.normal(size=n), [-inf] * m, [inf] * m, [0] * n, [inf] * n), A
+
+
+@pytest.mark.parametrize("seed", range(25))
+def test_spmv_against_dense_long_double(seed):
+    lp, A = random_lp(seed)
+    rng = np.random.default_rng(1000 + seed)
+    x = rng.normal(size=lp.cols)
+    x[rng.random(lp.cols) < 0.3] = 0.0
+    y = rng.normal(size=lp.rows)
+    Al = A.astype(np.longdouble)
+    np.testing.assert_allclose(orc.spmv(lp, x), (Al @ x.astype(np.longdouble)).astype(float), rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(orc.spmv_t(lp, y), (Al.T @ y.astype(np.longdouble)).astype(float), rtol=1e-14,
+                               atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_spmv_exact_on_integers(seed):
+    lp, A = random_lp(seed, 9, 6, integer=True)
+    x = np.arange(lp.cols, dtype=float) - 2.0
+    y = np.arange(lp.rows, dtype=float) * 3 - 7
+    assert np.array_equal(orc.spmv(lp, x), A @ x)
+    assert np.array_equal(orc.spmv_t(lp, y), A.T @ y)
+
+
+def test_ruiz_identity_and_convergence():
+    lp = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=200, cols=500, nnz_max=30, seed=4)
+    q = orc.equilibrate(lp, 60)
+    cols = np.repeat(np.arange(lp.cols), np.diff(lp.column_start))
+    # A_s = D_r A D_c (product of per-pass deltas; rounding of up to 60 products)
+    np.testing.assert_allclose(q["lp"].value, lp.value * q["row_scale"][lp.row_index] * q["col_scale"][cols],
+                               rtol=1e-12)
+    assert q["converged"]
+    a = np.abs(q["lp"].value)
+    rmax = np.zeros(lp.rows)
+    np.maximum.at(rmax, lp.row_index, a)
+    cmax = np.maximum.reduceat(a, lp.column_start[:-1])
+    assert np.all(np.abs(rmax[rmax > 0] - 1) < 1e-3) and np.all(np.abs(cmax - 1) < 1e-3)
+    # bounds scaled by D_r (rows) and D_c^-1 (columns), objective by D_c
+    fin = lp.var_upper["kind"] == mp.FINITE
+    np.testing.assert_array_equal(q["lp"].var_upper["value"][fin],
+                                  lp.var_upper["value"][fin] * (1.0 / q["col_scale"][fin]))
+    np.testing.assert_array_equal(q["lp"].objective, lp.objective * q["col_scale"])
+
+
+def test_ruiz_diagonal_one_pass():
+    d = np.array([4.0, 0.25, 9.0, 1.0])
+    lp = mp.LPData.from_dense(np.diag(d), np.ones(4), [1] * 4, [1] * 4, [0] * 4, [np.inf] * 4)
+    q = orc.equilibrate(lp, 10)
+    np.testing.assert_allclose(np.abs(q["lp"].value), 1.0, rtol=1e-15)
+    assert q["passes"] == 1 and q["converged"]
+
+
+def test_residuals_zero_at_planted_pair():
+    for fam, kw in ((mp.FAMILY_UNIFORM_EQ, dict(rows=300, cols=800)),
+                    (mp.FAMILY_ZIPF_MIXED, dict(rows=300, cols=800, nnz_max=50))):
+        lp = mp.generate(fam, seed=9, **kw)
+        k = orc.kkt(lp, lp.x_star, -lp.y_star)  # reported dual = -y* (g = c + A^T y convention)
+        assert k["score"] < 1e-11, k
+        assert abs(k["objective"] - lp.planted_objective) <= 1e-12 * abs(lp.planted_objective)
+        bad = orc.kkt(lp, lp.x_star, lp.y_star)  # wrong sign must be caught
+        assert bad["score"] > 1e-3
+
+
+def test_residuals_hand_case():
+    # min x1 + 2 x2  s.t.  x1 + x2 >= 2 (row 0), x1 <= 3 (row 1);  x1 in [0, inf), x2 in [0, 1]
+    inf = np.inf
+    lp = mp.LPData.from_dense([[1, 1], [1, 0]], [1, 2], [2, -inf], [inf, 3], [0, 0], [inf, 1])
+    # x = (1, 0.5): row0 = 1.5 < 2 -> violation 0.5 / (1 + 2) ; y = (-1, 0)
+    k = orc.kkt(lp, [1.0, 0.5], [-1.0, 0.0])
+    assert k["primal"] == pytest.approx(0.5 / 3.0, rel=1e-15)
+    # g = c + A^T y = (0, 1): x2 - clamp(x2 - 1, 0, 1) = 0.5 -> dual = 0.5 / max(1, |c|inf=2)
+    assert k["dual"] == pytest.approx(0.25, rel=1e-15)
+    # p = 1*1 + 2*0.5 = 2; d = -(-1)*2 [y<0 uses row lower] + g2>0 * lo2 = 2 ; gap 0
+    assert k["objective"] == 2.0 and k["gap"] == 0.0
+    # maximize quirk (pdlp.cpp:113-131): primal objective unsigned, dual signed
+    lpm = mp.LPData.from_dense([[1.0]], [1.0], [-inf], [1.0], [0], [1.0], sense=mp.MAXIMIZE)
+    km = orc.kkt(lpm, [1.0], [1.0])
+    assert km["objective"] == 1.0 and km["gap"] == pytest.approx(2.0 / 3.0)
+
+
+# ---- known answers held by the reference ------------------------------------
+def test_adr_p3_05_tiny_simplex():
+    """docs/decisions/ADR-P3-05-pdhg-pdlp.md: @1e-6 Optimal, obj ~ -9.0000013, x ~ (1.000008, 2.999989)."""
+    r = orc.solve(INPUTS["tiny_simplex"], tolerance=1e-6)
+    assert r["status"] == "optimal" and r["message"] == "PDLP converged"
+    assert round(r["objective"], 7) == -9.0000013
+    assert r["primal"][0] == pytest.approx(1.000008, abs=5e-7) and r["primal"][1] == pytest.approx(2.999989, abs=5e-7)
+    assert orc.verify(INPUTS["tiny_simplex"], r["primal"], r["objective"], 1e-6)["passed"]
+
+
+def test_pdlp_test_objectives():
+    """tests/pdlp_test.cpp:62 (-7 +- 0.1) and :119 (5 +- 0.1), verify at 1e-4."""
+    for name, want in (("blend", -7.0), ("equality", 5.0)):
+        r = orc.solve(INPUTS[name], tolerance=1e-4, max_iterations=200000)
+        assert abs(r["objective"] - want) < 0.1
+        assert r["status"] == "optimal" and orc.verify(INPUTS[name], r["primal"], r["objective"], 1e-4)["passed"]
+
+
+def test_kkt_scheme_reaches_planted_optimum():
+    """Extension scheme in the oracle: converges to tolerance, objective = planted optimum."""
+    lp = mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11)
+    r = orc.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, max_iterations=100000)
+    assert r["status"] == "optimal"
+    k = orc.kkt(lp, r["primal"], r["dual"])
+    assert k["score"] <= 1e-6 * 1.01
+    assert abs(r["objective"] - lp.planted_objective) <= 1e-5 * (1 + abs(lp.planted_objective))
