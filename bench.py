@@ -29,11 +29,17 @@ sys.path.insert(0, os.path.join(ROOT, "markov-zip1_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+METRIC = "time to 1e-6 rel KKT + SpMV HBM GB/s; mixed/batched/warm vs plain PDLP, 1 GPU"
+
 WORKLOADS = {
     "large": dict(family=2, spec={}, tol=1e-6,
                   name="single LP m=200000 n=500000, Zipf(1.2) nnz/col in [1,200], 70% '=' rows"),
     "cpu-lp": dict(family=1, spec={}, tol=1e-6,
                    name="single LP m=2000 n=5000, 5 nnz/col, '=' rows, 0<=x<=10"),
+    "batch": dict(tol=1e-6, name="256 planted LPs, m log-uniform [500,20000], n=2.5m, 4-8 nnz/col, "
+                                 "box [0,U[1,100]], one batched solve"),
+    "warm": dict(tol=1e-6, name="LP family on one structure m=2000 n=5000 6 nnz/col, b/c/u perturbed "
+                                "(lognormal 0.1, U[0.8,1.2]); learned warm start vs cold"),
 }
 
 
@@ -53,6 +59,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="approximate CPU sample budget")
+    ap.add_argument("--graph-chunk", type=int, default=0, help="iterations per HIP graph (<0: eager launches)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=16, help="batch: instances solved one by one (FP64)")
+    ap.add_argument("--train", type=int, default=384, help="warm: training instances")
+    ap.add_argument("--test", type=int, default=64, help="warm: test instances")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the batched and "
@@ -64,6 +76,10 @@ def main():
 
     if mp.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "warm":
+        return bench_warm(args)
     wl = WORKLOADS[args.workload]
     tol = args.tolerance or wl["tol"]
     t = time.time()
@@ -72,7 +88,8 @@ def main():
 
     prec = mp.PRECISION_MIXED if args.precision == "mixed" else mp.PRECISION_FP64
     opts = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=prec,
-                      restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations)
+                      restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations,
+                      graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
 
     def run(o):
         t0 = time.perf_counter()
@@ -104,7 +121,8 @@ def main():
     plain = None
     if not args.no_plain:
         po = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_FP64,
-                        restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations)
+                        restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations,
+                        graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
         pr, pwall = run(po)
         plain = {"ms": round(pwall, 3), "status": pr["status"], "iterations": pr["iterations"],
                  "kernel_ms": round(pr["kernel_ms"], 3)}
@@ -113,7 +131,7 @@ def main():
     # roofline of the fused iteration kernels (device events on the engine stream)
     eng = mp.Engine(0)
     eng.load(lp)
-    eng.setup()
+    eng.setup(autotune=not args.no_autotune)
     kern = {}
     for name in ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64", "spmv_fp32", "spmv_t_fp32"):
         us, b = eng.time_kernel(name, reps=50)
@@ -150,7 +168,7 @@ def main():
         log(f"[bench] cpu baseline: {ms_it:.2f} ms/iteration ({cpu['kind']})")
 
     line = {
-        "metric": "time to 1e-6 rel KKT + SpMV HBM GB/s; mixed/batched/warm vs plain PDLP, 1 GPU",
+        "metric": METRIC,
         "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "fp32+fp64" if prec == mp.PRECISION_MIXED else "fp64", "data": "synthetic (planted-feasible, seeded)",
@@ -167,6 +185,114 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def _line(value_ms, workload, cfg, extra, steps=1, warmup=0):
+    line = {"metric": METRIC, "value": round(value_ms, 3), "unit": "ms", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(value_ms, 3), "higher_is_better": False, "scaling": "strong",
+            "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic (planted-feasible, seeded)",
+            "config": {"workload": workload, **cfg, "parallelism": "single GPU"}}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """Batched persistent-launch PDLP vs the plain FP64 GPU PDLP run instance by instance."""
+    import numpy as np
+    import mcpdlp as mp
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as orc
+
+    tol = args.tolerance or 1e-6
+    t = time.time()
+    lps = mp.generate_batch(args.batch_count)
+    log(f"[bench] batch of {len(lps)}: sum m={sum(l.rows for l in lps)} sum nnz={sum(l.nnz for l in lps)} "
+        f"generated in {time.time() - t:.1f}s")
+    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=mp.SCHEME_KKT,
+              max_iterations=args.max_iterations, graph_chunk=args.graph_chunk)
+    t0 = time.perf_counter()
+    res = mp.solve_batch(lps, precision=mp.PRECISION_MIXED, **kw)
+    wall = (time.perf_counter() - t0) * 1e3
+    its = np.array([r["iterations"] for r in res])
+    ok = sum(r["status"] == "optimal" for r in res)
+    log(f"[bench] batched mixed: {ok}/{len(res)} optimal, iterations mean={its.mean():.0f} max={its.max()} "
+        f"wall={wall:.1f}ms kernel={res[0]['kernel_ms']:.1f}ms")
+    # independent FP64 re-score of a few instances
+    worst = 0.0
+    for k in range(0, len(lps), max(1, len(lps) // 8)):
+        sc = orc.kkt(lps[k], res[k]["primal"], res[k]["dual"])["score"]
+        worst = max(worst, sc)
+    # plain FP64 GPU PDLP, one instance at a time, on an evenly spaced sample by size
+    order = np.argsort([l.rows for l in lps])
+    sample = order[np.linspace(0, len(lps) - 1, min(args.plain_sample, len(lps))).astype(int)]
+    plain_ms, plain_it = [], []
+    for k in sample:
+        t1 = time.perf_counter()
+        r = mp.solve(lps[k], precision=mp.PRECISION_FP64, **kw)
+        plain_ms.append((time.perf_counter() - t1) * 1e3)
+        plain_it.append(r["iterations"])
+        log(f"[bench] plain FP64 instance m={lps[k].rows}: {r['status']} it={r['iterations']} {plain_ms[-1]:.1f}ms")
+    est = float(np.mean(plain_ms)) * len(lps)
+    _line(wall, WORKLOADS["batch"]["name"],
+          {"instances": len(lps), "sum_rows": int(sum(l.rows for l in lps)), "sum_nnz": int(sum(l.nnz for l in lps)),
+           "tolerance": tol, "precision": "mixed", "restart_scheme": "kkt"},
+          {"optimal": int(ok), "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+           "kkt_fp64_rescored_worst_of_sample": float(f"{worst:.4e}"),
+           "plain_fp64_per_instance": {"sampled": int(len(sample)), "mean_ms": round(float(np.mean(plain_ms)), 3),
+                                       "estimated_total_ms": round(est, 1),
+                                       "iterations_mean": float(np.mean(plain_it)),
+                                       "note": "evenly spaced by size; total = mean x instances (extrapolated)"},
+           "speedup_vs_plain_fp64_per_instance": round(est / wall, 2)})
+
+
+def bench_warm(args):
+    """Learned warm start (BF16 MLP on MFMA) vs cold start on a family of perturbed LPs."""
+    import numpy as np
+    import mcpdlp as mp
+    import warmstart
+
+    tol = args.tolerance or 1e-6
+    ntr, nte = args.train, args.test
+    t = time.time()
+    base, fam = mp.generate_family(dict(rows=2000, cols=5000, nnz_min=6, nnz_max=6),
+                                   seeds=[0x4D4350444C50 + 4000 + k for k in range(ntr + nte)], sigma=0.1)
+    train, test = fam[:ntr], fam[ntr:]
+    log(f"[bench] family: {ntr} train + {nte} test instances of {base.rows}x{base.cols} "
+        f"({base.nnz} nnz) in {time.time() - t:.1f}s")
+    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=mp.SCHEME_KKT,
+              max_iterations=args.max_iterations, precision=mp.PRECISION_MIXED, graph_chunk=args.graph_chunk)
+    t0 = time.perf_counter()
+    sols = []
+    for k in range(0, ntr, 128):
+        sols += mp.solve_batch(train[k:k + 128], **kw)
+    gen_ms = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] training solutions: {sum(r['status'] == 'optimal' for r in sols)}/{ntr} optimal in {gen_ms:.0f}ms")
+    ws = warmstart.WarmStart(base)
+    fit = ws.fit(train, [r["primal"] for r in sols], [r["dual"] for r in sols])
+    log(f"[bench] MLP fit: {fit}")
+    t1 = time.perf_counter()
+    cold = mp.solve_batch(test, **kw)
+    cold_ms = (time.perf_counter() - t1) * 1e3
+    t2 = time.perf_counter()
+    pred = ws.predict(test)
+    warm = mp.solve_batch(test, warm_primal=np.concatenate([p[0] for p in pred]),
+                          warm_dual=np.concatenate([p[1] for p in pred]), **kw)
+    warm_ms = (time.perf_counter() - t2) * 1e3
+    ci = np.array([r["iterations"] for r in cold])
+    wi = np.array([r["iterations"] for r in warm])
+    log(f"[bench] cold: it mean={ci.mean():.0f} max={ci.max()} {cold_ms:.0f}ms | warm: it mean={wi.mean():.0f} "
+        f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference)")
+    _line(warm_ms, WORKLOADS["warm"]["name"],
+          {"train": ntr, "test": nte, "m": base.rows, "n": base.cols, "nnz": base.nnz, "tolerance": tol,
+           "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": "kkt"},
+          {"cold_ms": round(cold_ms, 1), "warm_ms": round(warm_ms, 1),
+           "iterations_cold_mean": float(ci.mean()), "iterations_warm_mean": float(wi.mean()),
+           "iterations_cold_max": int(ci.max()), "iterations_warm_max": int(wi.max()),
+           "optimal_cold": int(sum(r["status"] == "optimal" for r in cold)),
+           "optimal_warm": int(sum(r["status"] == "optimal" for r in warm)),
+           "speedup_warm_vs_cold": round(cold_ms / warm_ms, 3), "fit": fit,
+           "training_solutions_ms": round(gen_ms, 1)})
 
 
 if __name__ == "__main__":

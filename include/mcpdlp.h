@@ -112,12 +112,12 @@ typedef struct mc_pdlp_options {
     double initial_primal_weight;     /* 0.0 => automatic */
     double primal_weight_smoothing;   /* 0.5 */
     int32_t ruiz_scaling;             /* 1 */
-    int32_t pad0_;
+    int32_t autotune;                 /* extension: time SpMV tile shapes on the device first (0) */
     uint64_t ruiz_iterations;         /* 10 */
     /* --- extensions (HIP engine only) --- */
     int32_t precision;                /* MC_PRECISION_FP64 */
     int32_t reduction_order;          /* MC_ORDER_FAST */
-    double polish_threshold;          /* mixed: switch to FP64 once the FP64-checked score <= this (0 => 10 x tolerance) */
+    double polish_threshold;          /* mixed: switch to FP64 once the FP64-checked score <= this (0 => only on stall) */
     int32_t graph_chunk;              /* iterations per captured HIP graph (0 => 64; < 0 => eager launches) */
     int32_t device;                   /* HIP device ordinal */
     int32_t require_gpu;              /* 1: a GPU request fails with MC_ERR_DEVICE instead of the cpu_fallback path */

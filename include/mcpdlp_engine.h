@@ -149,6 +149,11 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
  * the next mcp_solve re-initializes it. */
 int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes);
 
+/* Pin the tile policies of the loaded problem (rows of A, rows of A^T) and
+ * re-tile; the autotuner's choice is overwritten.  For tuning experiments and
+ * for callers that know their matrix. */
+int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const mcp_tile_policy* cols_policy);
+
 /* Current tile policies (after mcp_setup / autotune) and tile counts. */
 int mcp_get_tiling(mcp_engine* e, mcp_tile_policy* rows_policy, mcp_tile_policy* cols_policy,
                    int32_t* row_tiles, int32_t* col_tiles);

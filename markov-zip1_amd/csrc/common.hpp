@@ -5,27 +5,25 @@
 //   CSR(A)   rows = constraints: rp[m+1], ci[nnz] (int32, global column ids)
 //   CSR(A^T) rows = variables:   cp[n+1], ri[nnz] (int32, global row ids)
 //   values in FP64 (checks, FP64 iterations) and FP32 (mixed-precision iterations)
-// Work is cut into Tiles: runs of consecutive rows of one matrix that belong to
-// one instance and whose nonzeros fit the LDS staging buffer (or one long row).
+// Work is cut into Tiles, each the task of ONE wave64: a run of consecutive
+// rows of one matrix and one instance, 2^lg lanes per row.  The host orders
+// rows by length bucket (fast order), so a tile's rows have similar lengths.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace mcp {
 
-constexpr int kBlock = 256;        // threads per workgroup: 4 wave64s
+constexpr int kBlock = 256;        // threads per workgroup: 4 wave64s, 4 tiles
 constexpr int kWaves = kBlock / 64;
-constexpr int kNnzTile = 2048;     // LDS staging capacity (products) per tile
+constexpr int kFinal = 1024;       // threads of the per-instance finalize kernels
+constexpr int kMaxSteps = 8;       // row groups one wave sweeps per tile
 constexpr int kXcd = 8;            // MI355X: 8 XCDs, workgroups dealt round-robin
-
-// Tile.shape: bits 0..2 = log2(lanes per row in the reduce phase, 1..64);
-// bit 8 = long-row tile (a single row with more than kNnzTile nonzeros).
-constexpr int32_t kShapeLong = 1 << 8;
 
 struct Tile {
     int32_t r0, r1;  // rows [r0, r1)
     int32_t inst;    // instance id
-    int32_t shape;
+    int32_t shape;   // bits 0..2: lg = log2(lanes per row), 0..6
 };
 
 // Instance status values.
@@ -52,7 +50,7 @@ struct InstState {
     int32_t restart_flag;    // set by a check, consumed by k_post_check
     int32_t switch_flag;     // set by a check: convert FP32 iterate to FP64
     int32_t mode_at_check;   // mode in which the last check's iterate lives
-    int32_t stall;           // checks without a 10% score improvement (FP32)
+    int32_t stall;           // FP32: checks since the last new best score
     int32_t cand_cur;        // KKT scheme: this check's candidate is the current iterate
     int32_t answer_cur;      // the reported iterate is the current one (else the average)
     double prev_cand;        // KKT scheme: candidate score at the previous check of the epoch
@@ -77,13 +75,15 @@ struct IterOpts {
     int32_t pad_;
 };
 
-// Map a workgroup id to a tile so that each XCD works on a contiguous range
-// of tiles (neighbouring tiles touch neighbouring vector segments, which then
-// share that XCD's L2).  grid = kXcd * per, per = ceil(ntiles / kXcd).
-__device__ __forceinline__ int xcd_tile(int ntiles) {
-    const int per = (ntiles + kXcd - 1) / kXcd;
+// Map (workgroup, wave) to a tile so that each XCD works on a contiguous
+// range of tiles (neighbouring tiles touch neighbouring vector segments,
+// which then share that XCD's L2).  grid = kXcd * per workgroups,
+// per = ceil(ceil(ntiles / kWaves) / kXcd).
+__device__ __forceinline__ int wave_tile(int ntiles) {
+    const int nblk = (ntiles + kWaves - 1) / kWaves;
+    const int per = (nblk + kXcd - 1) / kXcd;
     const int b = blockIdx.x;
-    return (b % kXcd) * per + b / kXcd;
+    return ((b % kXcd) * per + b / kXcd) * kWaves + (threadIdx.x >> 6);
 }
 
 template <typename T>
@@ -108,10 +108,11 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
-// Deterministic block reductions: waves reduce by xor-shuffle, wave results
-// are combined in wave order by thread 0.  Result valid in thread 0.
-template <typename T>
-__device__ __forceinline__ T block_sum(T v, T* scratch /* >= kWaves */) {
+// Deterministic block reductions over NW waves: waves reduce by
+// xor-shuffle, wave results are combined in wave order by thread 0.  Result
+// valid in thread 0; every thread of the block must call.
+template <int NW, typename T>
+__device__ __forceinline__ T block_sum(T v, T* scratch /* >= NW */) {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __syncthreads();
@@ -119,10 +120,10 @@ __device__ __forceinline__ T block_sum(T v, T* scratch /* >= kWaves */) {
     __syncthreads();
     T r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kWaves; ++i) r += scratch[i];
+        for (int i = 0; i < NW; ++i) r += scratch[i];
     return r;
 }
-template <typename T>
+template <int NW, typename T>
 __device__ __forceinline__ T block_max(T v, T* scratch) {
     v = wave_max(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -131,64 +132,39 @@ __device__ __forceinline__ T block_max(T v, T* scratch) {
     __syncthreads();
     T r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kWaves; ++i) r = maxv(r, scratch[i]);
+        for (int i = 0; i < NW; ++i) r = maxv(r, scratch[i]);
     return r;
 }
 
-// Row sums of one tile of a CSR matrix:
-//   rsum[r - tl.r0] = sum_{p in row r} val[p] * vec[idx[p]]
-// Regular tiles stream the tile's nonzeros once, fully coalesced, into LDS
-// (phase 1) and then sum each row from LDS with 2^shape lanes per row
-// (phase 2).  With 1 lane per row the sum runs in increasing column order,
-// one rounding per product and per add -- the reference's serial order.
-// Long tiles (one row) stride the row with the whole block and reduce
-// (EXACT: thread 0 sums serially instead).  Returns the tile's row count;
-// rsum is valid for all threads after the call.
-template <bool EXACT, typename TS, typename TA, typename TV>
-__device__ __forceinline__ int tile_rowsums(const Tile& tl, const int32_t* __restrict__ rp,
-                                           const int32_t* __restrict__ idx,
-                                           const TA* __restrict__ val,
-                                           const TV* __restrict__ vec, TS* prod, TS* rsum,
-                                           TS* red) {
-    const int tid = threadIdx.x;
-    const int p0 = rp[tl.r0];
-    const int p1 = rp[tl.r1];
-    if (tl.shape & kShapeLong) {
-        TS s = 0;
-        if (EXACT) {
-            if (tid == 0)
-                for (int p = p0; p < p1; ++p) {
-                    const TS pr = (TS)val[p] * (TS)vec[idx[p]];
-                    s += pr;
-                }
-            __syncthreads();
-            if (tid == 0) rsum[0] = s;
-        } else {
-            for (int p = p0 + tid; p < p1; p += kBlock) {
-                const TS pr = (TS)val[p] * (TS)vec[idx[p]];
-                s += pr;
-            }
-            s = block_sum(s, red);
-            if (tid == 0) rsum[0] = s;
-        }
-        __syncthreads();
-        return 1;
-    }
-    for (int p = p0 + tid; p < p1; p += kBlock) prod[p - p0] = (TS)val[p] * (TS)vec[idx[p]];
-    __syncthreads();
-    const int lg = tl.shape & 7;
-    const int G = 1 << lg;
-    const int lane = tid & (G - 1);
-    const int row = tl.r0 + (tid >> lg);
+// Dot product of one CSR row [p0, p1) with a gathered vector, by the G lanes
+// of a row group (gl = lane within the group).  Each lane keeps four
+// (index, value) loads and four gathers in flight; the group then reduces by
+// xor-shuffle.  With G = 1 the lane adds the products in increasing index
+// order, one rounding per product and per add -- the reference's serial
+// order.  Every lane of the wave must call (the shuffles need their partners).
+template <typename TS, typename TA, typename TV>
+__device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const int32_t* __restrict__ idx,
+                                      const TA* __restrict__ val, const TV* __restrict__ vec) {
     TS s = 0;
-    if (row < tl.r1) {
-        const int q1 = rp[row + 1] - p0;
-        for (int q = rp[row] - p0 + lane; q < q1; q += G) s += prod[q];
+    int q = p0 + gl;
+    for (; q + 3 * G < p1; q += 4 * G) {
+        const int i0 = idx[q], i1 = idx[q + G], i2 = idx[q + 2 * G], i3 = idx[q + 3 * G];
+        const TA a0 = val[q], a1 = val[q + G], a2 = val[q + 2 * G], a3 = val[q + 3 * G];
+        const TS t0 = (TS)a0 * (TS)vec[i0];
+        const TS t1 = (TS)a1 * (TS)vec[i1];
+        const TS t2 = (TS)a2 * (TS)vec[i2];
+        const TS t3 = (TS)a3 * (TS)vec[i3];
+        s += t0;
+        s += t1;
+        s += t2;
+        s += t3;
+    }
+    for (; q < p1; q += G) {
+        const TS t = (TS)val[q] * (TS)vec[idx[q]];
+        s += t;
     }
     for (int off = G >> 1; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0 && row < tl.r1) rsum[tid >> lg] = s;
-    __syncthreads();
-    return tl.r1 - tl.r0;
+    return s;
 }
 
 }  // namespace mcp

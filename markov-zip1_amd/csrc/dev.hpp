@@ -76,18 +76,18 @@ struct Dev {
 };
 
 // ---- launch helpers (kernels.hip) ----
-// Iteration kernels.  `t` is the iteration's offset inside the chunk;
-// FP32 = iterate precision; EXACT = reference reduction order.
-void launch_primal(const Dev& d, int t, bool fp32, bool adapt, bool exact, hipStream_t s);
-void launch_dual(const Dev& d, int t, bool fp32, bool exact, hipStream_t s);
-void launch_adx(const Dev& d, int t, bool fp32, bool exact, hipStream_t s);
+// Iteration kernels (each instance runs in its own FP32/FP64 mode).  `t` is
+// the iteration's offset inside the chunk; exact = reference reduction order.
+void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s);
+void launch_dual(const Dev& d, int t, bool exact, hipStream_t s);
+void launch_adx(const Dev& d, int t, bool exact, hipStream_t s);
 void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s);
-void launch_resid(const Dev& d, bool fp32, bool exact, bool current, hipStream_t s);
+void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s);
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
                            hipStream_t s);
 void launch_post_check(const Dev& d, bool kkt, hipStream_t s);
 void launch_advance(const Dev& d, int len, hipStream_t s);
-// Setup kernels.
+// Setup kernels (run on the current tiles).
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);
 void launch_setup_data(const Dev& d, hipStream_t s);
 void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm_y, hipStream_t s);
@@ -95,9 +95,9 @@ void launch_setup_state(const Dev& d, double initial_weight, double step_reducti
                         hipStream_t s);
 void launch_fill(double* p, double v, int64_t n, hipStream_t s);
 void launch_unscale(const Dev& d, double* x_out, double* y_out, hipStream_t s);
-// Plain SpMV over a tile set: out[r] = sum val * vec (tests and micro-benchmarks).
+// Plain SpMV over a tile set: out[r] = sum val * vec (tests and micro-benchmarks);
+// tiles with one lane per row sum in the reference's serial order.
 void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const int32_t* idx,
-                       const void* val, const void* vec, void* out, bool fp32, bool exact,
-                       hipStream_t s);
+                       const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
 
 }  // namespace mcp

@@ -65,14 +65,15 @@ int lanes_log2(int len, int per_lane) {
     return lg;
 }
 
-// Cut rows [inst[b], inst[b+1]) of every instance into tiles: consecutive
-// rows whose nonzeros fit `budget`, at most kBlock >> lg rows (lg = log2 of
-// lanes per row, from the tile's longest row); a row longer than the budget
-// is a tile of its own.  exact => 1 lane per row.
+// Cut rows [inst[b], inst[b+1]) of every instance into wave tasks:
+// consecutive rows whose nonzeros fit `budget`, at most (64 >> lg) *
+// kMaxSteps rows, lanes per row 2^lg from the tile's longest row and the
+// per-lane target; a row longer than the budget is a tile of its own with 64
+// lanes.  exact => 1 lane per row (the reference's serial order).
 void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& inst, int B,
                  mcp_tile_policy pol, bool exact, std::vector<mcp::Tile>& tiles,
                  std::vector<int32_t>& begin) {
-    const int budget = std::max(1, std::min(pol.nnz_budget, mcp::kNnzTile));
+    const int budget = std::max(1, pol.nnz_budget);
     const int per = std::max(1, pol.per_lane);
     tiles.clear();
     begin.assign(B + 1, 0);
@@ -83,7 +84,7 @@ void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& in
         while (r < re) {
             const int len0 = ptr[r + 1] - ptr[r];
             if (len0 > budget) {
-                tiles.push_back({r, r + 1, b, mcp::kShapeLong});
+                tiles.push_back({r, r + 1, b, exact ? 0 : 6});
                 ++r;
                 continue;
             }
@@ -94,7 +95,7 @@ void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& in
                 if (len > budget) break;
                 const int nm = std::max(maxlen, len);
                 const int lgn = exact ? 0 : lanes_log2(nm, per);
-                if (nnz + len > budget || (r - r0 + 1) > (mcp::kBlock >> lgn)) break;
+                if (nnz + len > budget || (r - r0 + 1) > ((64 >> lgn) * mcp::kMaxSteps)) break;
                 nnz += len;
                 maxlen = nm;
                 lg = lgn;
@@ -104,16 +105,13 @@ void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& in
         }
     }
     begin[B] = (int32_t)tiles.size();
-    // invariants the kernels rely on (LDS capacity, one thread/group per row)
+    // invariants the kernels rely on
     for (const auto& t : tiles) {
-        const int nnz = ptr[t.r1] - ptr[t.r0];
-        if (t.shape & mcp::kShapeLong) {
-            if (t.r1 != t.r0 + 1) throw EngineError(MC_ERR_INTERNAL, "bad long tile");
-        } else {
-            const int lg = t.shape & 7;
-            if (t.r1 <= t.r0 || nnz > mcp::kNnzTile || (t.r1 - t.r0) > (mcp::kBlock >> lg) || lg > 6)
-                throw EngineError(MC_ERR_INTERNAL, "tile invariant violated");
-        }
+        const int lg = t.shape & 7;
+        const bool single = t.r1 == t.r0 + 1;
+        if (t.r1 <= t.r0 || lg > 6 || (t.shape & ~7) != 0 ||
+            (!single && (t.r1 - t.r0) > ((64 >> lg) * mcp::kMaxSteps)) || (exact && lg != 0))
+            throw EngineError(MC_ERR_INTERNAL, "tile invariant violated");
     }
 }
 
@@ -139,12 +137,15 @@ struct mcp_engine {
     std::vector<int32_t> inst_row, inst_col, rp, cp;
     std::vector<mcp::Tile> tA, tT;
     std::vector<int32_t> tA_begin, tT_begin;
-    mcp_tile_policy polA{2048, 4}, polT{2048, 4};
+    mcp_tile_policy polA{1024, 8}, polT{1024, 8};
     bool loaded = false, setup_done = false, exact_tiles = false;
     int ruiz_passes_req = -1, ruiz_on_req = -1;
     double h2d_ms = 0.0, setup_ms = 0.0;
     std::map<std::tuple<int64_t, int, int, int, int>, hipGraphExec_t> graphs;
     mcp::Dev graph_dev{};  // kernel arguments the cached graphs were captured with
+    // autotuned tile policies per problem signature (instances, rows, columns,
+    // nonzeros, pointer checksum), so repeated solves of one shape tune once
+    std::map<std::tuple<int, int, int, int64_t, uint64_t>, std::pair<mcp_tile_policy, mcp_tile_policy>> tuned;
     int32_t* pinned = nullptr;  // [2 slots x 2 counters]
     hipEvent_t ev[4]{};
 
@@ -211,8 +212,8 @@ struct mcp_engine {
     // Time the plain SpMV of one orientation under each candidate policy and
     // keep the fastest (FP32 values: the mixed-precision iteration's shape).
     void autotune_one(bool rows) {
-        static const int budgets[] = {512, 1024, 2048};
-        static const int pers[] = {2, 4, 8, 16};
+        static const int budgets[] = {256, 1024, 4096};
+        static const int pers[] = {4, 8, 16, 32};
         const auto& ptr = rows ? rp : cp;
         const auto& inst = rows ? inst_row : inst_col;
         const int nrows = rows ? m : n;
@@ -234,11 +235,11 @@ struct mcp_engine {
                 build_tiles(ptr, inst, B, p, false, tl, tb);
                 auto* dt = buf<mcp::Tile>("tune_tiles", tl.size());
                 HIPCHK(hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
-                mcp::launch_spmv_plain(dt, (int)tl.size(), dptr, didx, dval, vin, vout, true, false, stream);
+                mcp::launch_spmv_plain(dt, (int)tl.size(), dptr, didx, dval, vin, vout, true, stream);
                 HIPCHK(hipEventRecord(ev[2], stream));
                 const int reps = 5;
                 for (int r = 0; r < reps; ++r)
-                    mcp::launch_spmv_plain(dt, (int)tl.size(), dptr, didx, dval, vin, vout, true, false, stream);
+                    mcp::launch_spmv_plain(dt, (int)tl.size(), dptr, didx, dval, vin, vout, true, stream);
                 HIPCHK(hipEventRecord(ev[3], stream));
                 HIPCHK(hipEventSynchronize(ev[3]));
                 const double t = ms_between(ev[2], ev[3]) / reps;
@@ -267,32 +268,28 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.mixed = o->precision == MC_PRECISION_MIXED;
     io.kkt = o->restart_scheme == MC_SCHEME_KKT ? 1 : 0;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
-    io.polish_threshold = o->polish_threshold > 0.0 ? o->polish_threshold : 10.0 * tol;
+    // 0 => no score threshold: FP32 runs until it converges or stalls
+    io.polish_threshold = o->polish_threshold > 0.0 ? o->polish_threshold : 0.0;
+    (void)tol;
 }
 
 // Kernels of iterations [iter0, iter0 + len): the sequence depends only on
 // iter0 modulo lcm(10, restart_every), so a captured chunk is reusable.
 void enqueue_chunk(mcp_engine* e, const mcp::IterOpts& io, bool adaptive, bool exact, int64_t iter0,
-                   int len, bool use32, bool use64) {
+                   int len) {
     const auto& d = e->d;
     hipStream_t s = e->stream;
     for (int t = 0; t < len; ++t) {
         const int64_t it = iter0 + t;
         const bool adapt = adaptive && (it % 10 == 0);
-        if (use32) mcp::launch_primal(d, t, true, adapt, false, s);
-        if (use64) mcp::launch_primal(d, t, false, adapt, exact, s);
+        mcp::launch_primal(d, t, adapt, exact, s);
         if (adapt) {
-            if (use32) mcp::launch_adx(d, t, true, false, s);
-            if (use64) mcp::launch_adx(d, t, false, exact, s);
+            mcp::launch_adx(d, t, exact, s);
             mcp::launch_adapt_finalize(d, exact, s);
         }
-        if (use32) mcp::launch_dual(d, t, true, false, s);
-        if (use64) mcp::launch_dual(d, t, false, exact, s);
+        mcp::launch_dual(d, t, exact, s);
         if ((it + 1) % io.restart_every == 0) {
-            for (int cur = 0; cur <= io.kkt; ++cur) {
-                if (use32) mcp::launch_resid(d, true, false, cur != 0, s);
-                if (use64) mcp::launch_resid(d, false, exact, cur != 0, s);
-            }
+            for (int cur = 0; cur <= io.kkt; ++cur) mcp::launch_resid(d, exact, cur != 0, s);
             mcp::launch_check_finalize(d, io, t, exact, false, s);
             mcp::launch_post_check(d, io.kkt != 0, s);
         }
@@ -451,8 +448,8 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         e->h2d_ms = ms_between(e->ev[0], e->ev[1]);
         e->loaded = true;
         e->setup_done = false;
-        e->tA.clear();
-        e->tT.clear();
+        e->polA = e->polT = mcp_tile_policy{1024, 8};
+        e->retile(false);  // setup kernels run on these tiles
     });
 }
 
@@ -492,8 +489,19 @@ int mcp_setup(mcp_engine* e, const mcp_setup_opts* o) {
         e->ruiz_on_req = o->ruiz;
         e->ruiz_passes_req = o->ruiz_iterations;
         if (o->autotune && !o->exact) {
-            e->autotune_one(true);
-            e->autotune_one(false);
+            uint64_t h = 1469598103934665603ULL;
+            for (size_t k = 0; k < e->rp.size(); k += 1 + e->rp.size() / 4096) h = (h ^ (uint64_t)e->rp[k]) * 1099511628211ULL;
+            for (size_t k = 0; k < e->cp.size(); k += 1 + e->cp.size() / 4096) h = (h ^ (uint64_t)e->cp[k]) * 1099511628211ULL;
+            const auto key = std::make_tuple(e->B, e->m, e->n, e->nnz, h);
+            auto it = e->tuned.find(key);
+            if (it != e->tuned.end()) {
+                e->polA = it->second.first;
+                e->polT = it->second.second;
+            } else {
+                e->autotune_one(true);
+                e->autotune_one(false);
+                e->tuned[key] = {e->polA, e->polT};
+            }
         }
         e->retile(o->exact != 0);
         e->setup_done = true;
@@ -554,21 +562,19 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         // MCPDLP_PROGRESS=<seconds>: report the solve's own progress on stderr.
         const char* prog_env = std::getenv("MCPDLP_PROGRESS");
         const double prog_every = prog_env ? std::atof(prog_env) : 0.0;
-        auto prog_last = std::chrono::steady_clock::now();
+        const auto prog_start = std::chrono::steady_clock::now();
+        auto prog_last = prog_start;
         int64_t iter = 0;
-        int32_t fp32_running = mixed ? e->B : 0;
         bool finished = false;
         while (iter < max_it) {
             const int len = (int)std::min<int64_t>(chunk, max_it - iter);
-            const bool use32 = mixed && fp32_running > 0;
-            const bool use64 = true;
             if (use_graphs && len == chunk) {
-                const auto key = std::make_tuple(iter % period, len, (int)use32, (int)use64, (int)exact);
+                const auto key = std::make_tuple(iter % period, len, 0, 0, (int)exact);
                 auto it = e->graphs.find(key);
                 if (it == e->graphs.end()) {
                     hipGraph_t g;
                     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    enqueue_chunk(e, io, adaptive, exact, iter, len, use32, use64);
+                    enqueue_chunk(e, io, adaptive, exact, iter, len);
                     HIPCHK(hipStreamEndCapture(s, &g));
                     hipGraphExec_t ex;
                     HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
@@ -578,7 +584,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                 }
                 HIPCHK(hipGraphLaunch(it->second, s));
             } else {
-                enqueue_chunk(e, io, adaptive, exact, iter, len, use32, use64);
+                enqueue_chunk(e, io, adaptive, exact, iter, len);
             }
             HIPCHK(hipGetLastError());
             const int slot = chunks & 1;
@@ -593,13 +599,18 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                     finished = true;
                     break;
                 }
-                fp32_running = pin[2 * prev + 1];
                 if (prog_every > 0.0) {
                     const auto now = std::chrono::steady_clock::now();
                     if (std::chrono::duration<double>(now - prog_last).count() >= prog_every) {
                         prog_last = now;
-                        std::fprintf(stderr, "[mcpdlp] iterations=%lld running=%d fp32=%d\n",
-                                     (long long)iter, pin[2 * prev], pin[2 * prev + 1]);
+                        mcp::InstState s0;
+                        HIPCHK(hipMemcpy(&s0, d.st, sizeof(s0), hipMemcpyDeviceToHost));
+                        std::fprintf(stderr,
+                                     "[mcpdlp] %.1fs iterations=%lld running=%d fp32=%d | inst0 score=%.3e "
+                                     "(p=%.2e d=%.2e g=%.2e) omega=%.3g eta=%.3g mode=%s\n",
+                                     std::chrono::duration<double>(now - prog_start).count(), (long long)iter,
+                                     pin[2 * prev], pin[2 * prev + 1], s0.score, s0.primal, s0.dual, s0.gap,
+                                     s0.omega, s0.eta, s0.mode == mcp::kModeFp32 ? "fp32" : "fp64");
                         std::fflush(stderr);
                     }
                 }
@@ -610,10 +621,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         const int32_t still_running = chunks > 0 ? pin[2 * last] : e->B;
         if (!finished && still_running > 0) {
             // iteration limit: residuals of the final average (pdlp.cpp:263-265)
-            for (int cur = 0; cur <= io.kkt; ++cur) {
-                if (mixed) mcp::launch_resid(d, true, false, cur != 0, s);
-                mcp::launch_resid(d, false, exact, cur != 0, s);
-            }
+            for (int cur = 0; cur <= io.kkt; ++cur) mcp::launch_resid(d, exact, cur != 0, s);
             mcp::launch_check_finalize(d, io, -1, exact, true, s);
             HIPCHK(hipGetLastError());
         }
@@ -702,11 +710,10 @@ int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
         const int32_t* ptr = t ? d.cp : d.rp;
         const int32_t* idx = t ? d.ri : d.ci;
         const void* val = f32 ? (const void*)(t ? d.tv32 : d.av32) : (const void*)(t ? d.tv64 : d.av64);
-        mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, exact, s);
+        mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, s);
         const int nrep = std::max(1, reps);
         HIPCHK(hipEventRecord(e->ev[0], s));
-        for (int r = 0; r < nrep; ++r)
-            mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, exact, s);
+        for (int r = 0; r < nrep; ++r) mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, s);
         HIPCHK(hipEventRecord(e->ev[1], s));
         HIPCHK(hipGetLastError());
         if (f32) {
@@ -781,16 +788,12 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         HIPCHK(hipMemcpy(d.iter_base, &zero, sizeof(zero), hipMemcpyHostToDevice));
         auto launch = [&] {
             switch (kind) {
-                case 0: mcp::launch_primal(d, 1, true, false, false, s); break;
-                case 1: mcp::launch_dual(d, 1, true, false, s); break;
-                case 2: mcp::launch_primal(d, 1, false, false, false, s); break;
-                case 3: mcp::launch_dual(d, 1, false, false, s); break;
-                case 4:
-                    mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.x32, d.y32, true, false, s);
-                    break;
-                default:
-                    mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.x32, true, false, s);
-                    break;
+                case 0:
+                case 2: mcp::launch_primal(d, 1, false, false, s); break;
+                case 1:
+                case 3: mcp::launch_dual(d, 1, false, s); break;
+                case 4: mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.x32, d.y32, true, s); break;
+                default: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.x32, true, s); break;
             }
         };
         launch();
@@ -802,6 +805,19 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         HIPCHK(hipEventSynchronize(e->ev[1]));
         *avg_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / nrep;
         *bytes = b;
+    });
+}
+
+int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const mcp_tile_policy* cols_policy) {
+    return guarded([&] {
+        require(e && e->loaded && rows_policy && cols_policy, "mcp_set_tile_policy needs a loaded problem");
+        require(rows_policy->nnz_budget >= 1 && cols_policy->nnz_budget >= 1 && rows_policy->per_lane >= 1 &&
+                    cols_policy->per_lane >= 1,
+                "tile policy fields must be positive");
+        HIPCHK(hipSetDevice(e->device));
+        e->polA = *rows_policy;
+        e->polT = *cols_policy;
+        e->retile(e->exact_tiles);
     });
 }
 

@@ -9,174 +9,219 @@
 //                                                              (pdlp.cpp:220-227)
 // plus, every 10th iteration, k_adx (A dx) and k_adapt_finalize (eta update,
 // :207-219), and every restart_every iterations the FP64 check: k_resid
-// (A x_avg, A^T y_avg and the residual terms on the unscaled model, :62-135),
+// (A x_c, A^T y_c and the residual terms on the unscaled model, :62-135),
 // k_check_finalize (termination, restart, primal weight, :229-260) and
-// k_post_check (restart copy / FP32->FP64 switch).  Every kernel gates on the
-// instance's status and arithmetic mode, so finished instances cost nothing
-// and a batch advances with one launch per phase.
+// k_post_check (restart copy / FP32->FP64 switch).
 //
-// Step sizes are recomputed per element from (eta, omega) and the scaled L1
-// norms exactly as the reference's update_step_sizes() forms them
-// (tau = (eta/omega)/colnorm, sigma = (eta*omega)/rownorm), so no tau/sigma
-// vectors are stored or rewritten.  Built with -ffp-contract=off: every
-// product and sum is rounded once, as on the host.
+// Every row kernel is a set of wave tasks (one Tile per wave64, no LDS, no
+// workgroup barrier): a wave gates on its instance's status, picks FP32 or
+// FP64 from the instance's mode at run time (so mixed precision costs no
+// extra launches), sweeps its rows 64/G at a time and lets lane 0 of each
+// row group apply the update.  Step sizes are recomputed per element from
+// (eta, omega) and the scaled L1 norms exactly as the reference's
+// update_step_sizes() forms them (tau = (eta/omega)/colnorm,
+// sigma = (eta*omega)/rownorm).  Built with -ffp-contract=off: every product
+// and sum is rounded once, as on the host.
 #include <math.h>
 
 #include "dev.hpp"
 
 namespace mcp {
 
-template <typename T>
-struct Prec;
-template <>
-struct Prec<double> {
-    static constexpr int32_t mode = kModeFp64;
+struct Lanes {
+    int lane, lg, G, gl, step;
 };
-template <>
-struct Prec<float> {
-    static constexpr int32_t mode = kModeFp32;
-};
-
-template <typename T>
-__device__ __forceinline__ T* pick(double* p64, float* p32);
-template <>
-__device__ __forceinline__ double* pick<double>(double* p64, float*) { return p64; }
-template <>
-__device__ __forceinline__ float* pick<float>(double*, float* p32) { return p32; }
-
-__device__ __forceinline__ bool gate(const InstState& s, int32_t mode) {
-    return s.status == kRunning && s.mode == mode;
+__device__ __forceinline__ Lanes lanes_of(const Tile& tl) {
+    Lanes l;
+    l.lane = threadIdx.x & 63;
+    l.lg = tl.shape & 7;
+    l.G = 1 << l.lg;
+    l.gl = l.lane & (l.G - 1);
+    l.step = 64 >> l.lg;
+    return l;
 }
 
 // ---------------------------------------------------------------------------
 // k_primal: z = A^T y fused with the primal update of every column.
-template <typename T, bool ADAPT, bool EXACT>
+template <bool ADAPT, bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
-    const int tile = xcd_tile(d.ntT);
+    const int tile = wave_tile(d.ntT);
     if (tile >= d.ntT) return;
     const Tile tl = d.tT[tile];
     const InstState& s = d.st[tl.inst];
-    if (!gate(s, Prec<T>::mode)) return;
-    __shared__ T prod[kNnzTile];
-    __shared__ T rsum[kBlock];
-    __shared__ T redT[kWaves];
-    __shared__ double redD[kWaves];
+    if (s.status != kRunning) return;
+    const Lanes L = lanes_of(tl);
     const int64_t iter = *d.iter_base + t;
-    const T* y = pick<T>(d.y, d.y32);
-    const T* tv = pick<T>(d.tv64, d.tv32);
-    const int nr = tile_rowsums<EXACT, T>(tl, d.cp, d.ri, tv, y, prod, rsum, redT);
-    const int tid = threadIdx.x;
     const double eta = s.eta, omega = s.omega;
-    const T kavg = (T)(double)(iter - s.restart_iter + 1);
+    const double kavg = (double)(iter - s.restart_iter + 1);
     double part = 0.0;
-    if (tid < nr) {
-        const int j = tl.r0 + tid;
-        if constexpr (sizeof(T) == 8) {
-            const double g = d.c[j] + rsum[tid];
-            const double tau = (eta / omega) / d.cnorm[j];
-            const double xo = d.x[j];
-            const double xp = clampv(xo - tau * g, d.slo[j], d.shi[j]);
-            const double dd = xp - xo;
-            if (ADAPT) {
-                d.dx[j] = dd;
-                part = d.cnorm[j] * dd * dd;
+    if (!EXACT && s.mode == kModeFp32) {
+        const float tauf = (float)(eta / omega), kf = (float)kavg;
+        for (int base = tl.r0; base < tl.r1; base += L.step) {
+            const int j = base + (L.lane >> L.lg);
+            const bool own = j < tl.r1;
+            int p0 = 0, p1 = 0;
+            float4 pk = make_float4(0.f, 0.f, 0.f, 0.f);
+            float xo = 0.f, xa = 0.f;
+            double cn = 0.0;
+            if (own) {  // operands in flight while the column streams
+                p0 = d.cp[j];
+                p1 = d.cp[j + 1];
+                pk = d.colpk[j];  // c, lo, hi, 1/colnorm
+                xo = d.x32[j];
+                xa = d.xavg32[j];
+                if (ADAPT) cn = d.cnorm[j];
             }
-            d.x[j] = xp;
-            d.xbar[j] = 2.0 * xp - xo;
-            const double xa = d.xavg[j];
-            d.xavg[j] = xa + (xp - xa) / kavg;
-        } else {
-            const float4 pk = d.colpk[j];  // c, lo, hi, 1/colnorm
-            const float g = pk.x + rsum[tid];
-            const float tau = (float)(eta / omega) * pk.w;
-            const float xo = d.x32[j];
-            const float xp = clampv(xo - tau * g, pk.y, pk.z);
-            const float dd = xp - xo;
-            if (ADAPT) {
-                d.dx32[j] = dd;
-                part = d.cnorm[j] * (double)dd * (double)dd;
+            const float aty = row_dot<float>(p0, p1, L.gl, L.G, d.ri, d.tv32, d.y32);
+            if (own && L.gl == 0) {
+                const float g = pk.x + aty;
+                const float xp = clampv(xo - (tauf * pk.w) * g, pk.y, pk.z);
+                const float dd = xp - xo;
+                if (ADAPT) {
+                    d.dx32[j] = dd;
+                    part += cn * (double)dd * (double)dd;
+                }
+                d.x32[j] = xp;
+                d.xbar32[j] = 2.0f * xp - xo;
+                d.xavg32[j] = xa + (xp - xa) / kf;
             }
-            d.x32[j] = xp;
-            d.xbar32[j] = 2.0f * xp - xo;
-            const float xa = d.xavg32[j];
-            d.xavg32[j] = xa + (xp - xa) / kavg;
+        }
+    } else {
+        for (int base = tl.r0; base < tl.r1; base += L.step) {
+            const int j = base + (L.lane >> L.lg);
+            const bool own = j < tl.r1;
+            int p0 = 0, p1 = 0;
+            double c = 0, cn = 1, lo = 0, hi = 0, xo = 0, xa = 0;
+            if (own) {
+                p0 = d.cp[j];
+                p1 = d.cp[j + 1];
+                c = d.c[j];
+                cn = d.cnorm[j];
+                lo = d.slo[j];
+                hi = d.shi[j];
+                xo = d.x[j];
+                xa = d.xavg[j];
+            }
+            const double aty = row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.y);
+            if (own && L.gl == 0) {
+                const double g = c + aty;
+                const double tau = (eta / omega) / cn;
+                const double xp = clampv(xo - tau * g, lo, hi);
+                const double dd = xp - xo;
+                if (ADAPT) {
+                    d.dx[j] = dd;
+                    part += cn * dd * dd;
+                }
+                d.x[j] = xp;
+                d.xbar[j] = 2.0 * xp - xo;
+                d.xavg[j] = xa + (xp - xa) / kavg;
+            }
         }
     }
     if (ADAPT && !EXACT) {
-        part = block_sum(part, redD);
-        if (tid == 0) d.part_dxn[tile] = part;
+        part = wave_sum(part);
+        if (L.lane == 0) d.part_dxn[tile] = part;
     }
 }
 
 // ---------------------------------------------------------------------------
 // k_dual: A x_bar fused with the dual prox step and the dual running average.
-template <typename T, bool EXACT>
+template <bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
-    const int tile = xcd_tile(d.ntA);
+    const int tile = wave_tile(d.ntA);
     if (tile >= d.ntA) return;
     const Tile tl = d.tA[tile];
     const InstState& s = d.st[tl.inst];
-    if (!gate(s, Prec<T>::mode)) return;
-    __shared__ T prod[kNnzTile];
-    __shared__ T rsum[kBlock];
-    __shared__ T redT[kWaves];
+    if (s.status != kRunning) return;
+    const Lanes L = lanes_of(tl);
     const int64_t iter = *d.iter_base + t;
-    const T* xb = pick<T>(d.xbar, d.xbar32);
-    const T* av = pick<T>(d.av64, d.av32);
-    const int nr = tile_rowsums<EXACT, T>(tl, d.rp, d.ci, av, xb, prod, rsum, redT);
-    const int tid = threadIdx.x;
     const double eta = s.eta, omega = s.omega;
-    const T kavg = (T)(double)(iter - s.restart_iter + 1);
-    if (tid < nr) {
-        const int i = tl.r0 + tid;
-        if constexpr (sizeof(T) == 8) {
-            const double sigma = (eta * omega) / d.rnorm[i];
-            const double v = d.y[i] + sigma * rsum[tid];
-            const double yn = v - sigma * clampv(v / sigma, d.blo[i], d.bhi[i]);
-            d.y[i] = yn;
-            const double ya = d.yavg[i];
-            d.yavg[i] = ya + (yn - ya) / kavg;
-        } else {
-            const float4 pk = d.rowpk[i];  // blo, bhi, 1/rownorm
-            const float sigma = (float)(eta * omega) * pk.z;
-            const float v = d.y32[i] + sigma * rsum[tid];
-            const float yn = v - sigma * clampv(v / sigma, pk.x, pk.y);
-            d.y32[i] = yn;
-            const float ya = d.yavg32[i];
-            d.yavg32[i] = ya + (yn - ya) / kavg;
+    const double kavg = (double)(iter - s.restart_iter + 1);
+    if (!EXACT && s.mode == kModeFp32) {
+        const float sigf = (float)(eta * omega), kf = (float)kavg;
+        for (int base = tl.r0; base < tl.r1; base += L.step) {
+            const int i = base + (L.lane >> L.lg);
+            const bool own = i < tl.r1;
+            int p0 = 0, p1 = 0;
+            float4 pk = make_float4(0.f, 0.f, 1.f, 0.f);
+            float yo = 0.f, ya = 0.f;
+            if (own) {
+                p0 = d.rp[i];
+                p1 = d.rp[i + 1];
+                pk = d.rowpk[i];  // blo, bhi, 1/rownorm
+                yo = d.y32[i];
+                ya = d.yavg32[i];
+            }
+            const float ax = row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.xbar32);
+            if (own && L.gl == 0) {
+                const float sigma = sigf * pk.z;
+                const float v = yo + sigma * ax;
+                const float yn = v - sigma * clampv(v / sigma, pk.x, pk.y);
+                d.y32[i] = yn;
+                d.yavg32[i] = ya + (yn - ya) / kf;
+            }
+        }
+    } else {
+        for (int base = tl.r0; base < tl.r1; base += L.step) {
+            const int i = base + (L.lane >> L.lg);
+            const bool own = i < tl.r1;
+            int p0 = 0, p1 = 0;
+            double rn = 1, bl = 0, bh = 0, yo = 0, ya = 0;
+            if (own) {
+                p0 = d.rp[i];
+                p1 = d.rp[i + 1];
+                rn = d.rnorm[i];
+                bl = d.blo[i];
+                bh = d.bhi[i];
+                yo = d.y[i];
+                ya = d.yavg[i];
+            }
+            const double ax = row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.xbar);
+            if (own && L.gl == 0) {
+                const double sigma = (eta * omega) / rn;
+                const double v = yo + sigma * ax;
+                const double yn = v - sigma * clampv(v / sigma, bl, bh);
+                d.y[i] = yn;
+                d.yavg[i] = ya + (yn - ya) / kavg;
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // k_adx: A dx for the adaptive step-size estimate (pdlp.cpp:208-210).
-template <typename T, bool EXACT>
+template <bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_adx(Dev d, int) {
-    const int tile = xcd_tile(d.ntA);
+    const int tile = wave_tile(d.ntA);
     if (tile >= d.ntA) return;
     const Tile tl = d.tA[tile];
     const InstState& s = d.st[tl.inst];
-    if (!gate(s, Prec<T>::mode)) return;
-    __shared__ T prod[kNnzTile];
-    __shared__ T rsum[kBlock];
-    __shared__ T redT[kWaves];
-    __shared__ double redD[kWaves];
-    const T* dx = pick<T>(d.dx, d.dx32);
-    const T* av = pick<T>(d.av64, d.av32);
-    const int nr = tile_rowsums<EXACT, T>(tl, d.rp, d.ci, av, dx, prod, rsum, redT);
-    const int tid = threadIdx.x;
+    if (s.status != kRunning) return;
+    const Lanes L = lanes_of(tl);
+    const bool f32 = !EXACT && s.mode == kModeFp32;
     double part = 0.0;
-    if (tid < nr) {
-        const int i = tl.r0 + tid;
-        const double a = (double)rsum[tid];
-        if (EXACT)
-            d.ax[i] = a;
-        else
-            part = (a * a) / d.rnorm[i];
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int i = base + (L.lane >> L.lg);
+        const bool own = i < tl.r1;
+        int p0 = 0, p1 = 0;
+        double rn = 1.0;
+        if (own) {
+            p0 = d.rp[i];
+            p1 = d.rp[i + 1];
+            rn = d.rnorm[i];
+        }
+        const double a = f32 ? (double)row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.dx32)
+                             : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.dx);
+        if (own && L.gl == 0) {
+            if (EXACT)
+                d.ax[i] = a;
+            else
+                part += (a * a) / rn;
+        }
     }
     if (!EXACT) {
-        part = block_sum(part, redD);
-        if (tid == 0) d.part_adx[tile] = part;
+        part = wave_sum(part);
+        if (L.lane == 0) d.part_adx[tile] = part;
     }
 }
 
@@ -195,19 +240,51 @@ __device__ void adapt_decide(InstState& s, double dxn, double adxn) {
     }
 }
 
+// One workgroup per instance: the tile partials are summed by the whole
+// block (fixed strided assignment + fixed-order block reduction, so the result
+// is deterministic); EXACT sums the vectors serially in the reference's order.
+// One workgroup per instance: the tile partials are summed by the whole
+// block (fixed assignment of partials to threads, four independent loads per
+// thread, fixed-order block reduction -- deterministic); EXACT sums the
+// vectors serially in the reference's order.
+__device__ __forceinline__ double sum_strided(const double* p, int beg, int end, int stride) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = beg + (int)threadIdx.x;
+    for (; k + 3 * kFinal < end; k += 4 * kFinal) {
+        a0 += p[(size_t)k * stride];
+        a1 += p[(size_t)(k + kFinal) * stride];
+        a2 += p[(size_t)(k + 2 * kFinal) * stride];
+        a3 += p[(size_t)(k + 3 * kFinal) * stride];
+    }
+    for (; k < end; k += kFinal) a0 += p[(size_t)k * stride];
+    return (a0 + a1) + (a2 + a3);
+}
+__device__ __forceinline__ double max_strided(const double* p, int beg, int end, int stride) {
+    double a0 = 0.0, a1 = 0.0;
+    int k = beg + (int)threadIdx.x;
+    for (; k + kFinal < end; k += 2 * kFinal) {
+        a0 = maxv(a0, p[(size_t)k * stride]);
+        a1 = maxv(a1, p[(size_t)(k + kFinal) * stride]);
+    }
+    for (; k < end; k += kFinal) a0 = maxv(a0, p[(size_t)k * stride]);
+    return maxv(a0, a1);
+}
+
 template <bool EXACT>
-__global__ void k_adapt_finalize(Dev d) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= d.B) return;
+__global__ __launch_bounds__(kFinal) void k_adapt_finalize(Dev d) {
+    const int b = blockIdx.x;
     InstState& s = d.st[b];
-    if (s.status != kRunning) return;
+    if (s.status != kRunning) return;  // uniform over the block
+    __shared__ double red[kFinal / 64];
     double dxn = 0.0, adxn = 0.0;
-    if (EXACT) {  // serial sums in the reference's index order (FP64 only)
+    if (EXACT) {
+        if (threadIdx.x != 0) return;
         for (int j = d.inst_col[b]; j < d.inst_col[b + 1]; ++j) dxn += d.cnorm[j] * d.dx[j] * d.dx[j];
         for (int i = d.inst_row[b]; i < d.inst_row[b + 1]; ++i) adxn += (d.ax[i] * d.ax[i]) / d.rnorm[i];
     } else {
-        for (int k = d.tT_begin[b]; k < d.tT_begin[b + 1]; ++k) dxn += d.part_dxn[k];
-        for (int k = d.tA_begin[b]; k < d.tA_begin[b + 1]; ++k) adxn += d.part_adx[k];
+        dxn = block_sum<kFinal / 64>(sum_strided(d.part_dxn, d.tT_begin[b], d.tT_begin[b + 1], 1), red);
+        adxn = block_sum<kFinal / 64>(sum_strided(d.part_adx, d.tA_begin[b], d.tA_begin[b + 1], 1), red);
+        if (threadIdx.x != 0) return;
     }
     adapt_decide(s, dxn, adxn);
 }
@@ -215,113 +292,142 @@ __global__ void k_adapt_finalize(Dev d) {
 // ---------------------------------------------------------------------------
 // FP64 residual terms of a candidate iterate on the unscaled model
 // (pdlp.cpp:76-131).  CUR = false: the running average (the reference's only
-// candidate); CUR = true: the current iterate (KKT restart scheme).  Rows:
-// A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
-template <typename TV, bool EXACT, bool CUR>
+// candidate); CUR = true: the current iterate (KKT restart scheme).  The
+// iterate is read in the instance's precision, the matrix in FP64.
+// Rows: A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
+template <bool EXACT, bool CUR>
 __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
-    const int tile = xcd_tile(d.ntA);
+    const int tile = wave_tile(d.ntA);
     if (tile >= d.ntA) return;
     const Tile tl = d.tA[tile];
     const InstState& s = d.st[tl.inst];
-    if (!gate(s, Prec<TV>::mode)) return;
-    __shared__ double prod[kNnzTile];
-    __shared__ double rsum[kBlock];
-    __shared__ double red[kWaves];
-    const TV* xc = CUR ? pick<TV>(d.x, d.x32) : pick<TV>(d.xavg, d.xavg32);
-    const TV* yc = CUR ? pick<TV>(d.y, d.y32) : pick<TV>(d.yavg, d.yavg32);
-    const TV* yl = pick<TV>(d.ylast, d.ylast32);
-    const int nr = tile_rowsums<EXACT, double>(tl, d.rp, d.ci, d.av64, xc, prod, rsum, red);
-    const int tid = threadIdx.x;
+    if (s.status != kRunning) return;
+    const Lanes L = lanes_of(tl);
+    const bool f32 = !EXACT && s.mode == kModeFp32;
+    const double* x64 = CUR ? d.x : d.xavg;
+    const double* y64 = CUR ? d.y : d.yavg;
+    const float* x32 = CUR ? d.x32 : d.xavg32;
+    const float* y32 = CUR ? d.y32 : d.yavg32;
     double viol = 0.0, dob = 0.0, mv = 0.0;
-    if (tid < nr) {
-        const int i = tl.r0 + tid;
-        if (EXACT) {
-            (CUR ? d.ax2 : d.ax)[i] = rsum[tid];
-        } else {
-            const double rs = d.row_scale[i];
-            const double axu = rsum[tid] / rs;
-            const double lo = d.orow_lo[i], hi = d.orow_hi[i];
-            const double proj = clampv(axu, lo, hi);
-            const double r = fabs(axu - proj);
-            const double sc = 1.0 + maxv(fabs(axu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
-            viol = r / sc;
-            const double yu = (double)yc[i] * rs;
-            if (yu > 0.0 && hi < 1e299)
-                dob = -(yu * hi);
-            else if (yu < 0.0 && lo > -1e299)
-                dob = -(yu * lo);
-            const double dy = (double)yc[i] - (double)yl[i];
-            mv = dy * dy;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int i = base + (L.lane >> L.lg);
+        const bool own = i < tl.r1;
+        int p0 = 0, p1 = 0;
+        double rs = 1, lo = 0, hi = 0, yc = 0, yl = 0;
+        if (own) {
+            p0 = d.rp[i];
+            p1 = d.rp[i + 1];
+            rs = d.row_scale[i];
+            lo = d.orow_lo[i];
+            hi = d.orow_hi[i];
+            yc = f32 ? (double)y32[i] : y64[i];
+            yl = f32 ? (double)d.ylast32[i] : d.ylast[i];
+        }
+        const double ax = f32 ? row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, x32)
+                              : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, x64);
+        if (own && L.gl == 0) {
+            if (EXACT) {
+                (CUR ? d.ax2 : d.ax)[i] = ax;
+            } else {
+                const double axu = ax / rs;
+                const double proj = clampv(axu, lo, hi);
+                const double r = fabs(axu - proj);
+                const double sc = 1.0 + maxv(fabs(axu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
+                viol = maxv(viol, r / sc);
+                const double yu = yc * rs;
+                if (yu > 0.0 && hi < 1e299)
+                    dob += -(yu * hi);
+                else if (yu < 0.0 && lo > -1e299)
+                    dob += -(yu * lo);
+                const double dy = yc - yl;
+                mv += dy * dy;
+            }
         }
     }
     if (!EXACT) {
-        double* out = d.part_rows + 3 * ((CUR ? d.ntA : 0) + tile);
-        viol = block_max(viol, red);
-        if (tid == 0) out[0] = viol;
-        dob = block_sum(dob, red);
-        if (tid == 0) out[1] = dob;
-        mv = block_sum(mv, red);
-        if (tid == 0) out[2] = mv;
+        viol = wave_max(viol);
+        dob = wave_sum(dob);
+        mv = wave_sum(mv);
+        if (L.lane == 0) {
+            double* out = d.part_rows + 3 * ((CUR ? d.ntA : 0) + tile);
+            out[0] = viol;
+            out[1] = dob;
+            out[2] = mv;
+        }
     }
 }
 
 // Columns: A^T y_c, bound violation, dual residual, primal/dual objective
 // parts, |x_c - x_last|^2.
-template <typename TV, bool EXACT, bool CUR>
+template <bool EXACT, bool CUR>
 __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
-    const int tile = xcd_tile(d.ntT);
+    const int tile = wave_tile(d.ntT);
     if (tile >= d.ntT) return;
     const Tile tl = d.tT[tile];
     const InstState& s = d.st[tl.inst];
-    if (!gate(s, Prec<TV>::mode)) return;
-    __shared__ double prod[kNnzTile];
-    __shared__ double rsum[kBlock];
-    __shared__ double red[kWaves];
-    const TV* xc = CUR ? pick<TV>(d.x, d.x32) : pick<TV>(d.xavg, d.xavg32);
-    const TV* yc = CUR ? pick<TV>(d.y, d.y32) : pick<TV>(d.yavg, d.yavg32);
-    const TV* xl = pick<TV>(d.xlast, d.xlast32);
-    const int nr = tile_rowsums<EXACT, double>(tl, d.cp, d.ri, d.tv64, yc, prod, rsum, red);
-    const int tid = threadIdx.x;
+    if (s.status != kRunning) return;
+    const Lanes L = lanes_of(tl);
+    const bool f32 = !EXACT && s.mode == kModeFp32;
+    const double* x64 = CUR ? d.x : d.xavg;
+    const double* y64 = CUR ? d.y : d.yavg;
+    const float* x32 = CUR ? d.x32 : d.xavg32;
+    const float* y32 = CUR ? d.y32 : d.yavg32;
+    const double sense = d.sense[tl.inst];
     double viol = 0.0, dres = 0.0, pob = 0.0, dob = 0.0, mv = 0.0;
-    if (tid < nr) {
-        const int j = tl.r0 + tid;
-        if (EXACT) {
-            (CUR ? d.aty2 : d.aty)[j] = rsum[tid];
-        } else {
-            const double cs = d.col_scale[j];
-            const double xu = (double)xc[j] * cs;
-            const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
-            const double proj = clampv(xu, lo, hi);
-            const double r = fabs(xu - proj);
-            const double sc = 1.0 + maxv(fabs(xu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
-            viol = r / sc;
-            const double atyu = rsum[tid] / cs;
-            const double corig = d.sense[tl.inst] * d.obj[j];
-            const double g = corig + atyu;
-            const double xp = clampv(xu - g, lo, hi);
-            const double df = xu - xp;
-            dres = df * df;
-            pob = d.obj[j] * xu;
-            if (g > 0.0 && lo > -1e299)
-                dob = g * lo;
-            else if (g < 0.0 && hi < 1e299)
-                dob = g * hi;
-            const double dxm = (double)xc[j] - (double)xl[j];
-            mv = dxm * dxm;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int j = base + (L.lane >> L.lg);
+        const bool own = j < tl.r1;
+        int p0 = 0, p1 = 0;
+        double cs = 1, lo = 0, hi = 0, xc = 0, xl = 0, ob = 0;
+        if (own) {
+            p0 = d.cp[j];
+            p1 = d.cp[j + 1];
+            cs = d.col_scale[j];
+            lo = d.ovar_lo[j];
+            hi = d.ovar_hi[j];
+            ob = d.obj[j];
+            xc = f32 ? (double)x32[j] : x64[j];
+            xl = f32 ? (double)d.xlast32[j] : d.xlast[j];
+        }
+        const double aty = f32 ? row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, y32)
+                               : row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, y64);
+        if (own && L.gl == 0) {
+            if (EXACT) {
+                (CUR ? d.aty2 : d.aty)[j] = aty;
+            } else {
+                const double xu = xc * cs;
+                const double proj = clampv(xu, lo, hi);
+                const double r = fabs(xu - proj);
+                const double sc = 1.0 + maxv(fabs(xu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
+                viol = maxv(viol, r / sc);
+                const double g = sense * ob + aty / cs;
+                const double xp = clampv(xu - g, lo, hi);
+                const double df = xu - xp;
+                dres += df * df;
+                pob += ob * xu;
+                if (g > 0.0 && lo > -1e299)
+                    dob += g * lo;
+                else if (g < 0.0 && hi < 1e299)
+                    dob += g * hi;
+                const double dxm = xc - xl;
+                mv += dxm * dxm;
+            }
         }
     }
     if (!EXACT) {
-        double* out = d.part_cols + 5 * ((CUR ? d.ntT : 0) + tile);
-        viol = block_max(viol, red);
-        if (tid == 0) out[0] = viol;
-        dres = block_sum(dres, red);
-        if (tid == 0) out[1] = dres;
-        pob = block_sum(pob, red);
-        if (tid == 0) out[2] = pob;
-        dob = block_sum(dob, red);
-        if (tid == 0) out[3] = dob;
-        mv = block_sum(mv, red);
-        if (tid == 0) out[4] = mv;
+        viol = wave_max(viol);
+        dres = wave_sum(dres);
+        pob = wave_sum(pob);
+        dob = wave_sum(dob);
+        mv = wave_sum(mv);
+        if (L.lane == 0) {
+            double* out = d.part_cols + 5 * ((CUR ? d.ntT : 0) + tile);
+            out[0] = viol;
+            out[1] = dres;
+            out[2] = pob;
+            out[3] = dob;
+            out[4] = mv;
+        }
     }
 }
 
@@ -397,23 +503,21 @@ __device__ void serial_residuals(const Dev& d, int b, const double* xs, const do
     out.ymove2 = ym;
 }
 
-// Candidate residuals from the tile partials (fast order).
-__device__ void partial_residuals(const Dev& d, int b, int cand, double c_scale, CandResid& out) {
-    double mp = 0.0, dsq = 0.0, po = 0.0, dr = 0.0, dc = 0.0, xm = 0.0, ym = 0.0;
+// Candidate residuals from the tile partials (fast order), reduced by the
+// whole block; the result is valid in thread 0.
+__device__ void partial_residuals(const Dev& d, int b, int cand, double c_scale, CandResid& out,
+                                  double* red) {
     const double* pr = d.part_rows + 3 * (cand ? d.ntA : 0);
     const double* pc = d.part_cols + 5 * (cand ? d.ntT : 0);
-    for (int k = d.tA_begin[b]; k < d.tA_begin[b + 1]; ++k) {
-        mp = maxv(mp, pr[3 * k]);
-        dr += pr[3 * k + 1];
-        ym += pr[3 * k + 2];
-    }
-    for (int k = d.tT_begin[b]; k < d.tT_begin[b + 1]; ++k) {
-        mp = maxv(mp, pc[5 * k]);
-        dsq += pc[5 * k + 1];
-        po += pc[5 * k + 2];
-        dc += pc[5 * k + 3];
-        xm += pc[5 * k + 4];
-    }
+    const int a0 = d.tA_begin[b], a1 = d.tA_begin[b + 1], t0 = d.tT_begin[b], t1 = d.tT_begin[b + 1];
+    constexpr int NW = kFinal / 64;
+    const double mp = block_max<NW>(maxv(max_strided(pr, a0, a1, 3), max_strided(pc, t0, t1, 5)), red);
+    const double dr = block_sum<NW>(sum_strided(pr + 1, a0, a1, 3), red);
+    const double ym = block_sum<NW>(sum_strided(pr + 2, a0, a1, 3), red);
+    const double dsq = block_sum<NW>(sum_strided(pc + 1, t0, t1, 5), red);
+    const double po = block_sum<NW>(sum_strided(pc + 2, t0, t1, 5), red);
+    const double dc = block_sum<NW>(sum_strided(pc + 3, t0, t1, 5), red);
+    const double xm = block_sum<NW>(sum_strided(pc + 4, t0, t1, 5), red);
     out.primal = mp;
     out.dual = sqrt(dsq) / c_scale;
     out.objective = d.offset[b] + po;
@@ -438,25 +542,31 @@ __device__ __forceinline__ void stop_instance(const Dev& d, InstState& s, int32_
 // KKT scheme as documented for MC_SCHEME_KKT).  FINAL: the iteration limit
 // was reached -- record the better candidate's residuals and stop.
 template <bool EXACT, bool FINAL>
-__global__ void k_check_finalize(Dev d, IterOpts o, int t) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= d.B) return;
+__global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, int t) {
+    const int b = blockIdx.x;  // one workgroup per instance
     InstState& s = d.st[b];
-    s.restart_flag = 0;
-    s.switch_flag = 0;
-    if (s.status != kRunning) return;
+    __shared__ double red[kFinal / 64];
+    __shared__ int running;
+    if (threadIdx.x == 0) running = s.status == kRunning;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s.restart_flag = 0;
+        s.switch_flag = 0;
+    }
+    if (!running) return;
     const int64_t iter_after = *d.iter_base + t + 1;
     CandResid r, rc;
-    if (EXACT)
+    if (!EXACT) {
+        partial_residuals(d, b, 0, s.c_scale, r, red);
+        if (o.kkt) partial_residuals(d, b, 1, s.c_scale, rc, red);
+    }
+    if (threadIdx.x != 0) return;
+    if (EXACT) {
         serial_residuals(d, b, d.xavg, d.yavg, d.ax, d.aty, s.c_scale, r);
-    else
-        partial_residuals(d, b, 0, s.c_scale, r);
+        if (o.kkt) serial_residuals(d, b, d.x, d.y, d.ax2, d.aty2, s.c_scale, rc);
+    }
     int use_cur = 0;
     if (o.kkt) {
-        if (EXACT)
-            serial_residuals(d, b, d.x, d.y, d.ax2, d.aty2, s.c_scale, rc);
-        else
-            partial_residuals(d, b, 1, s.c_scale, rc);
         if (rc.score < r.score) {
             r = rc;
             use_cur = 1;
@@ -523,14 +633,15 @@ __global__ void k_check_finalize(Dev d, IterOpts o, int t) {
     }
     if (s.mode == kModeFp32) {
         // FP32 iterations stop paying once the score is near tolerance or has
-        // stopped improving; continue from this iterate in FP64.
-        if (score < 0.9 * s.best_score) {
+        // not set a new best for 50 checks (the FP32 rounding floor); continue
+        // from this iterate in FP64.
+        if (score < s.best_score) {
             s.best_score = score;
             s.stall = 0;
         } else {
             s.stall += 1;
         }
-        if (score <= o.polish_threshold || s.stall >= 12) {
+        if (score <= o.polish_threshold || s.stall >= 50) {
             s.switch_flag = 1;
             s.mode = kModeFp64;
             s.fp32_iters = iter_after;
@@ -543,7 +654,8 @@ __global__ void k_check_finalize(Dev d, IterOpts o, int t) {
 // the average, which also becomes the new x_last/y_last) and FP32 -> FP64
 // conversion of an instance's iterate.  Grid: column tiles, then row tiles.
 __global__ __launch_bounds__(kBlock) void k_post_check(Dev d, int kkt) {
-    const int g = blockIdx.x;
+    const int g = blockIdx.x * kWaves + (threadIdx.x >> 6);  // one wave per tile
+    if (g >= d.ntT + d.ntA) return;
     const bool cols = g < d.ntT;
     const Tile tl = cols ? d.tT[g] : d.tA[g - d.ntT];
     const InstState& s = d.st[tl.inst];
@@ -556,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_post_check(Dev d, int kkt) {
     float* v32 = cols ? d.x32 : d.y32;
     float* a32 = cols ? d.xavg32 : d.yavg32;
     float* l32 = cols ? d.xlast32 : d.ylast32;
-    for (int k = tl.r0 + threadIdx.x; k < tl.r1; k += kBlock) {
+    for (int k = tl.r0 + (int)(threadIdx.x & 63); k < tl.r1; k += 64) {
         if (s.restart_flag) {
             if (r32) {
                 if (to_avg) v32[k] = a32[k];
@@ -579,36 +691,50 @@ __global__ void k_advance(Dev d, int len) {
 }
 
 // ---------------------------------------------------------------------------
-// Setup: device Ruiz equilibration (ruiz_scaling.cpp:152-247).  All maxima
-// are order-independent, so the result equals the serial host pass bit for
-// bit.  Nonnegative doubles order like their bit patterns, so atomicMax on
-// the 64-bit pattern is an exact max.
+// Setup: device Ruiz equilibration (ruiz_scaling.cpp:152-247) on the same
+// wave tasks.  All maxima are order-independent and every scaling is a
+// product, so the result equals the serial host pass bit for bit.
+// Nonnegative doubles order like their bit patterns, so atomicMax on the
+// 64-bit pattern is an exact max.
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
     atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void k_ruiz_colmax(Dev d) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.n) return;
-    const int b = d.col_inst[j];
-    if (d.st[b].ruiz_done) return;
-    double mx = 0.0;
-    for (int p = d.cp[j]; p < d.cp[j + 1]; ++p) mx = maxv(mx, fabs(d.tv64[p]));
-    if (mx == 0.0) mx = 1.0;
-    d.ctmp[j] = mx;
-    atomic_max_nonneg(&d.st[b].ruiz_err_c, fabs(1.0 - mx));
+// Group max of |val| over one row; valid in every lane of the group.
+__device__ __forceinline__ double row_absmax(int p0, int p1, int gl, int G, const double* val) {
+    double m = 0.0;
+    for (int q = p0 + gl; q < p1; q += G) m = maxv(m, fabs(val[q]));
+    for (int off = G >> 1; off >= 1; off >>= 1) m = maxv(m, __shfl_xor(m, off, 64));
+    return m;
 }
 
-__global__ void k_ruiz_rowmax(Dev d) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.m) return;
-    const int b = d.row_inst[i];
-    if (d.st[b].ruiz_done) return;
-    double mx = 0.0;
-    for (int p = d.rp[i]; p < d.rp[i + 1]; ++p) mx = maxv(mx, fabs(d.av64[p]));
-    if (mx == 0.0) mx = 1.0;
-    d.rtmp[i] = mx;
-    atomic_max_nonneg(&d.st[b].ruiz_err_r, fabs(1.0 - mx));
+// ROWS = true: rows of A (row max -> rtmp, error -> ruiz_err_r);
+// false: rows of A^T = columns of A (-> ctmp, ruiz_err_c).
+template <bool ROWS>
+__global__ __launch_bounds__(kBlock) void k_ruiz_max(Dev d) {
+    const int nt = ROWS ? d.ntA : d.ntT;
+    const int tile = wave_tile(nt);
+    if (tile >= nt) return;
+    const Tile tl = (ROWS ? d.tA : d.tT)[tile];
+    InstState& s = d.st[tl.inst];
+    if (s.ruiz_done) return;
+    const Lanes L = lanes_of(tl);
+    const int32_t* ptr = ROWS ? d.rp : d.cp;
+    const double* val = ROWS ? d.av64 : d.tv64;
+    double* out = ROWS ? d.rtmp : d.ctmp;
+    double err = 0.0;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int r = base + (L.lane >> L.lg);
+        const bool own = r < tl.r1;
+        const double m = row_absmax(own ? ptr[r] : 0, own ? ptr[r + 1] : 0, L.gl, L.G, val);
+        if (own && L.gl == 0) {
+            const double mm = (m == 0.0) ? 1.0 : m;
+            out[r] = mm;
+            err = maxv(err, fabs(1.0 - mm));
+        }
+    }
+    err = wave_max(err);
+    if (L.lane == 0) atomic_max_nonneg(ROWS ? &s.ruiz_err_r : &s.ruiz_err_c, err);
 }
 
 __global__ void k_ruiz_decide(Dev d, double tol) {
@@ -646,77 +772,104 @@ __global__ void k_ruiz_deltas(Dev d) {
 }
 
 // A <- D_r A D_c on both stored orientations (value *= dr[i] * dc[j]).
-__global__ void k_ruiz_apply(Dev d) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < d.n) {
-        if (d.st[d.col_inst[k]].ruiz_done) return;
-        const double dc = d.ctmp[k];
-        for (int p = d.cp[k]; p < d.cp[k + 1]; ++p) d.tv64[p] *= d.rtmp[d.ri[p]] * dc;
-    } else if (k < d.n + d.m) {
-        const int i = k - d.n;
-        if (d.st[d.row_inst[i]].ruiz_done) return;
-        const double dr = d.rtmp[i];
-        for (int p = d.rp[i]; p < d.rp[i + 1]; ++p) d.av64[p] *= dr * d.ctmp[d.ci[p]];
+// A <- D_r A D_c on both stored orientations (value *= dr[i] * dc[j]).
+// Tiles of A^T first, then tiles of A; one wave per tile.
+__global__ __launch_bounds__(kBlock) void k_ruiz_apply(Dev d) {
+    const int g = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (g >= d.ntT + d.ntA) return;
+    const bool cols = g < d.ntT;
+    const Tile tl = cols ? d.tT[g] : d.tA[g - d.ntT];
+    if (d.st[tl.inst].ruiz_done) return;
+    const Lanes L = lanes_of(tl);
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int r = base + (L.lane >> L.lg);
+        if (r >= tl.r1) continue;
+        if (cols) {
+            const double dc = d.ctmp[r];
+            for (int p = d.cp[r] + L.gl; p < d.cp[r + 1]; p += L.G) d.tv64[p] *= d.rtmp[d.ri[p]] * dc;
+        } else {
+            const double dr = d.rtmp[r];
+            for (int p = d.rp[r] + L.gl; p < d.rp[r + 1]; p += L.G) d.av64[p] *= dr * d.ctmp[d.ci[p]];
+        }
     }
 }
 
 // Scaled bounds/objective (ruiz_scaling.cpp:221-244), sentinels and norms
-// (pdlp.cpp:153-176) and FP32 copies; once per loaded problem.  Original
-// bounds stay as IEEE +-inf for infinite kinds: every test the residuals make
-// on them (clamp, |proj| < 1e299, hi < 1e299) reads the same on inf as on the
-// reference's 1e300 sentinel.
-__global__ void k_setup_rows(Dev d) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.m) return;
-    const int b = d.row_inst[i];
-    const double rs = d.row_scale[i];
-    const double lo = d.orow_lo[i], hi = d.orow_hi[i];
-    const bool flo = !isinf(lo), fhi = !isinf(hi);
-    const double bl = flo ? lo * rs : -1e300;
-    const double bh = fhi ? hi * rs : 1e300;
-    d.blo[i] = bl;
-    d.bhi[i] = bh;
-    double bn = 0.0;
-    if (fabs(bl) < 1e299) bn = fabs(bl);
-    if (fabs(bh) < 1e299) bn = maxv(bn, fabs(bh));
-    atomic_max_nonneg(&d.st[b].b_norm_inf, bn);
-    double nrm = 0.0;
-    for (int p = d.rp[i]; p < d.rp[i + 1]; ++p) {
-        nrm += fabs(d.av64[p]);
-        d.av32[p] = (float)d.av64[p];
+// (pdlp.cpp:153-176) and FP32 copies; once per loaded problem.  The L1 norms
+// are summed serially by the group's first lane in the reference's order.
+// Original bounds stay as IEEE +-inf for infinite kinds: every test the
+// residuals make on them (clamp, |proj| < 1e299, hi < 1e299) reads the same
+// on inf as on the reference's 1e300 sentinel.
+__global__ __launch_bounds__(kBlock) void k_setup_rows(Dev d) {
+    const int tile = wave_tile(d.ntA);
+    if (tile >= d.ntA) return;
+    const Tile tl = d.tA[tile];
+    const Lanes L = lanes_of(tl);
+    double bn_max = 0.0;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int i = base + (L.lane >> L.lg);
+        if (i >= tl.r1) continue;
+        const int p0 = d.rp[i], p1 = d.rp[i + 1];
+        for (int p = p0 + L.gl; p < p1; p += L.G) d.av32[p] = (float)d.av64[p];
+        if (L.gl != 0) continue;
+        const double rs = d.row_scale[i];
+        const double lo = d.orow_lo[i], hi = d.orow_hi[i];
+        const bool flo = !isinf(lo), fhi = !isinf(hi);
+        const double bl = flo ? lo * rs : -1e300;
+        const double bh = fhi ? hi * rs : 1e300;
+        d.blo[i] = bl;
+        d.bhi[i] = bh;
+        if (fabs(bl) < 1e299) bn_max = maxv(bn_max, fabs(bl));
+        if (fabs(bh) < 1e299) bn_max = maxv(bn_max, fabs(bh));
+        double nrm = 0.0;
+        for (int p = p0; p < p1; ++p) nrm += fabs(d.av64[p]);
+        if (nrm < 1e-12) nrm = 1.0;
+        d.rnorm[i] = nrm;
+        d.rowpk[i] = make_float4(flo ? (float)bl : -INFINITY, fhi ? (float)bh : INFINITY, (float)(1.0 / nrm),
+                                 0.0f);
     }
-    if (nrm < 1e-12) nrm = 1.0;
-    d.rnorm[i] = nrm;
-    d.rowpk[i] = make_float4(flo ? (float)bl : -INFINITY, fhi ? (float)bh : INFINITY,
-                             (float)(1.0 / nrm), 0.0f);
+    bn_max = wave_max(bn_max);
+    if (L.lane == 0) atomic_max_nonneg(&d.st[tl.inst].b_norm_inf, bn_max);
 }
 
-__global__ void k_setup_cols(Dev d) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.n) return;
-    const int b = d.col_inst[j];
-    const double cs = d.col_scale[j];
-    const double inv = 1.0 / cs;
-    const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
-    const bool flo = !isinf(lo), fhi = !isinf(hi);
-    const double sl = flo ? lo * inv : -1e300;
-    const double sh = fhi ? hi * inv : 1e300;
-    d.slo[j] = sl;
-    d.shi[j] = sh;
-    const double sense = d.sense[b];
-    const double cj = sense * (d.obj[j] * cs);
-    d.c[j] = cj;
-    atomic_max_nonneg(&d.st[b].c_norm_inf, fabs(cj));
-    atomic_max_nonneg(&d.st[b].c_scale, fabs(sense * d.obj[j]));
-    double nrm = 0.0;
-    for (int p = d.cp[j]; p < d.cp[j + 1]; ++p) {
-        nrm += fabs(d.tv64[p]);
-        d.tv32[p] = (float)d.tv64[p];
+__global__ __launch_bounds__(kBlock) void k_setup_cols(Dev d) {
+    const int tile = wave_tile(d.ntT);
+    if (tile >= d.ntT) return;
+    const Tile tl = d.tT[tile];
+    const Lanes L = lanes_of(tl);
+    const double sense = d.sense[tl.inst];
+    double cmax = 0.0, cs_max = 0.0;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int j = base + (L.lane >> L.lg);
+        if (j >= tl.r1) continue;
+        const int p0 = d.cp[j], p1 = d.cp[j + 1];
+        for (int p = p0 + L.gl; p < p1; p += L.G) d.tv32[p] = (float)d.tv64[p];
+        if (L.gl != 0) continue;
+        const double cs = d.col_scale[j];
+        const double inv = 1.0 / cs;
+        const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
+        const bool flo = !isinf(lo), fhi = !isinf(hi);
+        const double sl = flo ? lo * inv : -1e300;
+        const double sh = fhi ? hi * inv : 1e300;
+        d.slo[j] = sl;
+        d.shi[j] = sh;
+        const double cj = sense * (d.obj[j] * cs);
+        d.c[j] = cj;
+        cmax = maxv(cmax, fabs(cj));
+        cs_max = maxv(cs_max, fabs(sense * d.obj[j]));
+        double nrm = 0.0;
+        for (int p = p0; p < p1; ++p) nrm += fabs(d.tv64[p]);
+        if (nrm < 1e-12) nrm = 1.0;
+        d.cnorm[j] = nrm;
+        d.colpk[j] = make_float4((float)cj, flo ? (float)sl : -INFINITY, fhi ? (float)sh : INFINITY,
+                                 (float)(1.0 / nrm));
     }
-    if (nrm < 1e-12) nrm = 1.0;
-    d.cnorm[j] = nrm;
-    d.colpk[j] = make_float4((float)cj, flo ? (float)sl : -INFINITY, fhi ? (float)sh : INFINITY,
-                             (float)(1.0 / nrm));
+    cmax = wave_max(cmax);
+    cs_max = wave_max(cs_max);
+    if (L.lane == 0) {
+        atomic_max_nonneg(&d.st[tl.inst].c_norm_inf, cmax);
+        atomic_max_nonneg(&d.st[tl.inst].c_scale, cs_max);
+    }
 }
 
 // Start point of a solve (pdlp.cpp:186-191): x = proj(0) (or the projected
@@ -795,70 +948,65 @@ __global__ void k_fill(double* p, double v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-template <typename T, bool EXACT>
-__global__ __launch_bounds__(kBlock) void k_spmv_plain(const Tile* tiles, int ntiles,
-                                                       const int32_t* ptr, const int32_t* idx,
-                                                       const T* val, const T* vec, T* out) {
-    const int tile = xcd_tile(ntiles);
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr,
+                                                       const int32_t* idx, const T* val, const T* vec, T* out) {
+    const int tile = wave_tile(ntiles);
     if (tile >= ntiles) return;
     const Tile tl = tiles[tile];
-    __shared__ T prod[kNnzTile];
-    __shared__ T rsum[kBlock];
-    __shared__ T red[kWaves];
-    const int nr = tile_rowsums<EXACT, T>(tl, ptr, idx, val, vec, prod, rsum, red);
-    if ((int)threadIdx.x < nr) out[tl.r0 + threadIdx.x] = rsum[threadIdx.x];
+    const Lanes L = lanes_of(tl);
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int r = base + (L.lane >> L.lg);
+        const bool own = r < tl.r1;
+        const T v = row_dot<T>(own ? ptr[r] : 0, own ? ptr[r + 1] : 0, L.gl, L.G, idx, val, vec);
+        if (own && L.gl == 0) out[r] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Host launch helpers.
-static inline int tile_grid(int ntiles) { return kXcd * ((ntiles + kXcd - 1) / kXcd); }
+static inline int tile_grid(int ntiles) {
+    const int nblk = (ntiles + kWaves - 1) / kWaves;
+    return kXcd * ((nblk + kXcd - 1) / kXcd);
+}
 static inline int elem_grid(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
-void launch_primal(const Dev& d, int t, bool fp32, bool adapt, bool exact, hipStream_t s) {
+void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s) {
     if (d.ntT == 0) return;
     const dim3 g(tile_grid(d.ntT)), b(kBlock);
-    if (fp32) {
+    if (exact) {
         if (adapt)
-            hipLaunchKernelGGL((k_primal<float, true, false>), g, b, 0, s, d, t);
+            hipLaunchKernelGGL((k_primal<true, true>), g, b, 0, s, d, t);
         else
-            hipLaunchKernelGGL((k_primal<float, false, false>), g, b, 0, s, d, t);
-    } else if (exact) {
-        if (adapt)
-            hipLaunchKernelGGL((k_primal<double, true, true>), g, b, 0, s, d, t);
-        else
-            hipLaunchKernelGGL((k_primal<double, false, true>), g, b, 0, s, d, t);
+            hipLaunchKernelGGL((k_primal<false, true>), g, b, 0, s, d, t);
     } else {
         if (adapt)
-            hipLaunchKernelGGL((k_primal<double, true, false>), g, b, 0, s, d, t);
+            hipLaunchKernelGGL((k_primal<true, false>), g, b, 0, s, d, t);
         else
-            hipLaunchKernelGGL((k_primal<double, false, false>), g, b, 0, s, d, t);
+            hipLaunchKernelGGL((k_primal<false, false>), g, b, 0, s, d, t);
     }
 }
 
-void launch_dual(const Dev& d, int t, bool fp32, bool exact, hipStream_t s) {
+void launch_dual(const Dev& d, int t, bool exact, hipStream_t s) {
     if (d.ntA == 0) return;
     const dim3 g(tile_grid(d.ntA)), b(kBlock);
-    if (fp32)
-        hipLaunchKernelGGL((k_dual<float, false>), g, b, 0, s, d, t);
-    else if (exact)
-        hipLaunchKernelGGL((k_dual<double, true>), g, b, 0, s, d, t);
+    if (exact)
+        hipLaunchKernelGGL((k_dual<true>), g, b, 0, s, d, t);
     else
-        hipLaunchKernelGGL((k_dual<double, false>), g, b, 0, s, d, t);
+        hipLaunchKernelGGL((k_dual<false>), g, b, 0, s, d, t);
 }
 
-void launch_adx(const Dev& d, int t, bool fp32, bool exact, hipStream_t s) {
+void launch_adx(const Dev& d, int t, bool exact, hipStream_t s) {
     if (d.ntA == 0) return;
     const dim3 g(tile_grid(d.ntA)), b(kBlock);
-    if (fp32)
-        hipLaunchKernelGGL((k_adx<float, false>), g, b, 0, s, d, t);
-    else if (exact)
-        hipLaunchKernelGGL((k_adx<double, true>), g, b, 0, s, d, t);
+    if (exact)
+        hipLaunchKernelGGL((k_adx<true>), g, b, 0, s, d, t);
     else
-        hipLaunchKernelGGL((k_adx<double, false>), g, b, 0, s, d, t);
+        hipLaunchKernelGGL((k_adx<false>), g, b, 0, s, d, t);
 }
 
 void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s) {
-    const dim3 g(elem_grid(d.B)), b(kBlock);
+    const dim3 g(d.B), b(kFinal);
     if (exact)
         hipLaunchKernelGGL((k_adapt_finalize<true>), g, b, 0, s, d);
     else
@@ -866,38 +1014,34 @@ void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s) {
 }
 
 template <bool CUR>
-static void resid_pair(const Dev& d, bool fp32, bool exact, hipStream_t s) {
+static void resid_pair(const Dev& d, bool exact, hipStream_t s) {
     const dim3 b(kBlock);
     if (d.ntA > 0) {
         const dim3 g(tile_grid(d.ntA));
-        if (fp32)
-            hipLaunchKernelGGL((k_resid_rows<float, false, CUR>), g, b, 0, s, d);
-        else if (exact)
-            hipLaunchKernelGGL((k_resid_rows<double, true, CUR>), g, b, 0, s, d);
+        if (exact)
+            hipLaunchKernelGGL((k_resid_rows<true, CUR>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_rows<double, false, CUR>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_rows<false, CUR>), g, b, 0, s, d);
     }
     if (d.ntT > 0) {
         const dim3 g(tile_grid(d.ntT));
-        if (fp32)
-            hipLaunchKernelGGL((k_resid_cols<float, false, CUR>), g, b, 0, s, d);
-        else if (exact)
-            hipLaunchKernelGGL((k_resid_cols<double, true, CUR>), g, b, 0, s, d);
+        if (exact)
+            hipLaunchKernelGGL((k_resid_cols<true, CUR>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_cols<double, false, CUR>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols<false, CUR>), g, b, 0, s, d);
     }
 }
 
-void launch_resid(const Dev& d, bool fp32, bool exact, bool current, hipStream_t s) {
+void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s) {
     if (current)
-        resid_pair<true>(d, fp32, exact, s);
+        resid_pair<true>(d, exact, s);
     else
-        resid_pair<false>(d, fp32, exact, s);
+        resid_pair<false>(d, exact, s);
 }
 
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
                            hipStream_t s) {
-    const dim3 g(elem_grid(d.B)), b(kBlock);
+    const dim3 g(d.B), b(kFinal);
     if (exact) {
         if (final_pass)
             hipLaunchKernelGGL((k_check_finalize<true, true>), g, b, 0, s, d, o, t);
@@ -912,8 +1056,9 @@ void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, b
 }
 
 void launch_post_check(const Dev& d, bool kkt, hipStream_t s) {
-    if (d.ntT + d.ntA == 0) return;
-    hipLaunchKernelGGL(k_post_check, dim3(d.ntT + d.ntA), dim3(kBlock), 0, s, d, kkt ? 1 : 0);
+    const int nt = d.ntT + d.ntA;
+    if (nt == 0) return;
+    hipLaunchKernelGGL(k_post_check, dim3((nt + kWaves - 1) / kWaves), dim3(kBlock), 0, s, d, kkt ? 1 : 0);
 }
 
 void launch_advance(const Dev& d, int len, hipStream_t s) {
@@ -922,17 +1067,18 @@ void launch_advance(const Dev& d, int len, hipStream_t s) {
 
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s) {
     const dim3 b(kBlock);
-    hipLaunchKernelGGL(k_ruiz_colmax, dim3(elem_grid(d.n)), b, 0, s, d);
-    hipLaunchKernelGGL(k_ruiz_rowmax, dim3(elem_grid(d.m)), b, 0, s, d);
+    if (d.ntT) hipLaunchKernelGGL((k_ruiz_max<false>), dim3(tile_grid(d.ntT)), b, 0, s, d);
+    if (d.ntA) hipLaunchKernelGGL((k_ruiz_max<true>), dim3(tile_grid(d.ntA)), b, 0, s, d);
     hipLaunchKernelGGL(k_ruiz_decide, dim3(elem_grid(d.B)), b, 0, s, d, tol);
     hipLaunchKernelGGL(k_ruiz_deltas, dim3(elem_grid((int64_t)d.m + d.n)), b, 0, s, d);
-    hipLaunchKernelGGL(k_ruiz_apply, dim3(elem_grid((int64_t)d.m + d.n)), b, 0, s, d);
+    const int nt = d.ntT + d.ntA;
+    if (nt) hipLaunchKernelGGL(k_ruiz_apply, dim3((nt + kWaves - 1) / kWaves), b, 0, s, d);
 }
 
 void launch_setup_data(const Dev& d, hipStream_t s) {
     const dim3 b(kBlock);
-    if (d.m > 0) hipLaunchKernelGGL(k_setup_rows, dim3(elem_grid(d.m)), b, 0, s, d);
-    if (d.n > 0) hipLaunchKernelGGL(k_setup_cols, dim3(elem_grid(d.n)), b, 0, s, d);
+    if (d.ntA) hipLaunchKernelGGL(k_setup_rows, dim3(tile_grid(d.ntA)), b, 0, s, d);
+    if (d.ntT) hipLaunchKernelGGL(k_setup_cols, dim3(tile_grid(d.ntT)), b, 0, s, d);
 }
 
 void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm_y, hipStream_t s) {
@@ -958,20 +1104,16 @@ void launch_fill(double* p, double v, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(elem_grid(n)), dim3(kBlock), 0, s, p, v, n);
 }
 
-void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const int32_t* idx,
-                       const void* val, const void* vec, void* out, bool fp32, bool exact,
-                       hipStream_t s) {
+void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const int32_t* idx, const void* val,
+                       const void* vec, void* out, bool fp32, hipStream_t s) {
     if (ntiles == 0) return;
     const dim3 g(tile_grid(ntiles)), b(kBlock);
     if (fp32)
-        hipLaunchKernelGGL((k_spmv_plain<float, false>), g, b, 0, s, tiles, ntiles, ptr, idx,
-                           (const float*)val, (const float*)vec, (float*)out);
-    else if (exact)
-        hipLaunchKernelGGL((k_spmv_plain<double, true>), g, b, 0, s, tiles, ntiles, ptr, idx,
-                           (const double*)val, (const double*)vec, (double*)out);
+        hipLaunchKernelGGL((k_spmv_plain<float>), g, b, 0, s, tiles, ntiles, ptr, idx, (const float*)val,
+                           (const float*)vec, (float*)out);
     else
-        hipLaunchKernelGGL((k_spmv_plain<double, false>), g, b, 0, s, tiles, ntiles, ptr, idx,
-                           (const double*)val, (const double*)vec, (double*)out);
+        hipLaunchKernelGGL((k_spmv_plain<double>), g, b, 0, s, tiles, ntiles, ptr, idx, (const double*)val,
+                           (const double*)vec, (double*)out);
 }
 
 }  // namespace mcp

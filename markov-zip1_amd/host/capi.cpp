@@ -54,6 +54,7 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
     p.primal_weight_smoothing = o.primal_weight_smoothing;
     p.ruiz_scaling = o.ruiz_scaling != 0;
     p.ruiz_iterations = o.ruiz_iterations;
+    p.autotune = o.autotune != 0;
     p.precision = o.precision == MC_PRECISION_MIXED ? fo::Precision::mixed : fo::Precision::fp64;
     p.reduction_order = o.reduction_order == MC_ORDER_REFERENCE ? fo::ReductionOrder::reference
                                                                 : fo::ReductionOrder::fast;
@@ -160,7 +161,12 @@ int mc_solve_pdlp_batch(const mc_lp* lps, int32_t count, const mc_pdlp_options* 
         if (count < 0 || (count > 0 && (!lps || !res)) || !opt) throw std::invalid_argument("null argument");
         std::vector<mc_lp> v(lps, lps + count);
         for (const auto& lp : v) check_lp(&lp);
-        const auto o = to_options(*opt, 0, 0);
+        int64_t totn = 0, totm = 0;
+        for (const auto& lp : v) {
+            totn += lp.cols;
+            totm += lp.rows;
+        }
+        const auto o = to_options(*opt, totn, totm);
         std::vector<fo::PdlpResult> out;
         auto host = [&](const char* label) {
             out.clear();

@@ -40,8 +40,9 @@ mc_lp view_of(const model::Model& md) {
     return v;
 }
 
-void csc_to_csr(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, const double* val, int32_t* rp,
-                int32_t* ci, double* rv, int threads) {
+template <class I>
+void csc_to_csr(int64_t m, int64_t n, const I* cs, const I* ri, const double* val, int32_t* rp, int32_t* ci,
+                double* rv, int threads) {
     const int64_t nnz = n > 0 ? cs[n] : 0;
     threads = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(1, nnz / (1 << 20))));
     // counts[t][i]: entries of row i in worker t's column range; worker t's
@@ -81,7 +82,49 @@ void csc_to_csr(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, cons
     });
 }
 
+template void csc_to_csr<int64_t>(int64_t, int64_t, const int64_t*, const int64_t*, const double*, int32_t*,
+                                  int32_t*, double*, int);
+template void csc_to_csr<int32_t>(int64_t, int64_t, const int32_t*, const int32_t*, const double*, int32_t*,
+                                  int32_t*, double*, int);
+
 namespace {
+
+template <class F>
+void parallel_for(int64_t n, int threads, F&& fn) {
+    threads = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(1, n / 4096)));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back([&, t] { fn(n * t / threads, n * (t + 1) / threads); });
+    fn(0, n / threads);
+    for (auto& th : pool) th.join();
+}
+
+// Length bucket of a row/column: exact below 32 nonzeros, log2 classes above.
+inline int length_bucket(int64_t len) {
+    if (len < 32) return (int)len;
+    int b = 32;
+    while (len >= 64) {
+        len >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+// Stable order of [0, k) by descending length bucket: neighbouring rows of a
+// tile then have similar lengths, and rows inside a bucket keep their order.
+std::vector<int32_t> bucket_order(const std::vector<int64_t>& len) {
+    const int64_t k = (int64_t)len.size();
+    std::vector<int32_t> order(k);
+    std::vector<int64_t> count(64, 0);
+    for (int64_t i = 0; i < k; ++i) ++count[63 - length_bucket(len[i])];
+    int64_t acc = 0;
+    for (auto& c : count) {
+        const int64_t t = c;
+        c = acc;
+        acc += t;
+    }
+    for (int64_t i = 0; i < k; ++i) order[count[63 - length_bucket(len[i])]++] = (int32_t)i;
+    return order;
+}
 
 struct EngineDeleter {
     void operator()(mcp_engine* e) const { mcp_engine_destroy(e); }
@@ -147,6 +190,12 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     if (m >= kMax || n >= kMax || nnz >= kMax) throw std::invalid_argument("problem exceeds int32 indexing");
     std::vector<int32_t> inst_row(B + 1), inst_col(B + 1), rp(m + 1), ci(nnz), cp(n + 1), ri(nnz);
     std::vector<double> rv(nnz), cv(nnz), rlo(m), rhi(m), vlo(n), vhi(n), obj(n), sense(B), offset(B);
+    // In the fast reduction order rows and columns are relabelled into length
+    // buckets (row_perm[new] = old, col_perm[new] = old, per instance), so each
+    // SpMV tile holds rows of similar length.  The reference order keeps the
+    // model's labels: its sums must run in the reference's index order.
+    const bool relabel = opt.reduction_order != ReductionOrder::reference;
+    std::vector<int32_t> row_perm(m), col_perm(n);
     const double inf = std::numeric_limits<double>::infinity();
     const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
     {
@@ -156,22 +205,66 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
             inst_row[b] = (int32_t)r0;
             inst_col[b] = (int32_t)c0;
             const int64_t lm = lp.rows, ln = lp.cols, lnz = lp.column_start[ln];
-            for (int64_t j = 0; j <= ln; ++j) cp[c0 + j] = (int32_t)(p0 + lp.column_start[j]);
-            for (int64_t k = 0; k < lnz; ++k) ri[p0 + k] = (int32_t)(r0 + lp.row_index[k]);
-            std::memcpy(cv.data() + p0, lp.value, sizeof(double) * lnz);
-            csc_to_csr(lm, ln, lp.column_start, lp.row_index, lp.value, rp.data() + r0, ci.data() + p0,
-                       rv.data() + p0, B == 1 ? hw : 1);
+            const int th = B == 1 ? hw : 1;
+            std::vector<int32_t> cperm(ln), rinv(lm);
+            if (relabel) {
+                std::vector<int64_t> clen(ln), rlen(lm, 0);
+                for (int64_t j = 0; j < ln; ++j) clen[j] = lp.column_start[j + 1] - lp.column_start[j];
+                for (int64_t k = 0; k < lnz; ++k) ++rlen[lp.row_index[k]];
+                cperm = bucket_order(clen);
+                const auto rperm = bucket_order(rlen);
+                for (int64_t i = 0; i < lm; ++i) {
+                    rinv[rperm[i]] = (int32_t)i;
+                    row_perm[r0 + i] = (int32_t)(r0 + rperm[i]);
+                }
+            } else {
+                for (int64_t j = 0; j < ln; ++j) cperm[j] = (int32_t)j;
+                for (int64_t i = 0; i < lm; ++i) {
+                    rinv[i] = (int32_t)i;
+                    row_perm[r0 + i] = (int32_t)(r0 + i);
+                }
+            }
+            // local CSC in the new labels (entries of a column sorted by new row)
+            std::vector<int32_t> lcs(ln + 1), lri(lnz);
+            std::vector<double> lcv(lnz);
+            lcs[0] = 0;
+            for (int64_t j = 0; j < ln; ++j) {
+                const int64_t oj = cperm[j];
+                lcs[j + 1] = lcs[j] + (int32_t)(lp.column_start[oj + 1] - lp.column_start[oj]);
+            }
+            parallel_for(ln, th, [&](int64_t j0, int64_t j1) {
+                std::vector<std::pair<int32_t, double>> tmp;
+                for (int64_t j = j0; j < j1; ++j) {
+                    const int64_t oj = cperm[j];
+                    const int64_t a = lp.column_start[oj], e = lp.column_start[oj + 1];
+                    tmp.clear();
+                    for (int64_t k = a; k < e; ++k) tmp.emplace_back(rinv[lp.row_index[k]], lp.value[k]);
+                    if (relabel) std::sort(tmp.begin(), tmp.end(), [](auto& u, auto& v) { return u.first < v.first; });
+                    for (int64_t k = 0; k < e - a; ++k) {
+                        lri[lcs[j] + k] = tmp[k].first;
+                        lcv[lcs[j] + k] = tmp[k].second;
+                    }
+                }
+            });
+            for (int64_t j = 0; j <= ln; ++j) cp[c0 + j] = (int32_t)(p0 + lcs[j]);
+            for (int64_t k = 0; k < lnz; ++k) ri[p0 + k] = (int32_t)(r0 + lri[k]);
+            std::memcpy(cv.data() + p0, lcv.data(), sizeof(double) * lnz);
+            csc_to_csr<int32_t>(lm, ln, lcs.data(), lri.data(), lcv.data(), rp.data() + r0, ci.data() + p0,
+                                rv.data() + p0, th);
             for (int64_t i = 0; i <= lm; ++i) rp[r0 + i] += (int32_t)p0;
             for (int64_t k = 0; k < lnz; ++k) ci[p0 + k] += (int32_t)c0;
             for (int64_t i = 0; i < lm; ++i) {
-                rlo[r0 + i] = bound_value(lp.row_lower[i], -inf);
-                rhi[r0 + i] = bound_value(lp.row_upper[i], inf);
+                const int64_t oi = row_perm[r0 + i] - r0;
+                rlo[r0 + i] = bound_value(lp.row_lower[oi], -inf);
+                rhi[r0 + i] = bound_value(lp.row_upper[oi], inf);
             }
             for (int64_t j = 0; j < ln; ++j) {
-                vlo[c0 + j] = bound_value(lp.var_lower[j], -inf);
-                vhi[c0 + j] = bound_value(lp.var_upper[j], inf);
+                const int64_t oj = cperm[j];
+                col_perm[c0 + j] = (int32_t)(c0 + oj);
+                vlo[c0 + j] = bound_value(lp.var_lower[oj], -inf);
+                vhi[c0 + j] = bound_value(lp.var_upper[oj], inf);
+                obj[c0 + j] = lp.objective[oj];
             }
-            std::memcpy(obj.data() + c0, lp.objective, sizeof(double) * ln);
             sense[b] = lp.sense == MC_MAXIMIZE ? -1.0 : 1.0;
             offset[b] = lp.objective_offset;
             r0 += lm;
@@ -210,10 +303,20 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     so.autotune = opt.autotune ? 1 : 0;
     check(mcp_setup(eng, &so), "mcp_setup");
 
-    // warm start (packed)
+    // warm start: vectors concatenated over all input LPs in input order
     std::vector<double> wx, wy;
-    const bool warm_x = B == 1 && opt.warm_primal.size() == (std::size_t)n;
-    const bool warm_y = B == 1 && opt.warm_dual.size() == (std::size_t)m;
+    int64_t totn = 0, totm = 0;
+    std::vector<int64_t> ocol(lps.size()), orow(lps.size());
+    for (std::size_t k = 0; k < lps.size(); ++k) {
+        ocol[k] = totn;
+        orow[k] = totm;
+        totn += lps[k].cols;
+        totm += lps[k].rows;
+    }
+    const bool warm_x = !opt.warm_primal.empty() && opt.warm_primal.size() == (std::size_t)totn;
+    const bool warm_y = !opt.warm_dual.empty() && opt.warm_dual.size() == (std::size_t)totm;
+    if ((!opt.warm_primal.empty() && !warm_x) || (!opt.warm_dual.empty() && !warm_y))
+        throw std::invalid_argument("warm start vectors must hold the columns/rows of all models");
     mcp_solve_opts o{};
     o.max_iterations = opt.max_iterations;
     o.restart_every = opt.restart_every;
@@ -234,14 +337,28 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
     o.graph_chunk = opt.graph_chunk;
     o.restart_scheme = opt.restart_scheme == RestartScheme::kkt ? MC_SCHEME_KKT : MC_SCHEME_REFERENCE;
-    o.warm_primal = warm_x ? opt.warm_primal.data() : nullptr;
-    o.warm_dual = warm_y ? opt.warm_dual.data() : nullptr;
+    if (warm_x) {
+        wx.resize(n);
+        for (int b = 0; b < B; ++b)
+            for (int64_t p = inst_col[b]; p < inst_col[b + 1]; ++p)
+                wx[p] = opt.warm_primal[ocol[live[b]] + (col_perm[p] - inst_col[b])];
+    }
+    if (warm_y) {
+        wy.resize(m);
+        for (int b = 0; b < B; ++b)
+            for (int64_t p = inst_row[b]; p < inst_row[b + 1]; ++p)
+                wy[p] = opt.warm_dual[orow[live[b]] + (row_perm[p] - inst_row[b])];
+    }
+    o.warm_primal = warm_x ? wx.data() : nullptr;
+    o.warm_dual = warm_y ? wy.data() : nullptr;
     mcp_solve_stats st{};
     check(mcp_solve(eng, &o, &st), "mcp_solve");
-    std::vector<double> x(n), y(m);
+    std::vector<double> xp(n), yp(m), x(n), y(m);
     std::vector<mcp_inst_result> ir(B);
     double d2h = 0.0;
-    check(mcp_fetch(eng, x.data(), y.data(), ir.data(), &d2h), "mcp_fetch");
+    check(mcp_fetch(eng, xp.data(), yp.data(), ir.data(), &d2h), "mcp_fetch");
+    for (int64_t j = 0; j < n; ++j) x[col_perm[j]] = xp[j];
+    for (int64_t i = 0; i < m; ++i) y[row_perm[i]] = yp[i];
     const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (int b = 0; b < B; ++b) {
         auto& r = out[live[b]];

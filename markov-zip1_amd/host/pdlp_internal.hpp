@@ -26,9 +26,10 @@ model::Model model_of(const mc_lp& lp);
 PdlpResult solve_cpu(const model::Model& md, const PdlpOptions& opt, const char* backend_label);
 std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptions& opt);
 
-// CSC (int64 pointers/indices) -> CSR (int32), entries of a row in increasing
-// column order; `threads` workers split the columns.
-void csc_to_csr(int64_t m, int64_t n, const int64_t* col_start, const int64_t* row_index, const double* val,
+// CSC -> CSR (int32), entries of a row in increasing column order;
+// `threads` workers split the columns.
+template <class I>
+void csc_to_csr(int64_t m, int64_t n, const I* col_start, const I* row_index, const double* val,
                 int32_t* row_ptr, int32_t* col_idx, double* row_val, int threads);
 
 }  // namespace markov_cero::lp::first_order::detail

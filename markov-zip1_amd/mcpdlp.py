@@ -61,7 +61,7 @@ class Options(C.Structure):
         ("restart_reduction_factor", C.c_double),
         ("adaptive_step_size", C.c_int32), ("adaptive_primal_weight", C.c_int32),
         ("initial_primal_weight", C.c_double), ("primal_weight_smoothing", C.c_double),
-        ("ruiz_scaling", C.c_int32), ("pad0_", C.c_int32), ("ruiz_iterations", C.c_uint64),
+        ("ruiz_scaling", C.c_int32), ("autotune", C.c_int32), ("ruiz_iterations", C.c_uint64),
         ("precision", C.c_int32), ("reduction_order", C.c_int32),
         ("polish_threshold", C.c_double),
         ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("restart_scheme", C.c_int32),
@@ -202,6 +202,7 @@ def engine_lib():
         lib.mcp_get_scaling.argtypes = [C.c_void_p, P_f64, P_f64, P_f64, P_i32, P_i32]
         lib.mcp_get_tiling.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy), P_i32, P_i32]
         lib.mcp_time_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P_f64, P_f64]
+        lib.mcp_set_tile_policy.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy)]
         _engine = lib
     return _engine
 
@@ -341,6 +342,15 @@ def generate(family: int = FAMILY_UNIFORM_EQ, **spec) -> LPData:
         return _lp_from_handle(h)
     finally:
         lib.mc_lp_free(h)
+
+
+def generate_batch(count: int = 256, m_lo: int = 500, m_hi: int = 20000, seed: int = 0x4D4350444C50 + 300):
+    """Uneven batch of planted LPs: m log-uniform in [m_lo, m_hi], n = 2.5 m,
+    4-8 nonzeros per column (+-U[0.1,10]), equality rows, 0 <= x <= U[1,100]."""
+    rng = np.random.default_rng(seed)
+    ms = np.rint(np.exp(rng.uniform(np.log(m_lo), np.log(m_hi), size=count))).astype(int)
+    return [generate(FAMILY_UNIFORM_EQ, rows=int(m), cols=int(round(2.5 * m)), nnz_min=4, nnz_max=8,
+                     upper_lo=1.0, upper_hi=100.0, seed=int(seed + k)) for k, m in enumerate(ms)]
 
 
 def generate_family(base_spec: dict, seeds, sigma: float = 0.1, family: int = FAMILY_UNIFORM_EQ):
@@ -532,6 +542,10 @@ class Engine:
                                             _ptr(cs, C.c_double), C.byref(passes), C.byref(conv)))
         return cv[:nnz], rs[: self.m], cs[: self.n], passes.value, conv.value
 
+    def set_tiling(self, rows=(1024, 4), cols=(1024, 4)):
+        a, t = TilePolicy(*rows), TilePolicy(*cols)
+        _check_eng(self.lib.mcp_set_tile_policy(self.h, C.byref(a), C.byref(t)))
+
     def tiling(self):
         a, t = TilePolicy(), TilePolicy()
         na, nt = C.c_int32(), C.c_int32()
@@ -582,3 +596,111 @@ class Engine:
                         "x": x[self.inst_col[b]:self.inst_col[b + 1]].copy(),
                         "y": y[self.inst_row[b]:self.inst_row[b + 1]].copy()})
         return out, d2h.value
+
+
+# ---- learned warm start network (mcpdlp_mlp.h) ------------------------------
+def _mlp_lib():
+    lib = engine_lib()
+    if not getattr(lib, "_mlp_bound", False):
+        lib.mcp_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                       C.POINTER(C.c_void_p)]
+        lib.mcp_mlp_destroy.argtypes = [C.c_void_p]
+        lib.mcp_mlp_last_error.restype = C.c_char_p
+        lib.mcp_mlp_param_count.argtypes = [C.c_void_p, P_i64]
+        lib.mcp_mlp_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        lib.mcp_mlp_set_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        lib.mcp_mlp_forward.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_float)]
+        lib.mcp_mlp_loss_grad.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        lib.mcp_mlp_train.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_float, C.c_uint64, C.POINTER(C.c_float), P_f64]
+        lib.mcp_gemm_bf16_test.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float), C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+        lib._mlp_bound = True
+    return lib
+
+
+def _chk_mlp(rc):
+    if rc != 0:
+        raise RuntimeError(f"mcpdlp mlp error {rc}: {_mlp_lib().mcp_mlp_last_error().decode()}")
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def gemm_bf16(A, B, transpose_a=False, transpose_b=False, device=0):
+    """op(A) @ op(B) on MFMA with A, B rounded to BF16 (FP32 accumulation)."""
+    A, B = _f32(A), _f32(B)
+    M = A.shape[1] if transpose_a else A.shape[0]
+    K = A.shape[0] if transpose_a else A.shape[1]
+    N = B.shape[0] if transpose_b else B.shape[1]
+    out = np.zeros((M, N), np.float32)
+    _chk_mlp(_mlp_lib().mcp_gemm_bf16_test(device, M, N, K, _ptr(A, C.c_float), _ptr(B, C.c_float),
+                                           int(transpose_a), int(transpose_b), _ptr(out, C.c_float)))
+    return out
+
+
+class MLP:
+    """BF16 MLP on MFMA: in -> (hidden, ReLU) x layers -> out (widths multiples of 32)."""
+
+    def __init__(self, in_dim=256, hidden=256, layers=3, out_dim=64, capacity=256, seed=1, device=0):
+        self.lib = _mlp_lib()
+        self.h = C.c_void_p()
+        _chk_mlp(self.lib.mcp_mlp_create(device, in_dim, hidden, layers, out_dim, capacity, seed, C.byref(self.h)))
+        n = C.c_int64()
+        self.lib.mcp_mlp_param_count(self.h, C.byref(n))
+        self.nparam = n.value
+        self.dims = [in_dim] + [hidden] * layers + [out_dim]
+        self.capacity = capacity
+
+    def close(self):
+        if self.h:
+            self.lib.mcp_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        _chk_mlp(self.lib.mcp_mlp_get_params(self.h, _ptr(p, C.c_float)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        _chk_mlp(self.lib.mcp_mlp_set_params(self.h, _ptr(p, C.c_float)))
+
+    def unpack(self, p):
+        """List of (W [in x out], b [out]) per layer from a flat parameter vector."""
+        out, off = [], 0
+        for a, b in zip(self.dims[:-1], self.dims[1:]):
+            W = p[off:off + a * b].reshape(a, b)
+            off += a * b
+            out.append((W, p[off:off + b]))
+            off += b
+        return out
+
+    def forward(self, x):
+        x = _f32(x)
+        y = np.zeros((x.shape[0], self.dims[-1]), np.float32)
+        _chk_mlp(self.lib.mcp_mlp_forward(self.h, _ptr(x, C.c_float), x.shape[0], _ptr(y, C.c_float)))
+        return y
+
+    def loss_grad(self, x, t):
+        x, t = _f32(x), _f32(t)
+        loss = C.c_float()
+        g = np.zeros(self.nparam, np.float32)
+        _chk_mlp(self.lib.mcp_mlp_loss_grad(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), x.shape[0],
+                                            C.byref(loss), _ptr(g, C.c_float)))
+        return loss.value, g
+
+    def train(self, x, t, batch=64, epochs=50, lr=1e-3, seed=7):
+        x, t = _f32(x), _f32(t)
+        loss, ms = C.c_float(), C.c_double()
+        _chk_mlp(self.lib.mcp_mlp_train(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), x.shape[0], batch, epochs,
+                                        lr, seed, C.byref(loss), C.byref(ms)))
+        return loss.value, ms.value

@@ -1,0 +1,92 @@
+"""Learned warm start for a family of LPs that share one sparsity structure.
+
+Extension with no counterpart in the reference (its PDLP always starts at
+x = proj(0), y = 0; src/lp/first_order/pdlp.cpp:186-191).  Pipeline:
+
+1. features: the instance's data perturbation relative to the family's base
+   LP, d = [(b - b0) / (1 + |b0|), (c - c0) / (1 + |c0|), (u - u0) / (1 + |u0|)],
+   projected on the top principal directions of the training perturbations
+   (256 "summary features" of b, c and the bounds);
+2. targets: the GPU PDLP solutions of the training instances, z = [x / sx; y / sy],
+   projected on their top principal directions (64 coefficients of a reduced
+   solution basis);
+3. a BF16 MLP on MFMA (mcpdlp.MLP: hand-written forward/backward/Adam kernels)
+   maps standardized features to standardized coefficients;
+4. predict(): reconstruct z from the predicted coefficients, hand (x0, y0) to
+   solve_pdlp as warm_primal / warm_dual.  Cold start remains the fallback and
+   the KKT score alone decides convergence, so a bad prediction costs
+   iterations, never correctness.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import mcpdlp as mp
+
+
+def _rhs(lp: mp.LPData) -> np.ndarray:
+    """One finite right-hand side per row (upper if finite, else lower)."""
+    up = lp.row_upper["kind"] == mp.FINITE
+    return np.where(up, lp.row_upper["value"], lp.row_lower["value"]).astype(np.float64)
+
+
+class WarmStart:
+    def __init__(self, base: mp.LPData, n_features: int = 256, n_coeffs: int = 64, hidden: int = 256,
+                 layers: int = 3, seed: int = 1, device: int = 0):
+        self.base = base
+        self.n_features, self.n_coeffs = n_features, n_coeffs
+        self.hidden, self.layers, self.seed, self.device = hidden, layers, seed, device
+        self.b0, self.c0, self.u0 = _rhs(base), base.objective.astype(np.float64), base.var_upper["value"].copy()
+        self.net = None
+
+    def perturbation(self, lp: mp.LPData) -> np.ndarray:
+        return np.concatenate([(_rhs(lp) - self.b0) / (1.0 + np.abs(self.b0)),
+                               (lp.objective - self.c0) / (1.0 + np.abs(self.c0)),
+                               (lp.var_upper["value"] - self.u0) / (1.0 + np.abs(self.u0))])
+
+    def features(self, lps) -> np.ndarray:
+        D = np.stack([self.perturbation(lp) for lp in lps])
+        F = (D - self.d_mean) @ self.d_basis
+        return ((F - self.f_mean) / self.f_std).astype(np.float32)
+
+    @staticmethod
+    def _pca(X: np.ndarray, k: int):
+        mean = X.mean(axis=0)
+        _, _, vt = np.linalg.svd(X - mean, full_matrices=False)
+        basis = vt[:k].T
+        if basis.shape[1] < k:  # fewer samples than components: pad with zeros
+            basis = np.pad(basis, ((0, 0), (0, k - basis.shape[1])))
+        return mean, basis
+
+    def fit(self, lps, xs, ys, epochs: int = 300, batch: int = 64, lr: float = 1e-3) -> dict:
+        D = np.stack([self.perturbation(lp) for lp in lps])
+        self.d_mean, self.d_basis = self._pca(D, self.n_features)
+        F = (D - self.d_mean) @ self.d_basis
+        self.f_mean, self.f_std = F.mean(axis=0), F.std(axis=0) + 1e-12
+        Fs = ((F - self.f_mean) / self.f_std).astype(np.float32)
+        X, Y = np.stack(xs), np.stack(ys)
+        self.sx, self.sy = float(np.sqrt(np.mean(X * X))) + 1e-12, float(np.sqrt(np.mean(Y * Y))) + 1e-12
+        Z = np.concatenate([X / self.sx, Y / self.sy], axis=1)
+        self.z_mean, self.z_basis = self._pca(Z, self.n_coeffs)
+        Cf = (Z - self.z_mean) @ self.z_basis
+        self.c_mean, self.c_std = Cf.mean(axis=0), Cf.std(axis=0) + 1e-12
+        Cs = ((Cf - self.c_mean) / self.c_std).astype(np.float32)
+        explained = 1.0 - np.sum((Z - self.z_mean - Cf @ self.z_basis.T) ** 2) / np.sum((Z - self.z_mean) ** 2)
+        self.net = mp.MLP(self.n_features, self.hidden, self.layers, self.n_coeffs,
+                          capacity=max(32, batch), seed=self.seed, device=self.device)
+        loss, ms = self.net.train(Fs, Cs, batch=batch, epochs=epochs, lr=lr, seed=self.seed + 1)
+        return {"train_loss": loss, "train_ms": ms, "pca_explained": float(explained),
+                "samples": len(lps), "params": self.net.nparam}
+
+    def predict(self, lps):
+        C = self.net.forward(self.features(lps)).astype(np.float64) * self.c_std + self.c_mean
+        Z = self.z_mean + C @ self.z_basis.T
+        n = self.base.cols
+        xs = Z[:, :n] * self.sx
+        ys = Z[:, n:] * self.sy
+        lo, hi = self.base.var_lower["value"], None
+        out = []
+        for lp, x, y in zip(lps, xs, ys):
+            hi = lp.var_upper["value"]
+            out.append((np.clip(x, lo, hi), y))
+        return out
