@@ -269,7 +269,8 @@ def bench_warm(args):
     gen_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] training solutions: {sum(r['status'] == 'optimal' for r in sols)}/{ntr} optimal in {gen_ms:.0f}ms")
     ws = warmstart.WarmStart(base)
-    fit = ws.fit(train, [r["primal"] for r in sols], [r["dual"] for r in sols])
+    fit = ws.fit(train, [r["primal"] for r in sols], [r["dual"] for r in sols],
+                 omegas=[r["primal_weight"] for r in sols])
     log(f"[bench] MLP fit: {fit}")
     t1 = time.perf_counter()
     cold = mp.solve_batch(test, **kw)
@@ -277,7 +278,7 @@ def bench_warm(args):
     t2 = time.perf_counter()
     pred = ws.predict(test)
     warm = mp.solve_batch(test, warm_primal=np.concatenate([p[0] for p in pred]),
-                          warm_dual=np.concatenate([p[1] for p in pred]), **kw)
+                          warm_dual=np.concatenate([p[1] for p in pred]), initial_primal_weight=ws.omega, **kw)
     warm_ms = (time.perf_counter() - t2) * 1e3
     ci = np.array([r["iterations"] for r in cold])
     wi = np.array([r["iterations"] for r in warm])

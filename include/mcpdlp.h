@@ -158,6 +158,8 @@ typedef struct mc_pdlp_result {
     /* --- extensions --- */
     uint64_t fp32_iterations;       /* iterations run in FP32 before the FP64 polish (mixed) */
     double setup_ms;                /* Ruiz + transpose + norms on the device          */
+    double primal_weight;           /* final primal weight omega (scaled space)        */
+    double step_size;               /* final step-size factor eta                      */
 } mc_pdlp_result;
 
 /* Fill `opt` with the reference defaults (pdlp.hpp:19-34) and the extension defaults. */
@@ -237,6 +239,13 @@ void mc_lp_free(mc_lp_handle* h);
  * scaled by U[0.8,1.2], y* and s scaled by lognormal(0, sigma)) and b, c are
  * recomputed so the instance stays feasible with an exact planted optimum. */
 int mc_lp_perturb(const mc_lp_handle* base, uint64_t seed, double sigma, mc_lp_handle** out);
+
+/* Reads a fixed- or free-format MPS file (reference io::read_mps_file,
+ * include/markov_cero/io/mps.hpp and src/io/mps.cpp) into a new handle --
+ * the CLI's input format, for FFI callers.  The reference parser's size caps
+ * (mps.hpp:13-20) are not applied.  A handle read this way has no planted
+ * pair: mc_lp_planted() returns MC_ERR_INVALID_ARGUMENT for it. */
+int mc_lp_read_mps(const char* path, mc_lp_handle** out);
 
 #ifdef __cplusplus
 }

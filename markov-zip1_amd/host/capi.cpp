@@ -90,6 +90,8 @@ void to_result(const fo::PdlpResult& r, mc_pdlp_result* out) {
     if (out->dual && !r.dual.empty()) std::memcpy(out->dual, r.dual.data(), sizeof(double) * r.dual.size());
     out->fp32_iterations = r.fp32_iterations;
     out->setup_ms = r.setup_ms;
+    out->primal_weight = r.primal_weight;
+    out->step_size = r.step_size;
 }
 
 void check_lp(const mc_lp* lp) {

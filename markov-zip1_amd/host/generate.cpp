@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/mcpdlp.h"
+#include "markov_cero/mps.hpp"
 
 struct mc_lp_handle {
     int64_t m = 0, n = 0;
@@ -342,7 +343,10 @@ int mc_lp_view(const mc_lp_handle* h, mc_lp* v) {
 }
 
 int mc_lp_planted(const mc_lp_handle* h, const double** x_star, const double** y_star, double* planted) {
-    if (!h) return MC_ERR_INVALID_ARGUMENT;
+    if (!h || h->at.size() != (size_t)h->n) {
+        mc_internal_set_error("mc_lp_planted needs a generated (planted) instance");
+        return MC_ERR_INVALID_ARGUMENT;
+    }
     if (x_star) *x_star = h->xs.data();
     if (y_star) *y_star = h->ys.data();
     if (planted) *planted = h->planted_obj;
@@ -382,6 +386,48 @@ int mc_lp_perturb(const mc_lp_handle* base, uint64_t seed, double sigma, mc_lp_h
     } catch (const std::exception& e) {
         mc_internal_set_error(e.what());
         return MC_ERR_INTERNAL;
+    }
+}
+
+int mc_lp_read_mps(const char* path, mc_lp_handle** out) {
+    if (!out) return MC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!path) {
+        mc_internal_set_error("mc_lp_read_mps: null path");
+        return MC_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        const auto md = markov_cero::io::read_mps_file(path);
+        auto h = std::make_unique<mc_lp_handle>();
+        h->m = (int64_t)md.matrix.row_count;
+        h->n = (int64_t)md.matrix.column_count;
+        h->cs.assign(md.matrix.column_start.begin(), md.matrix.column_start.end());
+        h->ri.assign(md.matrix.row_index.begin(), md.matrix.row_index.end());
+        h->val = md.matrix.value;
+        h->obj = md.objective;
+        auto conv = [](const std::vector<markov_cero::model::Bound>& v) {
+            std::vector<mc_bound> o(v.size());
+            for (size_t k = 0; k < v.size(); ++k) {
+                o[k].kind = static_cast<int32_t>(v[k].kind);
+                o[k].pad_ = 0;
+                o[k].value = v[k].value;
+            }
+            return o;
+        };
+        h->rlo = conv(md.row_lower);
+        h->rhi = conv(md.row_upper);
+        h->vlo = conv(md.variable_lower);
+        h->vhi = conv(md.variable_upper);
+        h->sense = md.objective_sense == markov_cero::model::ObjectiveSense::maximize ? MC_MAXIMIZE : MC_MINIMIZE;
+        h->offset = md.objective_offset;
+        *out = h.release();
+        return MC_OK;
+    } catch (const std::bad_alloc&) {
+        mc_internal_set_error("out of memory");
+        return MC_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        mc_internal_set_error(std::string("mc_lp_read_mps: ") + e.what());
+        return MC_ERR_INVALID_ARGUMENT;
     }
 }
 

@@ -86,6 +86,8 @@ struct PdlpResult {
     // --- extensions ---
     std::size_t fp32_iterations{0};
     double setup_ms{0.0};
+    double primal_weight{0.0};  // final omega (scaled space): seeds initial_primal_weight of a warm start
+    double step_size{0.0};      // final eta
 };
 
 // Solve an LP with matrix-free PDHG (only A*x and A^T*y, no factorization).

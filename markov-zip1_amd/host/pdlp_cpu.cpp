@@ -308,6 +308,8 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     res.dual_infeasibility = cur.dual;
     res.duality_gap = cur.gap;
     res.iterations = iter;
+    res.primal_weight = p.omega;
+    res.step_size = p.eta;
     res.kernel_ms = ms;
     res.total_ms = ms;
     return res;

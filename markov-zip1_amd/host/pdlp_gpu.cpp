@@ -378,6 +378,8 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
         r.duality_gap = q.duality_gap;
         r.iterations = (std::size_t)q.iterations;
         r.fp32_iterations = (std::size_t)q.fp32_iterations;
+        r.primal_weight = q.omega;
+        r.step_size = q.eta;
         r.h2d_ms = st.h2d_ms;
         r.setup_ms = st.setup_ms;
         r.kernel_ms = st.kernel_ms;

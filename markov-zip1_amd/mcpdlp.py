@@ -79,6 +79,7 @@ class Result(C.Structure):
         ("h2d_ms", C.c_double), ("kernel_ms", C.c_double), ("d2h_ms", C.c_double), ("total_ms", C.c_double),
         ("message", C.c_char * 128), ("backend_actually_used", C.c_char * 32),
         ("fp32_iterations", C.c_uint64), ("setup_ms", C.c_double),
+        ("primal_weight", C.c_double), ("step_size", C.c_double),
     ]
 
 
@@ -179,6 +180,7 @@ def host_lib():
         lib.mc_lp_planted.argtypes = [C.c_void_p, C.POINTER(P_f64), C.POINTER(P_f64), P_f64]
         lib.mc_lp_free.argtypes = [C.c_void_p]
         lib.mc_lp_perturb.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(C.c_void_p)]
+        lib.mc_lp_read_mps.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
         _host = lib
     return _host
 
@@ -306,7 +308,7 @@ class LPData:
                       kb(var_lower, True), kb(var_upper, False), sense, offset)
 
 
-def _lp_from_handle(h: C.c_void_p) -> LPData:
+def _lp_from_handle(h: C.c_void_p, planted: bool = True) -> LPData:
     lib = host_lib()
     v = LP()
     _check_host(lib.mc_lp_view(h, C.byref(v)))
@@ -322,11 +324,25 @@ def _lp_from_handle(h: C.c_void_p) -> LPData:
             return np.zeros(0, BOUND_DTYPE)
         raw = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(16 * k,)).copy()
         return raw.view(BOUND_DTYPE)
-    xs, ys, po = P_f64(), P_f64(), C.c_double()
-    _check_host(lib.mc_lp_planted(h, C.byref(xs), C.byref(ys), C.byref(po)))
+    x_star = y_star = planted_objective = None
+    if planted:
+        xs, ys, po = P_f64(), P_f64(), C.c_double()
+        _check_host(lib.mc_lp_planted(h, C.byref(xs), C.byref(ys), C.byref(po)))
+        x_star, y_star, planted_objective = arr(xs, n), arr(ys, m), float(po.value)
     return LPData(m, n, cs, arr(v.row_index, nnz, np.int64), arr(v.value, nnz), arr(v.objective, n),
                   barr(v.row_lower, m), barr(v.row_upper, m), barr(v.var_lower, n), barr(v.var_upper, n),
-                  int(v.sense), float(v.objective_offset), arr(xs, n), arr(ys, m), float(po.value))
+                  int(v.sense), float(v.objective_offset), x_star, y_star, planted_objective)
+
+
+def read_mps(path: str) -> LPData:
+    """LP read from an MPS file by the host library's reader (mc_lp_read_mps)."""
+    lib = host_lib()
+    h = C.c_void_p()
+    _check_host(lib.mc_lp_read_mps(os.fsencode(path), C.byref(h)))
+    try:
+        return _lp_from_handle(h, planted=False)
+    finally:
+        lib.mc_lp_free(h)
 
 
 def generate(family: int = FAMILY_UNIFORM_EQ, **spec) -> LPData:
@@ -383,10 +399,16 @@ def options(**kw) -> Options:
     tol = kw.pop("tolerance", None)
     if tol is not None:
         o.primal_tolerance = o.dual_tolerance = o.gap_tolerance = tol
+    keep = []
     for k, v in kw.items():
         if not hasattr(o, k):
             raise AttributeError(k)
+        if k in ("warm_primal", "warm_dual") and v is not None and not isinstance(v, C._Pointer):
+            arr = np.ascontiguousarray(v, dtype=np.float64)
+            keep.append(arr)  # the options borrow the array; keep it alive with them
+            v = _ptr(arr, C.c_double)
         setattr(o, k, v)
+    o._keep = keep
     return o
 
 
@@ -399,6 +421,7 @@ def _result_dict(r: Result, prim, dual) -> dict:
         "h2d_ms": r.h2d_ms, "kernel_ms": r.kernel_ms, "d2h_ms": r.d2h_ms, "total_ms": r.total_ms,
         "message": r.message.decode(), "backend_actually_used": r.backend_actually_used.decode(),
         "fp32_iterations": int(r.fp32_iterations), "setup_ms": r.setup_ms,
+        "primal_weight": r.primal_weight, "step_size": r.step_size,
     }
 
 

@@ -7,15 +7,19 @@ x = proj(0), y = 0; src/lp/first_order/pdlp.cpp:186-191).  Pipeline:
    LP, d = [(b - b0) / (1 + |b0|), (c - c0) / (1 + |c0|), (u - u0) / (1 + |u0|)],
    projected on the top principal directions of the training perturbations
    (256 "summary features" of b, c and the bounds);
-2. targets: the GPU PDLP solutions of the training instances, z = [x / sx; y / sy],
-   projected on their top principal directions (64 coefficients of a reduced
-   solution basis);
+2. targets: the GPU PDLP solutions of the training instances, z = [x / u; y / sy]
+   (the primal as a fraction of each variable's upper bound, so a variable at
+   a bound maps to exactly 0 or 1 whatever the instance's bound; the dual over
+   one global scale), projected on their top principal directions (64
+   coefficients of a reduced solution basis);
 3. a BF16 MLP on MFMA (mcpdlp.MLP: hand-written forward/backward/Adam kernels)
    maps standardized features to standardized coefficients;
 4. predict(): reconstruct z from the predicted coefficients, hand (x0, y0) to
-   solve_pdlp as warm_primal / warm_dual.  Cold start remains the fallback and
-   the KKT score alone decides convergence, so a bad prediction costs
-   iterations, never correctness.
+   solve_pdlp as warm_primal / warm_dual, together with the primal weight the
+   training solves ended at (median), since a start near the optimum with a
+   mismatched weight first drifts away from it.  Cold start remains the
+   fallback and the KKT score alone decides convergence, so a bad prediction
+   costs iterations, never correctness.
 """
 from __future__ import annotations
 
@@ -58,15 +62,22 @@ class WarmStart:
             basis = np.pad(basis, ((0, 0), (0, k - basis.shape[1])))
         return mean, basis
 
-    def fit(self, lps, xs, ys, epochs: int = 300, batch: int = 64, lr: float = 1e-3) -> dict:
+    @staticmethod
+    def _upper(lp: mp.LPData) -> np.ndarray:
+        u = lp.var_upper["value"].astype(np.float64)
+        return np.where((lp.var_upper["kind"] == mp.FINITE) & (u > 0), u, 1.0)
+
+    def fit(self, lps, xs, ys, omegas=None, epochs: int = 300, batch: int = 64, lr: float = 1e-3) -> dict:
         D = np.stack([self.perturbation(lp) for lp in lps])
         self.d_mean, self.d_basis = self._pca(D, self.n_features)
         F = (D - self.d_mean) @ self.d_basis
         self.f_mean, self.f_std = F.mean(axis=0), F.std(axis=0) + 1e-12
         Fs = ((F - self.f_mean) / self.f_std).astype(np.float32)
-        X, Y = np.stack(xs), np.stack(ys)
-        self.sx, self.sy = float(np.sqrt(np.mean(X * X))) + 1e-12, float(np.sqrt(np.mean(Y * Y))) + 1e-12
-        Z = np.concatenate([X / self.sx, Y / self.sy], axis=1)
+        X = np.stack([x / self._upper(lp) for x, lp in zip(xs, lps)])
+        Y = np.stack(ys)
+        self.sy = float(np.sqrt(np.mean(Y * Y))) + 1e-12
+        self.omega = float(np.median(omegas)) if omegas is not None and len(omegas) else 0.0
+        Z = np.concatenate([X, Y / self.sy], axis=1)
         self.z_mean, self.z_basis = self._pca(Z, self.n_coeffs)
         Cf = (Z - self.z_mean) @ self.z_basis
         self.c_mean, self.c_std = Cf.mean(axis=0), Cf.std(axis=0) + 1e-12
@@ -76,17 +87,17 @@ class WarmStart:
                           capacity=max(32, batch), seed=self.seed, device=self.device)
         loss, ms = self.net.train(Fs, Cs, batch=batch, epochs=epochs, lr=lr, seed=self.seed + 1)
         return {"train_loss": loss, "train_ms": ms, "pca_explained": float(explained),
-                "samples": len(lps), "params": self.net.nparam}
+                "samples": len(lps), "params": self.net.nparam, "primal_weight": self.omega}
 
     def predict(self, lps):
+        """[(x0, y0)] per instance; pass self.omega as initial_primal_weight."""
         C = self.net.forward(self.features(lps)).astype(np.float64) * self.c_std + self.c_mean
         Z = self.z_mean + C @ self.z_basis.T
         n = self.base.cols
-        xs = Z[:, :n] * self.sx
-        ys = Z[:, n:] * self.sy
-        lo, hi = self.base.var_lower["value"], None
         out = []
-        for lp, x, y in zip(lps, xs, ys):
-            hi = lp.var_upper["value"]
-            out.append((np.clip(x, lo, hi), y))
+        for lp, z in zip(lps, Z):
+            x = z[:n] * self._upper(lp)
+            lo = np.where(lp.var_lower["kind"] == mp.FINITE, lp.var_lower["value"], -np.inf)
+            hi = np.where(lp.var_upper["kind"] == mp.FINITE, lp.var_upper["value"], np.inf)
+            out.append((np.clip(x, lo, hi), z[n:] * self.sy))
         return out

@@ -490,6 +490,8 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
     res->dual_infeasibility = cur.dual;
     res->duality_gap = cur.gap;
     res->iterations = iter;
+    res->primal_weight = omega;
+    res->step_size = eta;
     res->primal_len = n;
     res->dual_len = m;
     if (res->primal) memcpy(res->primal, xo, sizeof(double) * (size_t)n);

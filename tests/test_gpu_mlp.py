@@ -3,10 +3,13 @@
 * the BF16 MFMA GEMM against an exact product (integer-valued operands, so
   BF16 rounding and FP32 accumulation are exact) for all four transposes --
   this pins the MFMA lane layout;
-* the device backward pass against an FP64 CPU restatement (tests/mlp_ref.py)
-  with the same BF16-rounded weights and inputs: loss within 1e-2 and every
-  gradient tensor within 3e-2 relative L2 (the device also rounds its
-  intermediate activations and gradients to BF16);
+* the device backward pass against an FP64 CPU restatement (tests/mlp_ref.py,
+  itself pinned by finite differences in test_mlp_ref.py) with the same
+  BF16-rounded weights and inputs: against the restatement that also rounds
+  the BF16-stored intermediates, loss within 1e-3 and every gradient tensor
+  within 5e-3 relative L2 (FP32 accumulation is the only difference left);
+  against the exact-intermediate restatement, within 2e-2 / 8e-2 (the size
+  of BF16 storage error through three layers);
 * training on the device lowers the loss of a fixed teacher mapping 5x.
 """
 import numpy as np
@@ -35,11 +38,15 @@ def test_mlp_gradients_match_fp64_reference():
     t = rng.standard_normal((64, 64)).astype(np.float32)
     loss, g = net.loss_grad(x, t)
     layers = net.unpack(net.params())
-    ref_loss, ref = mlp_ref.forward_backward(layers, x.astype(np.float64), t.astype(np.float64))
-    assert abs(loss - ref_loss) <= 1e-2 * ref_loss
-    for (dW, db), (rW, rb) in zip(net.unpack(g), ref):
-        assert np.linalg.norm(dW - rW) <= 3e-2 * np.linalg.norm(rW)
-        assert np.linalg.norm(db - rb) <= 3e-2 * np.linalg.norm(rb) + 1e-7
+    x64, t64 = x.astype(np.float64), t.astype(np.float64)
+    for emulate, tol_loss, tol_grad in ((True, 1e-3, 5e-3), (False, 2e-2, 8e-2)):
+        ref_loss, ref = mlp_ref.forward_backward(layers, x64, t64, emulate_bf16=emulate)
+        assert abs(loss - ref_loss) <= tol_loss * ref_loss, (emulate, loss, ref_loss)
+        for k, ((dW, db), (rW, rb)) in enumerate(zip(net.unpack(g), ref)):
+            ew = np.linalg.norm(dW - rW) / np.linalg.norm(rW)
+            eb = np.linalg.norm(db - rb) / (np.linalg.norm(rb) + 1e-12)
+            print(f"layer {k} emulate={emulate}: rel err dW {ew:.2e} db {eb:.2e}")
+            assert ew <= tol_grad and eb <= tol_grad, (emulate, k, ew, eb)
     net.close()
 
 

@@ -210,3 +210,38 @@ def test_kkt_scheme_converges_to_1e6(gpu, precision):
     assert abs(got["objective"] - lp.planted_objective) <= 1e-5 * scale
     if precision == mp.PRECISION_MIXED:
         assert got["fp32_iterations"] > 0
+
+
+# scripts/run_netlib.py:20-93 optima of the Netlib instances in tests/golden/netlib
+NETLIB = {"afiro": -464.753142857143, "adlittle": 225494.96316238, "recipe": -266.616}
+
+
+def netlib(name):
+    import os
+    return mp.read_mps(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "netlib", f"{name}.mps"))
+
+
+@pytest.mark.parametrize("name,scheme", [("afiro", 0), ("recipe", 0), ("adlittle", 1), ("recipe", 1)])
+def test_netlib_reference_order_bit_exact(gpu, name, scheme):
+    """Real LPs (free rows, ranges, bounds of every kind): the GPU reference
+    order reproduces the oracle's whole solve bit for bit."""
+    lp = netlib(name)
+    print("netlib case", name, scheme, flush=True)
+    kw = dict(tolerance=1e-6, restart_scheme=scheme, max_iterations=100000)
+    want = orc.solve(lp, **kw)
+    got = gpu_solve(lp, reduction_order=mp.ORDER_REFERENCE, **kw)
+    assert got["status"] == want["status"] == "optimal" and got["iterations"] == want["iterations"]
+    assert got["objective"] == want["objective"]
+    assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
+
+
+@pytest.mark.parametrize("name", sorted(NETLIB))
+@pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED])
+def test_netlib_fast_order_reaches_published_optimum(gpu, name, precision):
+    lp = netlib(name)
+    print("netlib fast", name, precision, flush=True)
+    r = gpu_solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, precision=precision, max_iterations=200000)
+    assert r["status"] == "optimal"
+    assert abs(r["objective"] - NETLIB[name]) <= 1e-6 * abs(NETLIB[name])
+    k = orc.kkt(lp, r["primal"], r["dual"])
+    assert k["score"] <= 1e-6 * 1.01
