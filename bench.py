@@ -11,12 +11,20 @@ updates, FP64 checks, FP64 polish) and the KKT restart scheme.
 
 One "step" is one full solve: host packing + H2D + device Ruiz + PDHG
 iterations + D2H (the reference's total_ms also starts before Ruiz,
-pdlp.cpp:141).  The JSON line also carries: the plain FP64 GPU solve of the
-same LP, the returned pair re-scored in FP64 on the host, the objective error
-against the planted optimum, the roofline of the dominant fused kernel
-(device-event timing on the engine's stream) and the CPU baseline (the
-compiled reference solve_pdlp, or the oracle, on one core: ms per iteration
-on this LP times the GPU's iteration count).
+pdlp.cpp:141).  The JSON line also carries: the returned pair re-scored in
+FP64 on the host, the objective error against the planted optimum, the
+roofline of the dominant fused kernel (device-event timing on the engine's
+stream, on the solve's layout; HBM traffic from the committed rocprofv3
+counter summary when present) and the CPU baseline (the compiled reference
+solve_pdlp, or the oracle, on one core: ms per iteration on this LP times the
+GPU's iteration count).  --plain adds the plain FP64 GPU solve of the same LP
+to the same tolerance (several more minutes on this LP).
+
+Other workloads: --workload batch (256 uneven LPs in one batched solve vs the
+plain FP64 GPU solve per instance), --workload warm (learned warm start vs
+cold start on a perturbed LP family), --workload cpu-lp (the reference's
+CPU-runnable 2000 x 5000 LP).  --profile-kernels KINDS only times fused
+kernels, for rocprofv3 runs.
 """
 import argparse
 import json
@@ -50,13 +58,14 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
     ap.add_argument("--tolerance", type=float, default=None)
     ap.add_argument("--max-iterations", type=int, default=2_000_000)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--plain", action="store_true",
+                    help="also time the plain FP64 GPU solve of the same LP to the same tolerance")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="approximate CPU sample budget")
     ap.add_argument("--graph-chunk", type=int, default=0, help="iterations per HIP graph (<0: eager launches)")
@@ -65,6 +74,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16, help="batch: instances solved one by one (FP64)")
     ap.add_argument("--train", type=int, default=384, help="warm: training instances")
     ap.add_argument("--test", type=int, default=64, help="warm: test instances")
+    ap.add_argument("--profile-kernels", default=None, metavar="KINDS",
+                    help="only time the listed fused kernels (comma-separated Engine.KERNELS names) on the "
+                         "workload's layout and exit -- the short program rocprofv3 runs wrap")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the batched and "
@@ -76,6 +88,8 @@ def main():
 
     if mp.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.profile_kernels:
+        return profile_kernels(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "warm":
@@ -119,7 +133,7 @@ def main():
         f"objective rel err vs planted {obj_err:.2e}")
 
     plain = None
-    if not args.no_plain:
+    if args.plain:
         po = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_FP64,
                         restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations,
                         graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
@@ -128,21 +142,14 @@ def main():
                  "kernel_ms": round(pr["kernel_ms"], 3)}
         log(f"[bench] plain FP64: {pr['status']} it={pr['iterations']} wall={pwall:.1f}ms")
 
-    # roofline of the fused iteration kernels (device events on the engine stream)
-    eng = mp.Engine(0)
-    eng.load(lp)
-    eng.setup(autotune=not args.no_autotune)
-    kern = {}
-    for name in ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64", "spmv_fp32", "spmv_t_fp32"):
-        us, b = eng.time_kernel(name, reps=50)
-        kern[name] = {"us": round(us, 3), "GBps": round(b / us / 1e3, 1), "bytes": int(b)}
-        log(f"[bench] kernel {name}: {us:.2f} us/launch, {b / 1e6:.1f} MB -> {b / us / 1e3:.0f} GB/s")
-    (ta, tt), = [eng.tiling()]
-    eng.close()
+    # roofline of the fused iteration kernels (device events on the engine stream),
+    # on the layout the solve uses (rows/columns relabelled by length bucket)
+    kern, (ta, tt) = time_kernels(mp, lp, ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64",
+                                           "spmv_fp32", "spmv_t_fp32", "copy"), not args.no_autotune)
     dom = max(("primal_fp32", "dual_fp32"), key=lambda k: kern[k]["us"])
     ach = kern[dom]["GBps"]
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, dom)}
 
     cpu = None
     if not args.no_cpu:
@@ -185,6 +192,57 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+KERNEL_SYMBOL = {"primal_fp32": "k_primal", "dual_fp32": "k_dual", "primal_fp64": "k_primal",
+                 "dual_fp64": "k_dual", "spmv_fp32": "k_spmv_plain", "spmv_t_fp32": "k_spmv_plain", "copy": "k_copy"}
+
+
+def time_kernels(mp, lp, kinds, autotune, reps=50):
+    """{kind: us/bytes/GBps} from HIP events on the engine's stream, and the tiling."""
+    eng = mp.Engine(0)
+    eng.load(mp.relabel_by_length(lp))
+    eng.setup(autotune=autotune)
+    kern = {}
+    for name in kinds:
+        us, b = eng.time_kernel(name, reps=reps)
+        kern[name] = {"us": round(us, 3), "GBps": round(b / us / 1e3, 1), "bytes": int(b)}
+        log(f"[bench] kernel {name} ({KERNEL_SYMBOL[name]}): {us:.2f} us/launch, {b / 1e6:.2f} MB algorithmic "
+            f"-> {b / us / 1e3:.0f} GB/s")
+    tiling = eng.tiling()
+    eng.close()
+    return kern, tiling
+
+
+def measured_traffic(workload, kind):
+    """HBM bytes per launch of `kind` on `workload` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/traffic_<workload>.txt: lines
+    'traffic <kind> <bytes>'), or None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kind:
+                return float(parts[2])
+    return None
+
+
+def profile_kernels(args):
+    """Time only the listed kernels on the workload's LP (for rocprofv3 runs)."""
+    import mcpdlp as mp
+    wl = WORKLOADS[args.workload]
+    if "family" not in wl:
+        raise SystemExit("--profile-kernels takes a single-LP workload (large or cpu-lp)")
+    lp = mp.generate(wl["family"], **wl["spec"])
+    kinds = [k for k in args.profile_kernels.split(",") if k]
+    for k in kinds:
+        if k not in mp.Engine.KERNELS:
+            raise SystemExit(f"unknown kernel kind {k!r}; choose from {sorted(mp.Engine.KERNELS)}")
+    kern, tiling = time_kernels(mp, lp, kinds, not args.no_autotune)
+    print(json.dumps({"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "kernels": kern,
+                      "tiles": {"rows": list(tiling[0]), "cols": list(tiling[1])}}), flush=True)
 
 
 def _line(value_ms, workload, cfg, extra, steps=1, warmup=0):

@@ -124,6 +124,8 @@ typedef struct mc_pdlp_options {
     int32_t restart_scheme;           /* MC_SCHEME_REFERENCE (default) or MC_SCHEME_KKT */
     const double* warm_primal;        /* optional [n] start point in the ORIGINAL space (NULL => cold x=proj(0)) */
     const double* warm_dual;          /* optional [m] start dual in the ORIGINAL space (reported sign convention) */
+    int32_t fp32_stall_checks;        /* mixed: checks without a new best score before the FP64 switch (0 => 50) */
+    int32_t pad_;
 } mc_pdlp_options;
 
 /* PdlpStatus (reference: pdlp.hpp:43). */

@@ -96,7 +96,7 @@ typedef struct mcp_solve_opts {
     int32_t reduction_order;    /* MC_ORDER_* */
     int32_t graph_chunk;        /* iterations per HIP graph; 0 => default; <0 => no graphs */
     int32_t restart_scheme;     /* MC_SCHEME_* */
-    int32_t pad_;
+    int32_t fp32_stall_checks;  /* mixed: checks without a new best score before the FP64 switch; 0 => 50 */
     const double* warm_primal;  /* optional [n] host, original space */
     const double* warm_dual;    /* optional [m] host, original space (reported convention) */
 } mcp_solve_opts;
@@ -142,7 +142,9 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
 /* Micro-benchmark of one fused iteration kernel on the loaded, set-up problem,
  * launched `reps` times on the engine's stream between two HIP events.
  * kind: 0 primal (A^T y + primal update) FP32, 1 dual (A x_bar + dual update)
- * FP32, 2 primal FP64, 3 dual FP64, 4 plain SpMV A x FP32, 5 plain A^T y FP32.
+ * FP32, 2 primal FP64, 3 dual FP64, 4 plain SpMV A x FP32, 5 plain A^T y FP32,
+ * 6 plain streaming copy of the FP32 values of A (the calibration kernel for
+ * HBM byte counters: it moves exactly 8 bytes per copied value).
  * *avg_us = mean device time per launch; *bytes = the kernel's algorithmic
  * HBM bytes per launch (matrix and pointer streams once, gathered vector once,
  * per-row/column operands once).  Leaves the iterate in an unspecified state;

@@ -72,7 +72,7 @@ struct IterOpts {
     int32_t ruiz;
     int32_t mixed;
     int32_t kkt;                 // restart scheme: 0 reference, 1 KKT (Applegate et al.)
-    int32_t pad_;
+    int32_t stall_checks;        // mixed: checks without a new best before the FP64 switch
 };
 
 // Map (workgroup, wave) to a tile so that each XCD works on a contiguous

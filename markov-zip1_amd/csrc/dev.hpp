@@ -94,6 +94,9 @@ void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm
 void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, bool mixed,
                         hipStream_t s);
 void launch_fill(double* p, double v, int64_t n, hipStream_t s);
+// Plain streaming copy of n floats (n a multiple of 4, 16-byte aligned
+// buffers): the reference kernel that calibrates the HBM byte counters.
+void launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
 void launch_unscale(const Dev& d, double* x_out, double* y_out, hipStream_t s);
 // Plain SpMV over a tile set: out[r] = sum val * vec (tests and micro-benchmarks);
 // tiles with one lane per row sum in the reference's serial order.

@@ -267,6 +267,7 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.ruiz = 1;
     io.mixed = o->precision == MC_PRECISION_MIXED;
     io.kkt = o->restart_scheme == MC_SCHEME_KKT ? 1 : 0;
+    io.stall_checks = o->fp32_stall_checks > 0 ? o->fp32_stall_checks : 50;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
     // 0 => no score threshold: FP32 runs until it converges or stalls
     io.polish_threshold = o->polish_threshold > 0.0 ? o->polish_threshold : 0.0;
@@ -753,7 +754,7 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
 int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes) {
     return guarded([&] {
         require(e && e->setup_done, "mcp_time_kernel before mcp_setup");
-        require(kind >= 0 && kind <= 5, "unknown kernel kind");
+        require(kind >= 0 && kind <= 6, "unknown kernel kind");
         HIPCHK(hipSetDevice(e->device));
         auto& d = e->d;
         hipStream_t s = e->stream;
@@ -771,8 +772,11 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
             b = (4.0 + w) * nnz + 4.0 * (m + 1) + w * n + (f32 ? 16.0 : 24.0) * m + 2 * w * m + 2 * w * m;
         else if (kind == 4)
             b = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 4.0 * m;
-        else
+        else if (kind == 5)
             b = 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 4.0 * n;
+        const int64_t ncopy = e->nnz / 4 * 4;  // kind 6: plain copy of the FP32 values of A
+        float* copy_dst = kind == 6 ? e->buf<float>("copy_dst", ncopy) : nullptr;
+        if (kind == 6) b = 8.0 * (double)ncopy;
         // every instance running in the kernel's precision, average count 1
         std::vector<mcp::InstState> st(e->B);
         HIPCHK(hipMemcpy(st.data(), d.st, sizeof(mcp::InstState) * e->B, hipMemcpyDeviceToHost));
@@ -793,7 +797,8 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 case 1:
                 case 3: mcp::launch_dual(d, 1, false, s); break;
                 case 4: mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.x32, d.y32, true, s); break;
-                default: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.x32, true, s); break;
+                case 5: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.x32, true, s); break;
+                default: mcp::launch_copy(d.av32, copy_dst, ncopy, s); break;
             }
         };
         launch();

@@ -24,6 +24,8 @@
 // and sum is rounded once, as on the host.
 #include <math.h>
 
+#include <algorithm>
+
 #include "dev.hpp"
 
 namespace mcp {
@@ -633,15 +635,15 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
     }
     if (s.mode == kModeFp32) {
         // FP32 iterations stop paying once the score is near tolerance or has
-        // not set a new best for 50 checks (the FP32 rounding floor); continue
-        // from this iterate in FP64.
+        // not set a new best for stall_checks checks (the FP32 rounding
+        // floor); continue from this iterate in FP64.
         if (score < s.best_score) {
             s.best_score = score;
             s.stall = 0;
         } else {
             s.stall += 1;
         }
-        if (score <= o.polish_threshold || s.stall >= 50) {
+        if (score <= o.polish_threshold || s.stall >= o.stall_checks) {
             s.switch_flag = 1;
             s.mode = kModeFp64;
             s.fp32_iters = iter_after;
@@ -1097,6 +1099,20 @@ void launch_unscale(const Dev& d, double* x_out, double* y_out, hipStream_t s) {
     const int64_t tot = (int64_t)d.m + d.n;
     if (tot == 0) return;
     hipLaunchKernelGGL(k_unscale, dim3(elem_grid(tot)), dim3(kBlock), 0, s, d, x_out, y_out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                  int64_t n4) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n4; k += stride) dst[k] = src[k];
+}
+
+void launch_copy(const float* src, float* dst, int64_t n, hipStream_t s) {
+    const int64_t n4 = n / 4;
+    if (n4 <= 0) return;
+    const int grid = (int)std::min<int64_t>(elem_grid(n4), 256 * 16);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, s, reinterpret_cast<const float4*>(src),
+                       reinterpret_cast<float4*>(dst), n4);
 }
 
 void launch_fill(double* p, double v, int64_t n, hipStream_t s) {

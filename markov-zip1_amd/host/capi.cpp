@@ -60,6 +60,7 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
                                                                 : fo::ReductionOrder::fast;
     p.restart_scheme = o.restart_scheme == MC_SCHEME_KKT ? fo::RestartScheme::kkt : fo::RestartScheme::reference;
     p.polish_threshold = o.polish_threshold;
+    p.fp32_stall_checks = o.fp32_stall_checks;
     p.graph_chunk = o.graph_chunk;
     p.device = o.device;
     p.require_gpu = o.require_gpu != 0;

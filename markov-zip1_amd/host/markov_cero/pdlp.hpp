@@ -49,6 +49,7 @@ struct PdlpOptions {
     Precision precision{Precision::fp64};
     ReductionOrder reduction_order{ReductionOrder::fast};
     double polish_threshold{0.0};   // mixed: FP64 from this checked score (0 => only when FP32 stalls)
+    int fp32_stall_checks{0};       // mixed: checks without a new best before FP64 (0 => 50)
     int graph_chunk{0};             // iterations per captured HIP graph (0 => 64, <0 => eager launches)
     int device{0};
     bool require_gpu{false};        // true: fail instead of the cpu_fallback path

@@ -334,6 +334,7 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.initial_primal_weight = opt.initial_primal_weight;
     o.primal_weight_smoothing = opt.primal_weight_smoothing;
     o.polish_threshold = opt.polish_threshold;
+    o.fp32_stall_checks = opt.fp32_stall_checks;
     o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
     o.graph_chunk = opt.graph_chunk;
     o.restart_scheme = opt.restart_scheme == RestartScheme::kkt ? MC_SCHEME_KKT : MC_SCHEME_REFERENCE;

@@ -66,6 +66,7 @@ class Options(C.Structure):
         ("polish_threshold", C.c_double),
         ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("restart_scheme", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
+        ("fp32_stall_checks", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -129,7 +130,7 @@ class EngSolve(C.Structure):
         ("initial_primal_weight", C.c_double), ("primal_weight_smoothing", C.c_double),
         ("polish_threshold", C.c_double),
         ("reduction_order", C.c_int32), ("graph_chunk", C.c_int32),
-        ("restart_scheme", C.c_int32), ("pad_", C.c_int32),
+        ("restart_scheme", C.c_int32), ("fp32_stall_checks", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
     ]
 
@@ -467,6 +468,41 @@ def verify(lp: LPData, x, objective: float, tol: float = 1e-4) -> dict:
 
 
 # ---- engine (device-level) access ------------------------------------------
+def _length_bucket(lens: np.ndarray) -> np.ndarray:
+    """Exact below 32 nonzeros, log2 classes above (host/pdlp_gpu.cpp length_bucket)."""
+    lens = np.asarray(lens, np.int64)
+    big = lens >= 32
+    out = lens.copy()
+    out[big] = 32 + np.floor(np.log2(lens[big] / 32.0)).astype(np.int64)
+    return out
+
+
+def relabel_by_length(lp: LPData) -> LPData:
+    """The same LP with rows and columns stably sorted by descending length
+    bucket -- the relabelling the host driver applies before mcp_load in the
+    fast reduction order, so Engine users can time kernels on the product's
+    layout.  Returns an LP without a planted pair."""
+    clen = np.diff(lp.column_start)
+    rlen = np.bincount(lp.row_index, minlength=lp.rows)
+    cperm = np.argsort(-_length_bucket(clen), kind="stable")  # new -> old
+    rperm = np.argsort(-_length_bucket(rlen), kind="stable")
+    rinv = np.empty(lp.rows, np.int64)
+    rinv[rperm] = np.arange(lp.rows)
+    cs = np.zeros(lp.cols + 1, np.int64)
+    cs[1:] = np.cumsum(clen[cperm])
+    ri = np.empty(lp.nnz, np.int64)
+    val = np.empty(lp.nnz)
+    col_of = np.repeat(np.arange(lp.cols), clen)          # old column of each entry
+    newc = np.empty(lp.cols, np.int64)
+    newc[cperm] = np.arange(lp.cols)
+    order = np.lexsort((rinv[lp.row_index], newc[col_of]))
+    ri[:] = rinv[lp.row_index[order]]
+    val[:] = lp.value[order]
+    return LPData(lp.rows, lp.cols, cs, ri, val, lp.objective[cperm].copy(), lp.row_lower[rperm].copy(),
+                  lp.row_upper[rperm].copy(), lp.var_lower[cperm].copy(), lp.var_upper[cperm].copy(),
+                  lp.sense, lp.offset)
+
+
 def _csc_to_csr(m, n, cs, ri, val):
     cols = np.repeat(np.arange(n, dtype=np.int64), np.diff(cs))
     order = np.lexsort((cols, ri))  # by row, then column
@@ -576,7 +612,7 @@ class Engine:
         return (a.nnz_budget, a.per_lane, na.value), (t.nnz_budget, t.per_lane, nt.value)
 
     KERNELS = {"primal_fp32": 0, "dual_fp32": 1, "primal_fp64": 2, "dual_fp64": 3,
-               "spmv_fp32": 4, "spmv_t_fp32": 5}
+               "spmv_fp32": 4, "spmv_t_fp32": 5, "copy": 6}
 
     def time_kernel(self, kind: str, reps: int = 20):
         """(mean device us per launch, algorithmic bytes per launch) of one fused kernel."""
