@@ -87,9 +87,18 @@ typedef struct mc_lp {
  *   (<= 0.8 x) without progress since the previous check, or when the epoch reaches
  *   0.36 x all iterations; primal weight from the movement since the last restart,
  *   omega <- exp(theta log(|dy|/|dx|) + (1-theta) log omega), theta = primal_weight_smoothing.
- * Termination, tolerances and the reported score are the reference's in both. */
+ * HALPERN: the KKT scheme's restarts on reflected Halpern iterations (Halpern 1967;
+ *   the reflected PDHG form of Lu & Yang, 2024-25): with T the PDHG step
+ *   (x, y) -> (x+, y+) and z0 the point of the last restart,
+ *   z <- w (2 T(z) - z) + (1 - w) z0, w = (k+1)/(k+2) after k iterations since
+ *   the restart; the check compares T(z) (in place of the average) with z.  The
+ *   primal weight at a restart is the geometric blend
+ *   omega <- omega_move^(3/4) omega_res^(1/4) of the movement rule above and the
+ *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)).
+ * Termination, tolerances and the reported score are the reference's in all three. */
 #define MC_SCHEME_REFERENCE 0
 #define MC_SCHEME_KKT 1
+#define MC_SCHEME_HALPERN 2
 /* Reduction order of the GPU engine (extension). */
 #define MC_ORDER_FAST 0      /* tile-parallel sums (deterministic, not reference order)   */
 #define MC_ORDER_REFERENCE 1 /* every sum in the reference's serial order: bit-exact FP64 */
@@ -121,7 +130,7 @@ typedef struct mc_pdlp_options {
     int32_t graph_chunk;              /* iterations per captured HIP graph (0 => 64; < 0 => eager launches) */
     int32_t device;                   /* HIP device ordinal */
     int32_t require_gpu;              /* 1: a GPU request fails with MC_ERR_DEVICE instead of the cpu_fallback path */
-    int32_t restart_scheme;           /* MC_SCHEME_REFERENCE (default) or MC_SCHEME_KKT */
+    int32_t restart_scheme;           /* MC_SCHEME_REFERENCE (default), MC_SCHEME_KKT or MC_SCHEME_HALPERN */
     const double* warm_primal;        /* optional [n] start point in the ORIGINAL space (NULL => cold x=proj(0)) */
     const double* warm_dual;          /* optional [m] start dual in the ORIGINAL space (reported sign convention) */
     int32_t fp32_stall_checks;        /* mixed: checks without a new best score before the FP64 switch (0 => 50) */

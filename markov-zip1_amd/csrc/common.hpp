@@ -73,6 +73,8 @@ struct IterOpts {
     int32_t mixed;
     int32_t kkt;                 // restart scheme: 0 reference, 1 KKT (Applegate et al.)
     int32_t stall_checks;        // mixed: checks without a new best before the FP64 switch
+    int32_t halpern;             // Halpern scheme: blended primal weight at restarts
+    int32_t pad_;
 };
 
 // Map (workgroup, wave) to a tile so that each XCD works on a contiguous

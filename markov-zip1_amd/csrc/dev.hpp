@@ -73,6 +73,9 @@ struct Dev {
     InstState* st;      // [B]
     int64_t* iter_base; // global iteration counter at the start of the current chunk
     int32_t* counters;  // [0] running instances, [1] running instances in FP32
+    int32_t halpern;    // reflected Halpern iterates: the average slots hold T(z), the
+                        // last-restart slots the anchor z0
+    int32_t pad_;
 };
 
 // ---- launch helpers (kernels.hip) ----

@@ -72,13 +72,14 @@ int lanes_log2(int len, int per_lane) {
 // lanes.  exact => 1 lane per row (the reference's serial order).
 void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& inst, int B,
                  mcp_tile_policy pol, bool exact, std::vector<mcp::Tile>& tiles,
-                 std::vector<int32_t>& begin) {
+                 std::vector<int32_t>& begin, const std::vector<char>* keep = nullptr) {
     const int budget = std::max(1, pol.nnz_budget);
     const int per = std::max(1, pol.per_lane);
     tiles.clear();
     begin.assign(B + 1, 0);
     for (int b = 0; b < B; ++b) {
         begin[b] = (int32_t)tiles.size();
+        if (keep && !(*keep)[b]) continue;  // instance left out: empty tile range
         int r = inst[b];
         const int re = inst[b + 1];
         while (r < re) {
@@ -139,6 +140,7 @@ struct mcp_engine {
     std::vector<int32_t> tA_begin, tT_begin;
     mcp_tile_policy polA{1024, 8}, polT{1024, 8};
     bool loaded = false, setup_done = false, exact_tiles = false;
+    bool compacted = false;  // tiles cover only the instances still running (inside mcp_solve)
     int ruiz_passes_req = -1, ruiz_on_req = -1;
     double h2d_ms = 0.0, setup_ms = 0.0;
     std::map<std::tuple<int64_t, int, int, int, int>, hipGraphExec_t> graphs;
@@ -202,10 +204,12 @@ struct mcp_engine {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
-    void retile(bool exact) {
-        build_tiles(rp, inst_row, B, polA, exact, tA, tA_begin);
-        build_tiles(cp, inst_col, B, polT, exact, tT, tT_begin);
+    // keep == nullptr: every instance; else only the instances with keep[b].
+    void retile(bool exact, const std::vector<char>* keep = nullptr) {
+        build_tiles(rp, inst_row, B, polA, exact, tA, tA_begin, keep);
+        build_tiles(cp, inst_col, B, polT, exact, tT, tT_begin, keep);
         exact_tiles = exact;
+        compacted = keep != nullptr;
         upload_tiles();
     }
 
@@ -266,7 +270,8 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.adaptive_weight = o->adaptive_primal_weight;
     io.ruiz = 1;
     io.mixed = o->precision == MC_PRECISION_MIXED;
-    io.kkt = o->restart_scheme == MC_SCHEME_KKT ? 1 : 0;
+    io.kkt = (o->restart_scheme == MC_SCHEME_KKT || o->restart_scheme == MC_SCHEME_HALPERN) ? 1 : 0;
+    io.halpern = o->restart_scheme == MC_SCHEME_HALPERN ? 1 : 0;
     io.stall_checks = o->fp32_stall_checks > 0 ? o->fp32_stall_checks : 50;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
     // 0 => no score threshold: FP32 runs until it converges or stalls
@@ -521,8 +526,11 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         HIPCHK(hipSetDevice(e->device));
         auto& d = e->d;
         hipStream_t s = e->stream;
+        require(o->restart_scheme >= MC_SCHEME_REFERENCE && o->restart_scheme <= MC_SCHEME_HALPERN,
+                "unknown restart scheme");
         mcp::IterOpts io{};
         make_opts(o, io);
+        d.halpern = io.halpern;
         // start point
         const double* wx = nullptr;
         const double* wy = nullptr;
@@ -567,6 +575,10 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         auto prog_last = prog_start;
         int64_t iter = 0;
         bool finished = false;
+        // Batch compaction: once half of the instances that were running at
+        // the last (re)tiling have finished, rebuild the tiles over the running
+        // instances only, so finished instances stop costing waves.
+        int32_t compact_at = e->B > 1 ? e->B / 2 : -1;
         while (iter < max_it) {
             const int len = (int)std::min<int64_t>(chunk, max_it - iter);
             if (use_graphs && len == chunk) {
@@ -600,6 +612,25 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                     finished = true;
                     break;
                 }
+                if (pin[2 * prev] <= compact_at) {
+                    HIPCHK(hipStreamSynchronize(s));
+                    std::vector<mcp::InstState> stv(e->B);
+                    HIPCHK(hipMemcpy(stv.data(), d.st, sizeof(mcp::InstState) * e->B, hipMemcpyDeviceToHost));
+                    std::vector<char> keep(e->B, 0);
+                    int32_t nk = 0;
+                    for (int b = 0; b < e->B; ++b) {
+                        keep[b] = stv[b].status == mcp::kRunning ? 1 : 0;
+                        nk += keep[b];
+                    }
+                    if (nk == 0) {
+                        finished = true;
+                        break;
+                    }
+                    e->retile(e->exact_tiles, &keep);
+                    e->drop_graphs();
+                    e->graph_dev = d;
+                    compact_at = nk / 2;
+                }
                 if (prog_every > 0.0) {
                     const auto now = std::chrono::steady_clock::now();
                     if (std::chrono::duration<double>(now - prog_last).count() >= prog_every) {
@@ -628,6 +659,11 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         }
         HIPCHK(hipEventRecord(e->ev[1], s));
         HIPCHK(hipStreamSynchronize(s));
+        if (e->compacted) {  // outside mcp_solve the tiles always cover every instance
+            e->retile(e->exact_tiles);
+            e->drop_graphs();
+            e->graph_dev = d;
+        }
         if (stats) {
             stats->kernel_ms = ms_between(e->ev[0], e->ev[1]);
             stats->setup_ms = e->setup_ms;

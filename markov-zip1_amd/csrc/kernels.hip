@@ -57,8 +57,13 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
     const double eta = s.eta, omega = s.omega;
     const double kavg = (double)(iter - s.restart_iter + 1);
     double part = 0.0;
+    // Halpern scheme: x_avg <- T(z)_x = x+, x <- w x_bar + (1 - w) x0 with
+    // x_bar = 2 x+ - x (the reflection), w = k/(k+1), k = kavg.
+    const bool halp = d.halpern != 0;
+    const double w = kavg / (kavg + 1.0);
     if (!EXACT && s.mode == kModeFp32) {
-        const float tauf = (float)(eta / omega), kf = (float)kavg;
+        const float tauf = (float)(eta / omega), kf = (float)kavg, wf = (float)w;
+        const float* xsrc = halp ? d.xlast32 : d.xavg32;
         for (int base = tl.r0; base < tl.r1; base += L.step) {
             const int j = base + (L.lane >> L.lg);
             const bool own = j < tl.r1;
@@ -71,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
                 p1 = d.cp[j + 1];
                 pk = d.colpk[j];  // c, lo, hi, 1/colnorm
                 xo = d.x32[j];
-                xa = d.xavg32[j];
+                xa = xsrc[j];     // running average, or the Halpern anchor
                 if (ADAPT) cn = d.cnorm[j];
             }
             const float aty = row_dot<float>(p0, p1, L.gl, L.G, d.ri, d.tv32, d.y32);
@@ -83,12 +88,19 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
                     d.dx32[j] = dd;
                     part += cn * (double)dd * (double)dd;
                 }
-                d.x32[j] = xp;
-                d.xbar32[j] = 2.0f * xp - xo;
-                d.xavg32[j] = xa + (xp - xa) / kf;
+                const float xb = 2.0f * xp - xo;
+                d.xbar32[j] = xb;
+                if (halp) {
+                    d.xavg32[j] = xp;
+                    d.x32[j] = wf * xb + (1.0f - wf) * xa;
+                } else {
+                    d.x32[j] = xp;
+                    d.xavg32[j] = xa + (xp - xa) / kf;
+                }
             }
         }
     } else {
+        const double* xsrc = halp ? d.xlast : d.xavg;
         for (int base = tl.r0; base < tl.r1; base += L.step) {
             const int j = base + (L.lane >> L.lg);
             const bool own = j < tl.r1;
@@ -102,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
                 lo = d.slo[j];
                 hi = d.shi[j];
                 xo = d.x[j];
-                xa = d.xavg[j];
+                xa = xsrc[j];
             }
             const double aty = row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.y);
             if (own && L.gl == 0) {
@@ -114,9 +126,15 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
                     d.dx[j] = dd;
                     part += cn * dd * dd;
                 }
-                d.x[j] = xp;
-                d.xbar[j] = 2.0 * xp - xo;
-                d.xavg[j] = xa + (xp - xa) / kavg;
+                const double xb = 2.0 * xp - xo;
+                d.xbar[j] = xb;
+                if (halp) {
+                    d.xavg[j] = xp;
+                    d.x[j] = w * xb + (1.0 - w) * xa;
+                } else {
+                    d.x[j] = xp;
+                    d.xavg[j] = xa + (xp - xa) / kavg;
+                }
             }
         }
     }
@@ -139,8 +157,12 @@ __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
     const int64_t iter = *d.iter_base + t;
     const double eta = s.eta, omega = s.omega;
     const double kavg = (double)(iter - s.restart_iter + 1);
+    // Halpern scheme: y_avg <- T(z)_y = y+, y <- w (2 y+ - y) + (1 - w) y0.
+    const bool halp = d.halpern != 0;
+    const double w = kavg / (kavg + 1.0);
     if (!EXACT && s.mode == kModeFp32) {
-        const float sigf = (float)(eta * omega), kf = (float)kavg;
+        const float sigf = (float)(eta * omega), kf = (float)kavg, wf = (float)w;
+        const float* ysrc = halp ? d.ylast32 : d.yavg32;
         for (int base = tl.r0; base < tl.r1; base += L.step) {
             const int i = base + (L.lane >> L.lg);
             const bool own = i < tl.r1;
@@ -152,18 +174,24 @@ __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
                 p1 = d.rp[i + 1];
                 pk = d.rowpk[i];  // blo, bhi, 1/rownorm
                 yo = d.y32[i];
-                ya = d.yavg32[i];
+                ya = ysrc[i];     // running average, or the Halpern anchor
             }
             const float ax = row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.xbar32);
             if (own && L.gl == 0) {
                 const float sigma = sigf * pk.z;
                 const float v = yo + sigma * ax;
                 const float yn = v - sigma * clampv(v / sigma, pk.x, pk.y);
-                d.y32[i] = yn;
-                d.yavg32[i] = ya + (yn - ya) / kf;
+                if (halp) {
+                    d.yavg32[i] = yn;
+                    d.y32[i] = wf * (2.0f * yn - yo) + (1.0f - wf) * ya;
+                } else {
+                    d.y32[i] = yn;
+                    d.yavg32[i] = ya + (yn - ya) / kf;
+                }
             }
         }
     } else {
+        const double* ysrc = halp ? d.ylast : d.yavg;
         for (int base = tl.r0; base < tl.r1; base += L.step) {
             const int i = base + (L.lane >> L.lg);
             const bool own = i < tl.r1;
@@ -176,15 +204,20 @@ __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
                 bl = d.blo[i];
                 bh = d.bhi[i];
                 yo = d.y[i];
-                ya = d.yavg[i];
+                ya = ysrc[i];
             }
             const double ax = row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.xbar);
             if (own && L.gl == 0) {
                 const double sigma = (eta * omega) / rn;
                 const double v = yo + sigma * ax;
                 const double yn = v - sigma * clampv(v / sigma, bl, bh);
-                d.y[i] = yn;
-                d.yavg[i] = ya + (yn - ya) / kavg;
+                if (halp) {
+                    d.yavg[i] = yn;
+                    d.y[i] = w * (2.0 * yn - yo) + (1.0 - w) * ya;
+                } else {
+                    d.y[i] = yn;
+                    d.yavg[i] = ya + (yn - ya) / kavg;
+                }
             }
         }
     }
@@ -620,12 +653,21 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
         } else {
             if (o.adaptive_weight) {
                 const double dX = sqrt(r.xmove2), dY = sqrt(r.ymove2);
+                double w = s.omega;
+                bool upd = false;
                 if (dX > 1e-10 && dY > 1e-10) {
                     const double th = o.smoothing, ratio = dY / dX;
-                    const double w = (th == 0.5) ? sqrt(s.omega * ratio)
-                                                 : pow(ratio, th) * pow(s.omega, 1.0 - th);
-                    s.omega = clampv(w, 1e-6, 1e6);
+                    w = (th == 0.5) ? sqrt(s.omega * ratio) : pow(ratio, th) * pow(s.omega, 1.0 - th);
+                    upd = true;
                 }
+                if (o.halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
+                    double wres = s.omega;
+                    if (r.primal > 1e-12 && r.dual > 1e-12)
+                        wres = s.omega * sqrt(clampv(sqrt(r.primal / r.dual), 0.05, 20.0));
+                    w = sqrt(w * sqrt(w * wres));
+                    upd = true;
+                }
+                if (upd) s.omega = clampv(w, 1e-6, 1e6);
             }
             s.prev_cand = 1e300;
         }

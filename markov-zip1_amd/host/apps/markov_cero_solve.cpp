@@ -59,7 +59,8 @@ std::string esc(const std::string& s) {
 int usage() {
     std::cerr << "usage: markov-cero-solve MODEL.mps|MODEL.mclp [--engine pdlp] [--backend cpu|gpu]\n"
                  "       [--tolerance T] [--iteration-limit N] [--output FILE]\n"
-                 "       [--precision fp64|mixed] [--order fast|reference] [--autotune]\n";
+                 "       [--precision fp64|mixed] [--order fast|reference] [--autotune]\n"
+                 "       [--restart-scheme reference|kkt|halpern]\n";
     return 4;
 }
 
@@ -67,7 +68,8 @@ int usage() {
 
 int main(int argc, char** argv) {
     const auto started = std::chrono::steady_clock::now();
-    std::string path, engine = "pdlp", backend = "cpu", output, precision = "fp64", order = "fast";
+    std::string path, engine = "pdlp", backend = "cpu", output, precision = "fp64", order = "fast",
+                scheme = "reference";
     double tol = 1e-4;
     std::size_t iteration_limit = 10000;
     bool autotune = false;
@@ -88,6 +90,7 @@ int main(int argc, char** argv) {
         else if (a == "--precision") precision = next();
         else if (a == "--order") order = next();
         else if (a == "--autotune") autotune = true;
+        else if (a == "--restart-scheme") scheme = next();
         else if (!a.empty() && a[0] == '-') return usage();
         else path = a;
     }
@@ -98,6 +101,10 @@ int main(int argc, char** argv) {
     }
     if (backend != "cpu" && backend != "gpu") {
         std::cerr << "invalid backend: " << backend << "\n";
+        return 4;
+    }
+    if (scheme != "reference" && scheme != "kkt" && scheme != "halpern") {
+        std::cerr << "invalid restart scheme: " << scheme << "\n";
         return 4;
     }
     if (!(tol > 0.0)) {
@@ -123,6 +130,9 @@ int main(int argc, char** argv) {
         o.precision = precision == "mixed" ? fo::Precision::mixed : fo::Precision::fp64;
         o.reduction_order = order == "reference" ? fo::ReductionOrder::reference : fo::ReductionOrder::fast;
         o.autotune = autotune;
+        o.restart_scheme = scheme == "kkt"       ? fo::RestartScheme::kkt
+                           : scheme == "halpern" ? fo::RestartScheme::halpern
+                                                 : fo::RestartScheme::reference;
         res = fo::solve_pdlp(md, o);
         message = res.message;
         if (res.status == fo::PdlpStatus::optimal) {

@@ -58,7 +58,9 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
     p.precision = o.precision == MC_PRECISION_MIXED ? fo::Precision::mixed : fo::Precision::fp64;
     p.reduction_order = o.reduction_order == MC_ORDER_REFERENCE ? fo::ReductionOrder::reference
                                                                 : fo::ReductionOrder::fast;
-    p.restart_scheme = o.restart_scheme == MC_SCHEME_KKT ? fo::RestartScheme::kkt : fo::RestartScheme::reference;
+    p.restart_scheme = o.restart_scheme == MC_SCHEME_KKT       ? fo::RestartScheme::kkt
+                       : o.restart_scheme == MC_SCHEME_HALPERN ? fo::RestartScheme::halpern
+                                                               : fo::RestartScheme::reference;
     p.polish_threshold = o.polish_threshold;
     p.fp32_stall_checks = o.fp32_stall_checks;
     p.graph_chunk = o.graph_chunk;

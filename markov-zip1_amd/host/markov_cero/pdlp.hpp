@@ -24,7 +24,7 @@ enum class ReductionOrder { fast, reference };      // extension
 // sufficient (0.2x) / necessary-without-progress (0.8x) decay or an epoch of
 // 0.36 x all iterations, primal weight from the movement since the last
 // restart (omega <- (|dy|/|dx|)^theta omega^(1-theta), theta = smoothing).
-enum class RestartScheme { reference, kkt };
+enum class RestartScheme { reference, kkt, halpern };
 
 struct PdlpOptions {
     std::size_t max_iterations{100000};

@@ -179,7 +179,9 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     p.ax.assign(m, 0.0);
     std::vector<double> xprev(n), xu, yu;
 
-    const bool kkt = opt.restart_scheme == RestartScheme::kkt;
+    const bool halpern = opt.restart_scheme == RestartScheme::halpern;
+    const bool kkt = opt.restart_scheme == RestartScheme::kkt || halpern;
+    std::vector<double> yprev(halpern ? m : 0);
     std::vector<double> xlast = p.x, ylast = p.y, ax2(m), aty2(n), xu2, yu2;
     std::size_t iter = 0, k = 0, since = 0;
     double last_score = 1e300, prev_cand = 1e300;
@@ -231,6 +233,7 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
             }
         }
         p.a_times(p.xbar, p.ax);
+        if (halpern) yprev = p.y;
         for (std::size_t i = 0; i < m; ++i) {
             const double sg = p.sigma(i);
             const double v = p.y[i] + sg * p.ax[i];
@@ -238,8 +241,20 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
         }
         ++k;
         const double kd = static_cast<double>(k);
-        for (std::size_t j = 0; j < n; ++j) p.xavg[j] += (p.x[j] - p.xavg[j]) / kd;
-        for (std::size_t i = 0; i < m; ++i) p.yavg[i] += (p.y[i] - p.yavg[i]) / kd;
+        if (halpern) {  // T(z) into the average slot; z <- w (2 T(z) - z) + (1 - w) z0
+            const double w = kd / (kd + 1.0);
+            for (std::size_t j = 0; j < n; ++j) {
+                p.xavg[j] = p.x[j];
+                p.x[j] = w * p.xbar[j] + (1.0 - w) * xlast[j];
+            }
+            for (std::size_t i = 0; i < m; ++i) {
+                p.yavg[i] = p.y[i];
+                p.y[i] = w * (2.0 * p.y[i] - yprev[i]) + (1.0 - w) * ylast[i];
+            }
+        } else {
+            for (std::size_t j = 0; j < n; ++j) p.xavg[j] += (p.x[j] - p.xavg[j]) / kd;
+            for (std::size_t i = 0; i < m; ++i) p.yavg[i] += (p.y[i] - p.yavg[i]) / kd;
+        }
         ++iter;
         ++since;
         if (iter % opt.restart_every != 0) continue;
@@ -277,12 +292,21 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                 for (std::size_t j = 0; j < n; ++j) dxs += (xc[j] - xlast[j]) * (xc[j] - xlast[j]);
                 for (std::size_t i = 0; i < m; ++i) dys += (yc[i] - ylast[i]) * (yc[i] - ylast[i]);
                 const double dX = std::sqrt(dxs), dY = std::sqrt(dys);
+                double w = p.omega;
+                bool upd = false;
                 if (dX > 1e-10 && dY > 1e-10) {
                     const double th = opt.primal_weight_smoothing, ratio = dY / dX;
-                    const double w = th == 0.5 ? std::sqrt(p.omega * ratio)
-                                               : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
-                    p.omega = std::clamp(w, 1e-6, 1e6);
+                    w = th == 0.5 ? std::sqrt(p.omega * ratio) : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
+                    upd = true;
                 }
+                if (halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
+                    double wres = p.omega;
+                    if (cur.primal > 1e-12 && cur.dual > 1e-12)
+                        wres = p.omega * std::sqrt(std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0));
+                    w = std::sqrt(w * std::sqrt(w * wres));
+                    upd = true;
+                }
+                if (upd) p.omega = std::clamp(w, 1e-6, 1e6);
             }
             if (!use_cur) {
                 p.x = p.xavg;

@@ -337,7 +337,9 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.fp32_stall_checks = opt.fp32_stall_checks;
     o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
     o.graph_chunk = opt.graph_chunk;
-    o.restart_scheme = opt.restart_scheme == RestartScheme::kkt ? MC_SCHEME_KKT : MC_SCHEME_REFERENCE;
+    o.restart_scheme = opt.restart_scheme == RestartScheme::kkt       ? MC_SCHEME_KKT
+                       : opt.restart_scheme == RestartScheme::halpern ? MC_SCHEME_HALPERN
+                                                                      : MC_SCHEME_REFERENCE;
     if (warm_x) {
         wx.resize(n);
         for (int b = 0; b < B; ++b)

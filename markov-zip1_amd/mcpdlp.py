@@ -27,7 +27,7 @@ BACKEND_CPU, BACKEND_GPU = 0, 1
 RESTART_NONE, RESTART_ADAPTIVE, RESTART_FIXED = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 ORDER_FAST, ORDER_REFERENCE = 0, 1
-SCHEME_REFERENCE, SCHEME_KKT = 0, 1
+SCHEME_REFERENCE, SCHEME_KKT, SCHEME_HALPERN = 0, 1, 2
 STATUS = {0: "optimal", 1: "iteration_limit", 2: "infeasible_or_unbounded", 3: "numerical_failure"}
 FAMILY_UNIFORM_EQ, FAMILY_ZIPF_MIXED, FAMILY_BLOCK_ANGULAR = 1, 2, 5
 

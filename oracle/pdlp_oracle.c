@@ -222,8 +222,10 @@ typedef struct or_trace {
 
 /* solve_pdlp, reference pdlp.cpp:139-276.  With o->restart_scheme ==
  * MC_SCHEME_KKT the restart decision follows the extension documented in
- * include/mcpdlp.h (Applegate et al. 2021 adaptive restarts); everything else
- * is unchanged.  `res->primal`/`res->dual` must hold
+ * include/mcpdlp.h (Applegate et al. 2021 adaptive restarts); with
+ * MC_SCHEME_HALPERN the iterate also follows the reflected Halpern update
+ * documented there (the running average slot holds T(z)); everything else is
+ * unchanged.  `res->primal`/`res->dual` must hold
  * n / m doubles (or be NULL).  Timings are wall-clock of this restatement.
  * Optional `trace` records the per-check scalar sequence. */
 int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res,
@@ -325,7 +327,9 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
         x_bar[j] = x[j];
         x_avg[j] = x[j];
     }
-    const int kkt = (o->restart_scheme == MC_SCHEME_KKT);
+    const int halpern = (o->restart_scheme == MC_SCHEME_HALPERN);
+    const int kkt = (o->restart_scheme == MC_SCHEME_KKT) || halpern;
+    double* y_prev = (double*)calloc((size_t)m, sizeof(double));
     double* x_last = (double*)malloc(sizeof(double) * (size_t)n);
     double* y_last = (double*)calloc((size_t)m, sizeof(double));
     double* Ax2 = (double*)malloc(sizeof(double) * (size_t)m);
@@ -392,13 +396,26 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
             }
         }
         or_spmv(m, n, cs, ri, val, x_bar, Ax); /* :220-224 */
+        if (halpern) memcpy(y_prev, y, sizeof(double) * (size_t)m);
         for (int64_t i = 0; i < m; ++i) {
             const double v = y[i] + sigma[i] * Ax[i];
             y[i] = v - sigma[i] * or_clamp(v / sigma[i], b_lo[i], b_hi[i]);
         }
         ++avg_count; /* :225-227 */
-        for (int64_t j = 0; j < n; ++j) x_avg[j] += (x[j] - x_avg[j]) / (double)avg_count;
-        for (int64_t i = 0; i < m; ++i) y_avg[i] += (y[i] - y_avg[i]) / (double)avg_count;
+        if (halpern) { /* T(z) to the average slot; z <- w (2 T(z) - z) + (1 - w) z0 */
+            const double w = (double)avg_count / ((double)avg_count + 1.0);
+            for (int64_t j = 0; j < n; ++j) {
+                x_avg[j] = x[j];
+                x[j] = w * x_bar[j] + (1.0 - w) * x_last[j];
+            }
+            for (int64_t i = 0; i < m; ++i) {
+                y_avg[i] = y[i];
+                y[i] = w * (2.0 * y[i] - y_prev[i]) + (1.0 - w) * y_last[i];
+            }
+        } else {
+            for (int64_t j = 0; j < n; ++j) x_avg[j] += (x[j] - x_avg[j]) / (double)avg_count;
+            for (int64_t i = 0; i < m; ++i) y_avg[i] += (y[i] - y_avg[i]) / (double)avg_count;
+        }
         ++iter;
         ++since;
         if (iter % o->restart_every == 0) { /* :229-261 */
@@ -446,12 +463,23 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                         for (int64_t j = 0; j < n; ++j) dxs += (xc[j] - x_last[j]) * (xc[j] - x_last[j]);
                         for (int64_t i = 0; i < m; ++i) dys += (yc[i] - y_last[i]) * (yc[i] - y_last[i]);
                         const double dX = sqrt(dxs), dY = sqrt(dys);
+                        double wmove = omega;
+                        int upd = 0;
                         if (dX > 1e-10 && dY > 1e-10) {
                             const double th = o->primal_weight_smoothing;
                             const double ratio = dY / dX;
-                            const double w = (th == 0.5) ? sqrt(omega * ratio)
-                                                         : pow(ratio, th) * pow(omega, 1.0 - th);
-                            omega = or_clamp(w, 1e-6, 1e6);
+                            wmove = (th == 0.5) ? sqrt(omega * ratio) : pow(ratio, th) * pow(omega, 1.0 - th);
+                            upd = 1;
+                        }
+                        if (halpern) { /* omega_move^(3/4) omega_res^(1/4), in square roots */
+                            double wres = omega;
+                            if (cur.primal > 1e-12 && cur.dual > 1e-12)
+                                wres = omega * sqrt(or_clamp(sqrt(cur.primal / cur.dual), 0.05, 20.0));
+                            wmove = sqrt(wmove * sqrt(wmove * wres));
+                            upd = 1;
+                        }
+                        if (upd) {
+                            omega = or_clamp(wmove, 1e-6, 1e6);
                             OR_UPDATE_STEPS();
                         }
                     }
@@ -501,7 +529,7 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
     free(row_norms); free(col_norms); free(tau); free(sigma);
     free(x); free(y); free(x_bar); free(x_avg); free(y_avg); free(dx); free(x_prev);
     free(At_y); free(Ax); free(xo); free(yo);
-    free(x_last); free(y_last); free(Ax2); free(At_y2); free(xo2); free(yo2);
+    free(x_last); free(y_last); free(Ax2); free(At_y2); free(xo2); free(yo2); free(y_prev);
 #undef OR_UPDATE_STEPS
     return 0;
 }

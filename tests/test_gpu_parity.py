@@ -151,8 +151,12 @@ def test_cfg1_full_shape_reference_order(gpu):
 
 
 def test_batch_reference_order_matches_single_solves(gpu):
-    lps = [small_uniform(seed=100 + k, m=100 + 37 * k, n=250 + 90 * k) for k in range(4)] + [tiny_simplex()]
-    kw = dict(tolerance=1e-6, max_iterations=1200)
+    """Instances finish at different iterations, so the batch is re-tiled
+    over the running instances several times (8 -> 4 -> 2 -> 1); every
+    instance still follows the oracle's trajectory bit for bit."""
+    lps = ([small_uniform(seed=100 + k, m=100 + 37 * k, n=250 + 90 * k) for k in range(5)] + [tiny_simplex(), blend()]
+           + [small_mixed(seed=77, m=120, n=260)])
+    kw = dict(tolerance=1e-6, max_iterations=3000)
     got = mp.solve_batch(lps, backend=mp.BACKEND_GPU, require_gpu=1, reduction_order=mp.ORDER_REFERENCE, **kw)
     for lp, g in zip(lps, got):
         w = orc.solve(lp, **kw)
@@ -180,20 +184,23 @@ KKT_CASES = [
 ]
 
 
+@pytest.mark.parametrize("scheme", [mp.SCHEME_KKT, mp.SCHEME_HALPERN], ids=["kkt", "halpern"])
 @pytest.mark.parametrize("name,maker,kw", KKT_CASES, ids=[c[0] for c in KKT_CASES])
-def test_kkt_scheme_reference_order_bit_exact(gpu, name, maker, kw):
-    """KKT restart scheme (extension): GPU reference order == oracle, bit for bit."""
+def test_kkt_scheme_reference_order_bit_exact(gpu, name, maker, kw, scheme):
+    """KKT and Halpern restart schemes (extensions): GPU reference order ==
+    oracle, bit for bit."""
     lp = maker()
-    print("kkt case", name, flush=True)
-    want = orc.solve(lp, restart_scheme=mp.SCHEME_KKT, **kw)
-    got = gpu_solve(lp, restart_scheme=mp.SCHEME_KKT, reduction_order=mp.ORDER_REFERENCE, **kw)
+    print("scheme", scheme, "case", name, flush=True)
+    want = orc.solve(lp, restart_scheme=scheme, **kw)
+    got = gpu_solve(lp, restart_scheme=scheme, reduction_order=mp.ORDER_REFERENCE, **kw)
     assert got["status"] == want["status"] and got["iterations"] == want["iterations"]
     assert got["objective"] == want["objective"]
     assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
 
 
+@pytest.mark.parametrize("scheme", [mp.SCHEME_KKT, mp.SCHEME_HALPERN], ids=["kkt", "halpern"])
 @pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED])
-def test_kkt_scheme_converges_to_1e6(gpu, precision):
+def test_kkt_scheme_converges_to_1e6(gpu, precision, scheme):
     """Fast order, FP64 or mixed: reaches 1e-6 on a planted LP; the returned pair
     is re-scored independently by the oracle's FP64 residuals on the host, and its
     objective matches the planted optimum c^T x* to 1e-5 relative to the size of
@@ -201,7 +208,7 @@ def test_kkt_scheme_converges_to_1e6(gpu, precision):
     residuals, not |p - p*| itself; this LP's objective is ~0.1 while its terms
     are O(1e3))."""
     lp = small_uniform()  # the oracle converges on it in 8720 iterations (FP64, KKT scheme)
-    got = gpu_solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, precision=precision,
+    got = gpu_solve(lp, tolerance=1e-6, restart_scheme=scheme, precision=precision,
                     max_iterations=200000)
     assert got["status"] == "optimal", got["message"]
     k = orc.kkt(lp, got["primal"], got["dual"])
@@ -221,7 +228,8 @@ def netlib(name):
     return mp.read_mps(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "netlib", f"{name}.mps"))
 
 
-@pytest.mark.parametrize("name,scheme", [("afiro", 0), ("recipe", 0), ("adlittle", 1), ("recipe", 1)])
+@pytest.mark.parametrize("name,scheme", [("afiro", 0), ("recipe", 0), ("adlittle", 1), ("recipe", 1),
+                                         ("adlittle", 2), ("recipe", 2)])
 def test_netlib_reference_order_bit_exact(gpu, name, scheme):
     """Real LPs (free rows, ranges, bounds of every kind): the GPU reference
     order reproduces the oracle's whole solve bit for bit."""
@@ -235,12 +243,13 @@ def test_netlib_reference_order_bit_exact(gpu, name, scheme):
     assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
 
 
+@pytest.mark.parametrize("scheme", [mp.SCHEME_KKT, mp.SCHEME_HALPERN], ids=["kkt", "halpern"])
 @pytest.mark.parametrize("name", sorted(NETLIB))
 @pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED])
-def test_netlib_fast_order_reaches_published_optimum(gpu, name, precision):
+def test_netlib_fast_order_reaches_published_optimum(gpu, name, precision, scheme):
     lp = netlib(name)
-    print("netlib fast", name, precision, flush=True)
-    r = gpu_solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, precision=precision, max_iterations=200000)
+    print("netlib fast", name, precision, scheme, flush=True)
+    r = gpu_solve(lp, tolerance=1e-6, restart_scheme=scheme, precision=precision, max_iterations=200000)
     assert r["status"] == "optimal"
     assert abs(r["objective"] - NETLIB[name]) <= 1e-6 * abs(NETLIB[name])
     k = orc.kkt(lp, r["primal"], r["dual"])

@@ -63,6 +63,10 @@ CASES = [
      dict(tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, max_iterations=20000)),
     (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
      dict(tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, max_iterations=3000)),
+    (lambda: mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, max_iterations=20000)),
+    (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, max_iterations=3000)),
 ]
 
 
@@ -224,3 +228,17 @@ def test_generators_deterministic_and_planted():
         assert np.array_equal(q.row_index, base.row_index) and np.array_equal(q.value, base.value)
         assert not np.array_equal(q.objective, base.objective)
         assert orc.kkt(q, q.x_star, -q.y_star)["score"] < 1e-11
+
+
+def test_cli_restart_scheme_flag():
+    """--restart-scheme halpern solves Netlib adlittle (optimum 225494.96316238,
+    scripts/run_netlib.py) to 1e-6; the reference rule does not within the
+    CLI's 100000-iteration default; an unknown scheme is a usage error."""
+    mps = os.path.join(GOLDEN, "netlib", "adlittle.mps")
+    code, js, err = run_cli([mps, "--engine", "pdlp", "--tolerance", "1e-6", "--restart-scheme", "halpern"])
+    assert code == 0 and js["status"] == "Optimal" and js["verified"]
+    assert abs(js["objective"] - 225494.96316238) <= 1e-6 * 225494.96316238
+    code, js, err = run_cli([mps, "--engine", "pdlp", "--tolerance", "1e-6"])
+    assert code == 6 and js["status"] == "IterationLimit"
+    p = subprocess.run([mp.CLI, mps, "--restart-scheme", "bogus"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 4 and "invalid restart scheme" in p.stderr

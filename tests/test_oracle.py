@@ -216,11 +216,12 @@ def netlib(name):
     return mp.read_mps(os.path.join(GOLDEN, "netlib", f"{name}.mps"))
 
 
-@pytest.mark.parametrize("name,scheme", [("afiro", 0), ("recipe", 0), ("afiro", 1), ("adlittle", 1), ("recipe", 1)])
+@pytest.mark.parametrize("name,scheme", [("afiro", 0), ("recipe", 0), ("afiro", 1), ("adlittle", 1), ("recipe", 1),
+                                         ("afiro", 2), ("adlittle", 2), ("recipe", 2)])
 def test_netlib_optima(name, scheme):
     """The oracle reaches each published optimum (1e-6 KKT, objective within
     1e-6 relative) -- under the reference's restart rule where that rule
-    converges in 400k iterations, and under the KKT restart scheme."""
+    converges in 400k iterations, and under the KKT and Halpern schemes."""
     lp = netlib(name)
     r = orc.solve(lp, tolerance=1e-6, restart_scheme=scheme, max_iterations=400000)
     assert r["status"] == "optimal"
@@ -234,3 +235,39 @@ def test_netlib_adlittle_reference_rule_stalls():
     KKT restart scheme exists to fix."""
     r = orc.solve(netlib("adlittle"), tolerance=1e-6, max_iterations=400000)
     assert r["status"] == "iteration_limit" and r["dual_infeasibility"] > 1e-5
+
+
+def test_halpern_scheme_reaches_planted_optimum():
+    """Halpern scheme (extension) in the oracle: converges to tolerance on a
+    uniform and a Zipf LP, the pair re-scored independently is within
+    tolerance and the objective equals the planted optimum."""
+    for lp in (mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11),
+               mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=31)):
+        r = orc.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, max_iterations=200000)
+        assert r["status"] == "optimal"
+        k = orc.kkt(lp, r["primal"], r["dual"])
+        assert k["score"] <= 1e-6 * 1.01
+        assert abs(r["objective"] - lp.planted_objective) <= 1e-5 * (1 + abs(lp.planted_objective))
+
+
+def test_halpern_iterates_by_hand():
+    """Three Halpern steps worked by hand on min x s.t. x = 1, 0 <= x <= 2
+    (no scaling, omega = 1, eta = 0.9 fixed, so tau = sigma = 0.9), from
+    z0 = (0, 0); T is the PDHG step, z_k+1 = w (2 T(z_k) - z_k) + (1 - w) z0
+    with w = (k+1)/(k+2):
+      k=0: T = (0, -0.9),    w = 1/2: z1 = (0, -0.9)
+      k=1: T = (0, -1.8),    w = 2/3: z2 = (0, -1.8)
+      k=2: T = (0.72, -1.404), x_bar = 1.44, w = 3/4:
+           z3 = (3/4 * 1.44, 3/4 * (2 * -1.404 + 1.8)) = (1.08, -0.756).
+    z3 has the lower KKT score of the two candidates, so it is returned.  The
+    plain PDHG schemes return other points here (a mistake in w, in the
+    reflection or in the anchor changes z3)."""
+    lp = mp.LPData.from_dense([[1.0]], [1.0], [1.0], [1.0], [0.0], [2.0])
+    kw = dict(tolerance=1e-12, ruiz_scaling=0, initial_primal_weight=1.0, adaptive_step_size=0)
+    want = {1: (0.0, -0.9), 2: (0.0, -1.8), 3: (1.08, -0.756)}
+    for it, (x, y) in want.items():
+        r = orc.solve(lp, restart_scheme=mp.SCHEME_HALPERN, max_iterations=it, **kw)
+        assert r["primal"][0] == pytest.approx(x, rel=1e-14, abs=1e-15)
+        assert r["dual"][0] == pytest.approx(y, rel=1e-14)
+    r = orc.solve(lp, restart_scheme=mp.SCHEME_KKT, max_iterations=3, **kw)
+    assert r["primal"][0] != pytest.approx(1.08)
