@@ -7,7 +7,9 @@ Workload (default): the single large planted LP -- m = 200,000 rows (70 % '=',
 score <= 1e-6 (the reference's score: max of per-row relative primal
 violation, relative dual residual, relative gap; pdlp.cpp:62-135) through the
 product C-ABI mc_solve_pdlp with Backend::gpu, mixed precision (FP32 SpMV and
-updates, FP64 checks, FP64 polish) and the KKT restart scheme.
+updates, FP64 checks, FP64 polish) and the reflected Halpern restart
+scheme (--scheme; the plain FP64 comparison uses the classic PDLP restarts to
+the average, --plain-scheme kkt).
 
 One "step" is one full solve: host packing + H2D + device Ruiz + PDHG
 iterations + D2H (the reference's total_ms also starts before Ruiz,
@@ -51,6 +53,9 @@ WORKLOADS = {
 }
 
 
+SCHEMES = {"reference": 0, "kkt": 1, "halpern": 2}  # mcpdlp.SCHEME_*
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -70,6 +75,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="approximate CPU sample budget")
     ap.add_argument("--graph-chunk", type=int, default=0, help="iterations per HIP graph (<0: eager launches)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--scheme", default="halpern", choices=sorted(SCHEMES),
+                    help="restart scheme of the measured solves")
+    ap.add_argument("--plain-scheme", default="kkt", choices=sorted(SCHEMES),
+                    help="restart scheme of the plain FP64 comparison (classic PDLP restarts to the average)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=16, help="batch: instances solved one by one (FP64)")
     ap.add_argument("--train", type=int, default=384, help="warm: training instances")
@@ -102,7 +111,7 @@ def main():
 
     prec = mp.PRECISION_MIXED if args.precision == "mixed" else mp.PRECISION_FP64
     opts = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=prec,
-                      restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations,
+                      restart_scheme=SCHEMES[args.scheme], max_iterations=args.max_iterations,
                       graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
 
     def run(o):
@@ -135,10 +144,11 @@ def main():
     plain = None
     if args.plain:
         po = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_FP64,
-                        restart_scheme=mp.SCHEME_KKT, max_iterations=args.max_iterations,
+                        restart_scheme=SCHEMES[args.plain_scheme], max_iterations=args.max_iterations,
                         graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
         pr, pwall = run(po)
         plain = {"ms": round(pwall, 3), "status": pr["status"], "iterations": pr["iterations"],
+                 "restart_scheme": args.plain_scheme,
                  "kernel_ms": round(pr["kernel_ms"], 3)}
         log(f"[bench] plain FP64: {pr['status']} it={pr['iterations']} wall={pwall:.1f}ms")
 
@@ -180,7 +190,7 @@ def main():
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "fp32+fp64" if prec == mp.PRECISION_MIXED else "fp64", "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
-                   "precision": args.precision, "restart_scheme": "kkt", "parallelism": "single GPU"},
+                   "precision": args.precision, "restart_scheme": args.scheme, "parallelism": "single GPU"},
         "status": res["status"], "iterations": res["iterations"], "fp32_iterations": res["fp32_iterations"],
         "kernel_ms": round(res["kernel_ms"], 3), "setup_ms": round(res["setup_ms"], 3),
         "h2d_ms": round(res["h2d_ms"], 3), "d2h_ms": round(res["d2h_ms"], 3),
@@ -216,9 +226,9 @@ def time_kernels(mp, lp, kinds, autotune, reps=50):
 
 def measured_traffic(workload, kind):
     """HBM bytes per launch of `kind` on `workload` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (profiles/traffic_<workload>.txt: lines
+    FETCH_SIZE/WRITE_SIZE summary (profiles/k_primal_k_dual_<workload>_traffic.txt: lines
     'traffic <kind> <bytes>'), or None when no such summary exists."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    path = os.path.join(ROOT, "profiles", f"k_primal_k_dual_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -267,10 +277,10 @@ def bench_batch(args):
     lps = mp.generate_batch(args.batch_count)
     log(f"[bench] batch of {len(lps)}: sum m={sum(l.rows for l in lps)} sum nnz={sum(l.nnz for l in lps)} "
         f"generated in {time.time() - t:.1f}s")
-    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=mp.SCHEME_KKT,
+    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1,
               max_iterations=args.max_iterations, graph_chunk=args.graph_chunk)
     t0 = time.perf_counter()
-    res = mp.solve_batch(lps, precision=mp.PRECISION_MIXED, **kw)
+    res = mp.solve_batch(lps, precision=mp.PRECISION_MIXED, restart_scheme=SCHEMES[args.scheme], **kw)
     wall = (time.perf_counter() - t0) * 1e3
     its = np.array([r["iterations"] for r in res])
     ok = sum(r["status"] == "optimal" for r in res)
@@ -287,14 +297,15 @@ def bench_batch(args):
     plain_ms, plain_it = [], []
     for k in sample:
         t1 = time.perf_counter()
-        r = mp.solve(lps[k], precision=mp.PRECISION_FP64, **kw)
+        r = mp.solve(lps[k], precision=mp.PRECISION_FP64, restart_scheme=SCHEMES[args.plain_scheme], **kw)
         plain_ms.append((time.perf_counter() - t1) * 1e3)
         plain_it.append(r["iterations"])
         log(f"[bench] plain FP64 instance m={lps[k].rows}: {r['status']} it={r['iterations']} {plain_ms[-1]:.1f}ms")
     est = float(np.mean(plain_ms)) * len(lps)
     _line(wall, WORKLOADS["batch"]["name"],
           {"instances": len(lps), "sum_rows": int(sum(l.rows for l in lps)), "sum_nnz": int(sum(l.nnz for l in lps)),
-           "tolerance": tol, "precision": "mixed", "restart_scheme": "kkt"},
+           "tolerance": tol, "precision": "mixed", "restart_scheme": args.scheme,
+           "plain_restart_scheme": args.plain_scheme},
           {"optimal": int(ok), "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
            "kkt_fp64_rescored_worst_of_sample": float(f"{worst:.4e}"),
            "plain_fp64_per_instance": {"sampled": int(len(sample)), "mean_ms": round(float(np.mean(plain_ms)), 3),
@@ -318,7 +329,7 @@ def bench_warm(args):
     train, test = fam[:ntr], fam[ntr:]
     log(f"[bench] family: {ntr} train + {nte} test instances of {base.rows}x{base.cols} "
         f"({base.nnz} nnz) in {time.time() - t:.1f}s")
-    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=mp.SCHEME_KKT,
+    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
               max_iterations=args.max_iterations, precision=mp.PRECISION_MIXED, graph_chunk=args.graph_chunk)
     t0 = time.perf_counter()
     sols = []
@@ -344,7 +355,7 @@ def bench_warm(args):
         f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference)")
     _line(warm_ms, WORKLOADS["warm"]["name"],
           {"train": ntr, "test": nte, "m": base.rows, "n": base.cols, "nnz": base.nnz, "tolerance": tol,
-           "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": "kkt"},
+           "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": args.scheme},
           {"cold_ms": round(cold_ms, 1), "warm_ms": round(warm_ms, 1),
            "iterations_cold_mean": float(ci.mean()), "iterations_warm_mean": float(wi.mean()),
            "iterations_cold_max": int(ci.max()), "iterations_warm_max": int(wi.max()),
