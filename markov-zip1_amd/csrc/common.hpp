@@ -147,11 +147,15 @@ __device__ __forceinline__ T block_max(T v, T* scratch) {
 template <typename TS, typename TA, typename TV>
 __device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const int32_t* __restrict__ idx,
                                       const TA* __restrict__ val, const TV* __restrict__ vec) {
+    // The matrix streams (idx, val) are read once per sweep: non-temporal
+    // loads keep them from evicting the gathered vector's lines from L2.
     TS s = 0;
     int q = p0 + gl;
     for (; q + 3 * G < p1; q += 4 * G) {
-        const int i0 = idx[q], i1 = idx[q + G], i2 = idx[q + 2 * G], i3 = idx[q + 3 * G];
-        const TA a0 = val[q], a1 = val[q + G], a2 = val[q + 2 * G], a3 = val[q + 3 * G];
+        const int i0 = __builtin_nontemporal_load(idx + q), i1 = __builtin_nontemporal_load(idx + q + G),
+                  i2 = __builtin_nontemporal_load(idx + q + 2 * G), i3 = __builtin_nontemporal_load(idx + q + 3 * G);
+        const TA a0 = __builtin_nontemporal_load(val + q), a1 = __builtin_nontemporal_load(val + q + G),
+                 a2 = __builtin_nontemporal_load(val + q + 2 * G), a3 = __builtin_nontemporal_load(val + q + 3 * G);
         const TS t0 = (TS)a0 * (TS)vec[i0];
         const TS t1 = (TS)a1 * (TS)vec[i1];
         const TS t2 = (TS)a2 * (TS)vec[i2];
@@ -162,7 +166,7 @@ __device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const int32
         s += t3;
     }
     for (; q < p1; q += G) {
-        const TS t = (TS)val[q] * (TS)vec[idx[q]];
+        const TS t = (TS)__builtin_nontemporal_load(val + q) * (TS)vec[__builtin_nontemporal_load(idx + q)];
         s += t;
     }
     for (int off = G >> 1; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
