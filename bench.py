@@ -25,7 +25,8 @@ to the same tolerance (several more minutes on this LP).
 Other workloads: --workload batch (256 uneven LPs in one batched solve vs the
 plain FP64 GPU solve per instance), --workload warm (learned warm start vs
 cold start on a perturbed LP family), --workload cpu-lp (the reference's
-CPU-runnable 2000 x 5000 LP).  --profile-kernels KINDS only times fused
+CPU-runnable 2000 x 5000 LP), --workload scale (the 48-period block-angular
+LP, 4.8M columns, ~48M nonzeros).  --profile-kernels KINDS only times fused
 kernels, for rocprofv3 runs.
 """
 import argparse
@@ -46,6 +47,9 @@ WORKLOADS = {
                   name="single LP m=200000 n=500000, Zipf(1.2) nnz/col in [1,200], 70% '=' rows"),
     "cpu-lp": dict(family=1, spec={}, tol=1e-6,
                    name="single LP m=2000 n=5000, 5 nnz/col, '=' rows, 0<=x<=10"),
+    "scale": dict(family=5, spec={}, tol=1e-6,
+                  name="block-angular LP, 48 periods x (40000 rows x 100000 cols, 8 nnz/col) + 200000 "
+                       "linking rows (2 nnz/col), |a| log-uniform [1e-2,1e2]"),
     "batch": dict(tol=1e-6, name="256 planted LPs, m log-uniform [500,20000], n=2.5m, 4-8 nnz/col, "
                                  "box [0,U[1,100]], one batched solve"),
     "warm": dict(tol=1e-6, name="LP family on one structure m=2000 n=5000 6 nnz/col, b/c/u perturbed "
@@ -244,7 +248,7 @@ def profile_kernels(args):
     import mcpdlp as mp
     wl = WORKLOADS[args.workload]
     if "family" not in wl:
-        raise SystemExit("--profile-kernels takes a single-LP workload (large or cpu-lp)")
+        raise SystemExit("--profile-kernels takes a single-LP workload (large, scale or cpu-lp)")
     lp = mp.generate(wl["family"], **wl["spec"])
     kinds = [k for k in args.profile_kernels.split(",") if k]
     for k in kinds:

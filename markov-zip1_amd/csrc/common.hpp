@@ -147,27 +147,26 @@ __device__ __forceinline__ T block_max(T v, T* scratch) {
 template <typename TS, typename TA, typename TV>
 __device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const int32_t* __restrict__ idx,
                                       const TA* __restrict__ val, const TV* __restrict__ vec) {
-    // The matrix streams (idx, val) are read once per sweep: non-temporal
-    // loads keep them from evicting the gathered vector's lines from L2.
+    // Every round issues four index/value loads and four gathers per lane
+    // before the first product is used, the tail round included: a lane
+    // past its row's end re-reads its first entry of the round (always in
+    // bounds) and skips the sum, so a row costs ceil(len / 4G) memory
+    // round trips.  Products are added in index order (bit-exact with a
+    // serial sum when G == 1).
     TS s = 0;
-    int q = p0 + gl;
-    for (; q + 3 * G < p1; q += 4 * G) {
-        const int i0 = __builtin_nontemporal_load(idx + q), i1 = __builtin_nontemporal_load(idx + q + G),
-                  i2 = __builtin_nontemporal_load(idx + q + 2 * G), i3 = __builtin_nontemporal_load(idx + q + 3 * G);
-        const TA a0 = __builtin_nontemporal_load(val + q), a1 = __builtin_nontemporal_load(val + q + G),
-                 a2 = __builtin_nontemporal_load(val + q + 2 * G), a3 = __builtin_nontemporal_load(val + q + 3 * G);
+    for (int q = p0 + gl; q < p1; q += 4 * G) {
+        const bool m1 = q + G < p1, m2 = q + 2 * G < p1, m3 = q + 3 * G < p1;
+        const int q1 = m1 ? q + G : q, q2 = m2 ? q + 2 * G : q, q3 = m3 ? q + 3 * G : q;
+        const int i0 = idx[q], i1 = idx[q1], i2 = idx[q2], i3 = idx[q3];
+        const TA a0 = val[q], a1 = val[q1], a2 = val[q2], a3 = val[q3];
         const TS t0 = (TS)a0 * (TS)vec[i0];
         const TS t1 = (TS)a1 * (TS)vec[i1];
         const TS t2 = (TS)a2 * (TS)vec[i2];
         const TS t3 = (TS)a3 * (TS)vec[i3];
         s += t0;
-        s += t1;
-        s += t2;
-        s += t3;
-    }
-    for (; q < p1; q += G) {
-        const TS t = (TS)__builtin_nontemporal_load(val + q) * (TS)vec[__builtin_nontemporal_load(idx + q)];
-        s += t;
+        if (m1) s += t1;
+        if (m2) s += t2;
+        if (m3) s += t3;
     }
     for (int off = G >> 1; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;

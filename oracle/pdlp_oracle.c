@@ -20,7 +20,10 @@
  *                               fixtures (iterations,
  *                               objective, residuals, vector hashes), ADR-P3-05
  *                               known answer, pdlp_test objectives, planted
- *                               optimum c^T x*, Netlib optima
+ *                               optimum c^T x*, Netlib optima (afiro, adlittle,
+ *                               recipe); the KKT and Halpern extension schemes
+ *                               by planted and Netlib optima, and the Halpern
+ *                               update by three iterations worked by hand
  *   or_verify_primal         -- hand cases + compiled reference
  */
 #include <math.h>
