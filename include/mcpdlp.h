@@ -76,7 +76,9 @@ typedef struct mc_lp {
 #define MC_RESTART_FIXED 2
 /* Arithmetic of the GPU engine (extension; no reference counterpart). */
 #define MC_PRECISION_FP64 0  /* every iteration in FP64                                   */
-#define MC_PRECISION_MIXED 1 /* FP32 SpMV/updates, FP64 checks, FP64 polish on stall     */
+#define MC_PRECISION_MIXED 1 /* FP32 SpMVs (values, gathered vectors, products and sums); the
+                              * iterate, its updates and every check stay FP64; all-FP64
+                              * iterations once the FP32-SpMV phase stalls              */
 /* Restart scheme (extension; both backends).
  * REFERENCE: the reference's rule (pdlp.cpp:246-260): restart to the average when
  *   its score fell to 0.368 x the last restart's, primal weight from sqrt(primal/dual).
@@ -167,7 +169,7 @@ typedef struct mc_pdlp_result {
     char message[128];
     char backend_actually_used[32]; /* "cpu", "hip" or "cpu_fallback" */
     /* --- extensions --- */
-    uint64_t fp32_iterations;       /* iterations run in FP32 before the FP64 polish (mixed) */
+    uint64_t fp32_iterations;       /* iterations with FP32 SpMVs before the all-FP64 phase (mixed) */
     double setup_ms;                /* Ruiz + transpose + norms on the device          */
     double primal_weight;           /* final primal weight omega (scaled space)        */
     double step_size;               /* final step-size factor eta                      */

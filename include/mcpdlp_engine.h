@@ -114,7 +114,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats);
 /* Per-instance outcome. */
 typedef struct mcp_inst_result {
     int32_t status;             /* MC_PDLP_* */
-    int32_t mode;               /* 0 FP64, 1 FP32 (final iterate precision) */
+    int32_t mode;               /* SpMV precision at the end: 0 FP64, 1 FP32 */
     uint64_t iterations;
     uint64_t fp32_iterations;
     double objective;

@@ -46,10 +46,11 @@ struct InstState {
     int64_t done_iter;       // iteration count when the instance stopped
     int64_t fp32_iters;      // iterations spent in FP32 before switching to FP64
     int32_t status;
-    int32_t mode;            // kModeFp64 / kModeFp32
+    int32_t mode;            // kModeFp64 / kModeFp32 (FP32 SpMV values and gathers; the
+                             // iterate and every update stay FP64 in both modes)
     int32_t restart_flag;    // set by a check, consumed by k_post_check
-    int32_t switch_flag;     // set by a check: convert FP32 iterate to FP64
-    int32_t mode_at_check;   // mode in which the last check's iterate lives
+    int32_t switch_flag;     // set by the check that moved the instance from FP32 to FP64
+    int32_t mode_at_check;   // mode of the iterations before the last check
     int32_t stall;           // FP32: checks since the last new best score
     int32_t cand_cur;        // KKT scheme: this check's candidate is the current iterate
     int32_t answer_cur;      // the reported iterate is the current one (else the average)

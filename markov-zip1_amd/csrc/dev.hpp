@@ -50,15 +50,13 @@ struct Dev {
     double* slo;
     double* shi;
     double* cnorm;   // [n]
-    float4* rowpk;   // [m] {blo, bhi, 1/rnorm, 0} in FP32 (+-inf for open bounds)
-    float4* colpk;   // [n] {c, lo, hi, 1/cnorm}
-    // iterates
+    // iterates (FP64 in both precision modes)
     double *x, *xbar, *xavg, *dx;
     double *y, *yavg;
-    float *x32, *xbar32, *xavg32, *dx32;
-    float *y32, *yavg32;
-    double *xlast, *ylast;      // KKT scheme: iterate at the last restart
-    float *xlast32, *ylast32;
+    double *xlast, *ylast;      // KKT/Halpern schemes: iterate at the last restart
+    // FP32 copies that FP32-mode SpMVs gather (written by the step that
+    // produces the FP64 value)
+    float *xbar32, *dx32, *y32;
     double* ax;    // [m] reference-order mode: A*v results (average candidate)
     double* aty;   // [n] reference-order mode: A^T*v results
     double* ax2;   // [m] reference-order mode, current-iterate candidate

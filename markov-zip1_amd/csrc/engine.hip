@@ -425,24 +425,17 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         d.slo = e->buf<double>("slo", n);
         d.shi = e->buf<double>("shi", n);
         d.cnorm = e->buf<double>("cnorm", n);
-        d.rowpk = e->buf<float4>("rowpk", m);
-        d.colpk = e->buf<float4>("colpk", n);
         d.x = e->buf<double>("x", n);
         d.xbar = e->buf<double>("xbar", n);
         d.xavg = e->buf<double>("xavg", n);
         d.dx = e->buf<double>("dx", n);
         d.y = e->buf<double>("y", m);
         d.yavg = e->buf<double>("yavg", m);
-        d.x32 = e->buf<float>("x32", n);
         d.xbar32 = e->buf<float>("xbar32", n);
-        d.xavg32 = e->buf<float>("xavg32", n);
         d.dx32 = e->buf<float>("dx32", n);
         d.y32 = e->buf<float>("y32", m);
-        d.yavg32 = e->buf<float>("yavg32", m);
         d.xlast = e->buf<double>("xlast", n);
         d.ylast = e->buf<double>("ylast", m);
-        d.xlast32 = e->buf<float>("xlast32", n);
-        d.ylast32 = e->buf<float>("ylast32", m);
         d.ax = e->buf<double>("ax", m);
         d.aty = e->buf<double>("aty", n);
         d.ax2 = e->buf<double>("ax2", m);
@@ -797,15 +790,17 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         const double m = e->m, n = e->n, nnz = (double)e->nnz;
         const bool f32 = kind == 0 || kind == 1 || kind >= 4;
         const double w = f32 ? 4.0 : 8.0;  // bytes per value
-        // Per-element operands: primal reads c/lo/hi/colnorm (packed float4 in
-        // FP32, four doubles in FP64), x, x_avg and writes x, x_bar, x_avg;
-        // dual reads blo/bhi/rownorm (float4 / three doubles), y, y_avg and
-        // writes y, y_avg.
+        // Matrix (index + value of width w) and pointers once, the gathered
+        // vector (width w) once.  Per element, FP64 in both modes: the primal
+        // step reads c/lo/hi/colnorm (32 B) and x, x_avg (16 B) and writes x,
+        // x_bar, x_avg (24 B), plus the FP32 copy of x_bar (4 B) in FP32 mode;
+        // the dual step reads blo/bhi/rownorm (24 B) and y, y_avg (16 B) and
+        // writes y, y_avg (16 B), plus the FP32 copy of y in FP32 mode.
         double b = 0.0;
         if (kind == 0 || kind == 2)
-            b = (4.0 + w) * nnz + 4.0 * (n + 1) + w * m + (f32 ? 16.0 : 32.0) * n + 2 * w * n + 3 * w * n;
+            b = (4.0 + w) * nnz + 4.0 * (n + 1) + w * m + (72.0 + (f32 ? 4.0 : 0.0)) * n;
         else if (kind == 1 || kind == 3)
-            b = (4.0 + w) * nnz + 4.0 * (m + 1) + w * n + (f32 ? 16.0 : 24.0) * m + 2 * w * m + 2 * w * m;
+            b = (4.0 + w) * nnz + 4.0 * (m + 1) + w * n + (56.0 + (f32 ? 4.0 : 0.0)) * m;
         else if (kind == 4)
             b = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 4.0 * m;
         else if (kind == 5)
@@ -832,8 +827,8 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 case 2: mcp::launch_primal(d, 1, false, false, s); break;
                 case 1:
                 case 3: mcp::launch_dual(d, 1, false, s); break;
-                case 4: mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.x32, d.y32, true, s); break;
-                case 5: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.x32, true, s); break;
+                case 4: mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.xbar32, d.y32, true, s); break;
+                case 5: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.dx32, true, s); break;
                 default: mcp::launch_copy(d.av32, copy_dst, ncopy, s); break;
             }
         };

@@ -45,6 +45,9 @@ __device__ __forceinline__ Lanes lanes_of(const Tile& tl) {
 
 // ---------------------------------------------------------------------------
 // k_primal: z = A^T y fused with the primal update of every column.
+// The iterate is FP64 in both modes; FP32 mode gathers the FP32 copy of y
+// through the FP32 values of A^T (FP32 products and sums) and also writes the
+// FP32 copies of x_bar (and dx) that the next FP32 SpMVs gather.
 template <bool ADAPT, bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
     const int tile = wave_tile(d.ntT);
@@ -56,85 +59,49 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
     const int64_t iter = *d.iter_base + t;
     const double eta = s.eta, omega = s.omega;
     const double kavg = (double)(iter - s.restart_iter + 1);
-    double part = 0.0;
+    const bool f32 = !EXACT && s.mode == kModeFp32;
     // Halpern scheme: x_avg <- T(z)_x = x+, x <- w x_bar + (1 - w) x0 with
     // x_bar = 2 x+ - x (the reflection), w = k/(k+1), k = kavg.
     const bool halp = d.halpern != 0;
     const double w = kavg / (kavg + 1.0);
-    if (!EXACT && s.mode == kModeFp32) {
-        const float tauf = (float)(eta / omega), kf = (float)kavg, wf = (float)w;
-        const float* xsrc = halp ? d.xlast32 : d.xavg32;
-        for (int base = tl.r0; base < tl.r1; base += L.step) {
-            const int j = base + (L.lane >> L.lg);
-            const bool own = j < tl.r1;
-            int p0 = 0, p1 = 0;
-            float4 pk = make_float4(0.f, 0.f, 0.f, 0.f);
-            float xo = 0.f, xa = 0.f;
-            double cn = 0.0;
-            if (own) {  // operands in flight while the column streams
-                p0 = d.cp[j];
-                p1 = d.cp[j + 1];
-                pk = d.colpk[j];  // c, lo, hi, 1/colnorm
-                xo = d.x32[j];
-                xa = xsrc[j];     // running average, or the Halpern anchor
-                if (ADAPT) cn = d.cnorm[j];
-            }
-            const float aty = row_dot<float>(p0, p1, L.gl, L.G, d.ri, d.tv32, d.y32);
-            if (own && L.gl == 0) {
-                const float g = pk.x + aty;
-                const float xp = clampv(xo - (tauf * pk.w) * g, pk.y, pk.z);
-                const float dd = xp - xo;
-                if (ADAPT) {
-                    d.dx32[j] = dd;
-                    part += cn * (double)dd * (double)dd;
-                }
-                const float xb = 2.0f * xp - xo;
-                d.xbar32[j] = xb;
-                if (halp) {
-                    d.xavg32[j] = xp;
-                    d.x32[j] = wf * xb + (1.0f - wf) * xa;
-                } else {
-                    d.x32[j] = xp;
-                    d.xavg32[j] = xa + (xp - xa) / kf;
-                }
-            }
+    const double* xsrc = halp ? d.xlast : d.xavg;
+    double part = 0.0;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int j = base + (L.lane >> L.lg);
+        const bool own = j < tl.r1;
+        int p0 = 0, p1 = 0;
+        double c = 0, cn = 1, lo = 0, hi = 0, xo = 0, xa = 0;
+        if (own) {  // operands in flight while the column streams
+            p0 = d.cp[j];
+            p1 = d.cp[j + 1];
+            c = d.c[j];
+            cn = d.cnorm[j];
+            lo = d.slo[j];
+            hi = d.shi[j];
+            xo = d.x[j];
+            xa = xsrc[j];  // running average, or the Halpern anchor
         }
-    } else {
-        const double* xsrc = halp ? d.xlast : d.xavg;
-        for (int base = tl.r0; base < tl.r1; base += L.step) {
-            const int j = base + (L.lane >> L.lg);
-            const bool own = j < tl.r1;
-            int p0 = 0, p1 = 0;
-            double c = 0, cn = 1, lo = 0, hi = 0, xo = 0, xa = 0;
-            if (own) {
-                p0 = d.cp[j];
-                p1 = d.cp[j + 1];
-                c = d.c[j];
-                cn = d.cnorm[j];
-                lo = d.slo[j];
-                hi = d.shi[j];
-                xo = d.x[j];
-                xa = xsrc[j];
+        const double aty = f32 ? (double)row_dot<float>(p0, p1, L.gl, L.G, d.ri, d.tv32, d.y32)
+                               : row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.y);
+        if (own && L.gl == 0) {
+            const double g = c + aty;
+            const double tau = (eta / omega) / cn;
+            const double xp = clampv(xo - tau * g, lo, hi);
+            const double dd = xp - xo;
+            if (ADAPT) {
+                d.dx[j] = dd;
+                if (f32) d.dx32[j] = (float)dd;
+                part += cn * dd * dd;
             }
-            const double aty = row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.y);
-            if (own && L.gl == 0) {
-                const double g = c + aty;
-                const double tau = (eta / omega) / cn;
-                const double xp = clampv(xo - tau * g, lo, hi);
-                const double dd = xp - xo;
-                if (ADAPT) {
-                    d.dx[j] = dd;
-                    part += cn * dd * dd;
-                }
-                const double xb = 2.0 * xp - xo;
-                d.xbar[j] = xb;
-                if (halp) {
-                    d.xavg[j] = xp;
-                    d.x[j] = w * xb + (1.0 - w) * xa;
-                } else {
-                    d.x[j] = xp;
-                    d.xavg[j] = xa + (xp - xa) / kavg;
-                }
+            const double xb = 2.0 * xp - xo;
+            d.xbar[j] = xb;
+            if (f32) d.xbar32[j] = (float)xb;
+            if (halp) {
+                d.xavg[j] = xp;
+                d.x[j] = w * xb + (1.0 - w) * xa;
+            } else {
+                d.x[j] = xp;
+                d.xavg[j] = xa + (xp - xa) / kavg;
             }
         }
     }
@@ -145,7 +112,9 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
 }
 
 // ---------------------------------------------------------------------------
-// k_dual: A x_bar fused with the dual prox step and the dual running average.
+// k_dual: A x_bar fused with the dual prox step and the dual running average
+// (FP64 iterate; FP32 mode gathers the FP32 x_bar through the FP32 values of
+// A and refreshes the FP32 copy of y that the next primal step gathers).
 template <bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
     const int tile = wave_tile(d.ntA);
@@ -157,68 +126,41 @@ __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
     const int64_t iter = *d.iter_base + t;
     const double eta = s.eta, omega = s.omega;
     const double kavg = (double)(iter - s.restart_iter + 1);
+    const bool f32 = !EXACT && s.mode == kModeFp32;
     // Halpern scheme: y_avg <- T(z)_y = y+, y <- w (2 y+ - y) + (1 - w) y0.
     const bool halp = d.halpern != 0;
     const double w = kavg / (kavg + 1.0);
-    if (!EXACT && s.mode == kModeFp32) {
-        const float sigf = (float)(eta * omega), kf = (float)kavg, wf = (float)w;
-        const float* ysrc = halp ? d.ylast32 : d.yavg32;
-        for (int base = tl.r0; base < tl.r1; base += L.step) {
-            const int i = base + (L.lane >> L.lg);
-            const bool own = i < tl.r1;
-            int p0 = 0, p1 = 0;
-            float4 pk = make_float4(0.f, 0.f, 1.f, 0.f);
-            float yo = 0.f, ya = 0.f;
-            if (own) {
-                p0 = d.rp[i];
-                p1 = d.rp[i + 1];
-                pk = d.rowpk[i];  // blo, bhi, 1/rownorm
-                yo = d.y32[i];
-                ya = ysrc[i];     // running average, or the Halpern anchor
-            }
-            const float ax = row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.xbar32);
-            if (own && L.gl == 0) {
-                const float sigma = sigf * pk.z;
-                const float v = yo + sigma * ax;
-                const float yn = v - sigma * clampv(v / sigma, pk.x, pk.y);
-                if (halp) {
-                    d.yavg32[i] = yn;
-                    d.y32[i] = wf * (2.0f * yn - yo) + (1.0f - wf) * ya;
-                } else {
-                    d.y32[i] = yn;
-                    d.yavg32[i] = ya + (yn - ya) / kf;
-                }
-            }
+    const double* ysrc = halp ? d.ylast : d.yavg;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int i = base + (L.lane >> L.lg);
+        const bool own = i < tl.r1;
+        int p0 = 0, p1 = 0;
+        double rn = 1, bl = 0, bh = 0, yo = 0, ya = 0;
+        if (own) {
+            p0 = d.rp[i];
+            p1 = d.rp[i + 1];
+            rn = d.rnorm[i];
+            bl = d.blo[i];
+            bh = d.bhi[i];
+            yo = d.y[i];
+            ya = ysrc[i];
         }
-    } else {
-        const double* ysrc = halp ? d.ylast : d.yavg;
-        for (int base = tl.r0; base < tl.r1; base += L.step) {
-            const int i = base + (L.lane >> L.lg);
-            const bool own = i < tl.r1;
-            int p0 = 0, p1 = 0;
-            double rn = 1, bl = 0, bh = 0, yo = 0, ya = 0;
-            if (own) {
-                p0 = d.rp[i];
-                p1 = d.rp[i + 1];
-                rn = d.rnorm[i];
-                bl = d.blo[i];
-                bh = d.bhi[i];
-                yo = d.y[i];
-                ya = ysrc[i];
+        const double ax = f32 ? (double)row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.xbar32)
+                              : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.xbar);
+        if (own && L.gl == 0) {
+            const double sigma = (eta * omega) / rn;
+            const double v = yo + sigma * ax;
+            const double yn = v - sigma * clampv(v / sigma, bl, bh);
+            double ynext;
+            if (halp) {
+                d.yavg[i] = yn;
+                ynext = w * (2.0 * yn - yo) + (1.0 - w) * ya;
+            } else {
+                ynext = yn;
+                d.yavg[i] = ya + (yn - ya) / kavg;
             }
-            const double ax = row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.xbar);
-            if (own && L.gl == 0) {
-                const double sigma = (eta * omega) / rn;
-                const double v = yo + sigma * ax;
-                const double yn = v - sigma * clampv(v / sigma, bl, bh);
-                if (halp) {
-                    d.yavg[i] = yn;
-                    d.y[i] = w * (2.0 * yn - yo) + (1.0 - w) * ya;
-                } else {
-                    d.y[i] = yn;
-                    d.yavg[i] = ya + (yn - ya) / kavg;
-                }
-            }
+            d.y[i] = ynext;
+            if (f32) d.y32[i] = (float)ynext;
         }
     }
 }
@@ -276,9 +218,6 @@ __device__ void adapt_decide(InstState& s, double dxn, double adxn) {
 }
 
 // One workgroup per instance: the tile partials are summed by the whole
-// block (fixed strided assignment + fixed-order block reduction, so the result
-// is deterministic); EXACT sums the vectors serially in the reference's order.
-// One workgroup per instance: the tile partials are summed by the whole
 // block (fixed assignment of partials to threads, four independent loads per
 // thread, fixed-order block reduction -- deterministic); EXACT sums the
 // vectors serially in the reference's order.
@@ -328,7 +267,7 @@ __global__ __launch_bounds__(kFinal) void k_adapt_finalize(Dev d) {
 // FP64 residual terms of a candidate iterate on the unscaled model
 // (pdlp.cpp:76-131).  CUR = false: the running average (the reference's only
 // candidate); CUR = true: the current iterate (KKT restart scheme).  The
-// iterate is read in the instance's precision, the matrix in FP64.
+// iterate and the matrix are read in FP64 in both modes.
 // Rows: A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
 template <bool EXACT, bool CUR>
 __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
@@ -338,11 +277,8 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
     const InstState& s = d.st[tl.inst];
     if (s.status != kRunning) return;
     const Lanes L = lanes_of(tl);
-    const bool f32 = !EXACT && s.mode == kModeFp32;
     const double* x64 = CUR ? d.x : d.xavg;
     const double* y64 = CUR ? d.y : d.yavg;
-    const float* x32 = CUR ? d.x32 : d.xavg32;
-    const float* y32 = CUR ? d.y32 : d.yavg32;
     double viol = 0.0, dob = 0.0, mv = 0.0;
     for (int base = tl.r0; base < tl.r1; base += L.step) {
         const int i = base + (L.lane >> L.lg);
@@ -355,11 +291,10 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
             rs = d.row_scale[i];
             lo = d.orow_lo[i];
             hi = d.orow_hi[i];
-            yc = f32 ? (double)y32[i] : y64[i];
-            yl = f32 ? (double)d.ylast32[i] : d.ylast[i];
+            yc = y64[i];
+            yl = d.ylast[i];
         }
-        const double ax = f32 ? row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, x32)
-                              : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, x64);
+        const double ax = row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, x64);
         if (own && L.gl == 0) {
             if (EXACT) {
                 (CUR ? d.ax2 : d.ax)[i] = ax;
@@ -402,11 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
     const InstState& s = d.st[tl.inst];
     if (s.status != kRunning) return;
     const Lanes L = lanes_of(tl);
-    const bool f32 = !EXACT && s.mode == kModeFp32;
     const double* x64 = CUR ? d.x : d.xavg;
     const double* y64 = CUR ? d.y : d.yavg;
-    const float* x32 = CUR ? d.x32 : d.xavg32;
-    const float* y32 = CUR ? d.y32 : d.yavg32;
     const double sense = d.sense[tl.inst];
     double viol = 0.0, dres = 0.0, pob = 0.0, dob = 0.0, mv = 0.0;
     for (int base = tl.r0; base < tl.r1; base += L.step) {
@@ -421,11 +353,10 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
             lo = d.ovar_lo[j];
             hi = d.ovar_hi[j];
             ob = d.obj[j];
-            xc = f32 ? (double)x32[j] : x64[j];
-            xl = f32 ? (double)d.xlast32[j] : d.xlast[j];
+            xc = x64[j];
+            xl = d.xlast[j];
         }
-        const double aty = f32 ? row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, y32)
-                               : row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, y64);
+        const double aty = row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, y64);
         if (own && L.gl == 0) {
             if (EXACT) {
                 (CUR ? d.aty2 : d.aty)[j] = aty;
@@ -694,39 +625,28 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
     }
 }
 
-// Restart to the candidate (pdlp.cpp:257; KKT scheme: the current iterate or
-// the average, which also becomes the new x_last/y_last) and FP32 -> FP64
-// conversion of an instance's iterate.  Grid: column tiles, then row tiles.
+// Restart to the candidate (pdlp.cpp:257; KKT/Halpern schemes: the current
+// iterate or the average slot, which also becomes the new x_last/y_last).
+// The iterate is FP64 in both precision modes, so a switch to FP64 needs no
+// conversion.  Grid: column tiles, then row tiles.
 __global__ __launch_bounds__(kBlock) void k_post_check(Dev d, int kkt) {
     const int g = blockIdx.x * kWaves + (threadIdx.x >> 6);  // one wave per tile
     if (g >= d.ntT + d.ntA) return;
     const bool cols = g < d.ntT;
     const Tile tl = cols ? d.tT[g] : d.tA[g - d.ntT];
     const InstState& s = d.st[tl.inst];
-    if (!(s.restart_flag || s.switch_flag)) return;
-    const bool r32 = s.mode_at_check == kModeFp32;
+    if (!s.restart_flag) return;
     const bool to_avg = !s.cand_cur;
+    // FP32 gathers read y32: refresh it after the restart moved y.  (x_bar32
+    // is rewritten by the next primal step before it is gathered.)
+    const bool refresh32 = !cols && s.mode == kModeFp32;
     double* v64 = cols ? d.x : d.y;
-    double* a64 = cols ? d.xavg : d.yavg;
+    const double* a64 = cols ? d.xavg : d.yavg;
     double* l64 = cols ? d.xlast : d.ylast;
-    float* v32 = cols ? d.x32 : d.y32;
-    float* a32 = cols ? d.xavg32 : d.yavg32;
-    float* l32 = cols ? d.xlast32 : d.ylast32;
     for (int k = tl.r0 + (int)(threadIdx.x & 63); k < tl.r1; k += 64) {
-        if (s.restart_flag) {
-            if (r32) {
-                if (to_avg) v32[k] = a32[k];
-                if (kkt) l32[k] = v32[k];
-            } else {
-                if (to_avg) v64[k] = a64[k];
-                if (kkt) l64[k] = v64[k];
-            }
-        }
-        if (s.switch_flag) {
-            v64[k] = (double)v32[k];
-            a64[k] = (double)a32[k];
-            l64[k] = (double)l32[k];
-        }
+        if (to_avg) v64[k] = a64[k];
+        if (kkt) l64[k] = v64[k];
+        if (refresh32) d.y32[k] = (float)v64[k];
     }
 }
 
@@ -869,8 +789,6 @@ __global__ __launch_bounds__(kBlock) void k_setup_rows(Dev d) {
         for (int p = p0; p < p1; ++p) nrm += fabs(d.av64[p]);
         if (nrm < 1e-12) nrm = 1.0;
         d.rnorm[i] = nrm;
-        d.rowpk[i] = make_float4(flo ? (float)bl : -INFINITY, fhi ? (float)bh : INFINITY, (float)(1.0 / nrm),
-                                 0.0f);
     }
     bn_max = wave_max(bn_max);
     if (L.lane == 0) atomic_max_nonneg(&d.st[tl.inst].b_norm_inf, bn_max);
@@ -905,8 +823,6 @@ __global__ __launch_bounds__(kBlock) void k_setup_cols(Dev d) {
         for (int p = p0; p < p1; ++p) nrm += fabs(d.tv64[p]);
         if (nrm < 1e-12) nrm = 1.0;
         d.cnorm[j] = nrm;
-        d.colpk[j] = make_float4((float)cj, flo ? (float)sl : -INFINITY, fhi ? (float)sh : INFINITY,
-                                 (float)(1.0 / nrm));
     }
     cmax = wave_max(cmax);
     cs_max = wave_max(cs_max);
@@ -925,20 +841,15 @@ __global__ void k_init_iterates(Dev d, const double* warm_x, const double* warm_
         d.x[k] = x0;
         d.xbar[k] = x0;
         d.xavg[k] = x0;
-        d.x32[k] = (float)x0;
         d.xbar32[k] = (float)x0;
-        d.xavg32[k] = (float)x0;
         d.xlast[k] = x0;
-        d.xlast32[k] = (float)x0;
     } else if (k < d.n + d.m) {
         const int i = k - d.n;
         const double y0 = warm_y ? warm_y[i] / d.row_scale[i] : 0.0;
         d.y[i] = y0;
         d.yavg[i] = y0;
         d.y32[i] = (float)y0;
-        d.yavg32[i] = (float)y0;
         d.ylast[i] = y0;
-        d.ylast32[i] = (float)y0;
     }
 }
 
@@ -973,17 +884,11 @@ __global__ void k_unscale(Dev d, double* xo, double* yo) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < d.n) {
         const InstState& s = d.st[d.col_inst[k]];
-        const bool f32 = s.mode == kModeFp32;
-        const double v = s.answer_cur ? (f32 ? (double)d.x32[k] : d.x[k])
-                                      : (f32 ? (double)d.xavg32[k] : d.xavg[k]);
-        xo[k] = v * d.col_scale[k];
+        xo[k] = (s.answer_cur ? d.x[k] : d.xavg[k]) * d.col_scale[k];
     } else if (k < d.n + d.m) {
         const int i = k - d.n;
         const InstState& s = d.st[d.row_inst[i]];
-        const bool f32 = s.mode == kModeFp32;
-        const double v = s.answer_cur ? (f32 ? (double)d.y32[i] : d.y[i])
-                                      : (f32 ? (double)d.yavg32[i] : d.yavg[i]);
-        yo[i] = v * d.row_scale[i];
+        yo[i] = (s.answer_cur ? d.y[i] : d.yavg[i]) * d.row_scale[i];
     }
 }
 
