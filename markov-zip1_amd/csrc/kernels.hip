@@ -501,7 +501,10 @@ __device__ __forceinline__ void stop_instance(const Dev& d, InstState& s, int32_
     s.status = status;
     s.done_iter = it;
     atomicSub(&d.counters[0], 1);
-    if (s.mode == kModeFp32) atomicSub(&d.counters[1], 1);
+    if (s.mode == kModeFp32) {  // finished without leaving the FP32-SpMV phase
+        s.fp32_iters = it;
+        atomicSub(&d.counters[1], 1);
+    }
 }
 
 // Termination and restart decision of one instance (pdlp.cpp:229-260; the

@@ -254,3 +254,21 @@ def test_netlib_fast_order_reaches_published_optimum(gpu, name, precision, schem
     assert abs(r["objective"] - NETLIB[name]) <= 1e-6 * abs(NETLIB[name])
     k = orc.kkt(lp, r["primal"], r["dual"])
     assert k["score"] <= 1e-6 * 1.01
+
+
+def test_repeated_solves_follow_their_own_options(gpu):
+    """Solving one LP again on the same engine with other options (tolerance,
+    FP32 stall window) must use those options, not the ones the cached HIP
+    graphs were captured with; a repeat of the first solve is bit-identical."""
+    lp = small_uniform(seed=21)
+    kw = dict(restart_scheme=mp.SCHEME_HALPERN, max_iterations=200000)
+    a = gpu_solve(lp, tolerance=1e-3, **kw)
+    b = gpu_solve(lp, tolerance=1e-6, **kw)
+    c = gpu_solve(lp, tolerance=1e-3, **kw)
+    assert a["status"] == b["status"] == c["status"] == "optimal"
+    assert b["iterations"] > a["iterations"]
+    assert orc.kkt(lp, b["primal"], b["dual"])["score"] <= 1.5e-6
+    assert c["iterations"] == a["iterations"] and np.array_equal(c["primal"], a["primal"])
+    m1 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1, **kw)
+    m2 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1000000, **kw)
+    assert m1["fp32_iterations"] < m2["fp32_iterations"]
