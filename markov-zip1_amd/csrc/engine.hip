@@ -59,6 +59,14 @@ int guarded(F&& f) {
     }
 }
 
+// Tile policies: throughput shape (long tasks, few waves) for problems that
+// fill the chip, latency shape (one short sweep per wave, many waves) for
+// problems of up to kLatencyNnz nonzeros, where a launch is a handful of
+// dependent memory round trips rather than a stream.
+constexpr mcp_tile_policy kThroughputPolicy{1024, 8};
+constexpr mcp_tile_policy kLatencyPolicy{128, 2};
+constexpr int64_t kLatencyNnz = 2 * 1024 * 1024;
+
 int lanes_log2(int len, int per_lane) {
     int lg = 0;
     while (lg < 6 && len > (per_lane << lg)) ++lg;
@@ -217,13 +225,13 @@ struct mcp_engine {
     // Time the plain SpMV of one orientation under each candidate policy and
     // keep the fastest (FP32 values: the mixed-precision iteration's shape).
     // Each candidate's time is the best of three runs of five launches, and a
-    // candidate replaces the incumbent (the default policy, timed first) only
-    // when it is at least 2 % faster, so run-to-run noise does not flip the
-    // choice between nearly equal shapes.
+    // candidate replaces the incumbent (the size-based default policy, timed
+    // first) only when it is at least 2 % faster, so run-to-run noise does not
+    // flip the choice between nearly equal shapes.
     void autotune_one(bool rows) {
-        static const mcp_tile_policy cands[] = {{1024, 8}, {256, 4},  {256, 8},   {256, 16},  {256, 32},
-                                                {1024, 4}, {1024, 16}, {1024, 32}, {4096, 4}, {4096, 8},
-                                                {4096, 16}, {4096, 32}};
+        const mcp_tile_policy cands[] = {rows ? polA : polT, {128, 2},  {256, 4},   {256, 8},  {256, 16},
+                                         {256, 32},          {1024, 4}, {1024, 8},  {1024, 16}, {1024, 32},
+                                         {4096, 4},          {4096, 8}, {4096, 16}, {4096, 32}};
         const auto& ptr = rows ? rp : cp;
         const auto& inst = rows ? inst_row : inst_col;
         const int nrows = rows ? m : n;
@@ -455,6 +463,7 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         e->loaded = true;
         e->setup_done = false;
         e->polA = e->polT = mcp_tile_policy{1024, 8};
+        e->polA = e->polT = e->nnz <= kLatencyNnz ? kLatencyPolicy : kThroughputPolicy;
         e->retile(false);  // setup kernels run on these tiles
     });
 }
@@ -581,6 +590,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         // the last (re)tiling have finished, rebuild the tiles over the running
         // instances only, so finished instances stop costing waves.
         int32_t compact_at = e->B > 1 ? e->B / 2 : -1;
+        const mcp_tile_policy polA0 = e->polA, polT0 = e->polT;
         while (iter < max_it) {
             const int len = (int)std::min<int64_t>(chunk, max_it - iter);
             if (use_graphs && len == chunk) {
@@ -628,6 +638,10 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                         finished = true;
                         break;
                     }
+                    int64_t live_nnz = 0;
+                    for (int b = 0; b < e->B; ++b)
+                        if (keep[b]) live_nnz += e->rp[e->inst_row[b + 1]] - e->rp[e->inst_row[b]];
+                    if (live_nnz <= kLatencyNnz) e->polA = e->polT = kLatencyPolicy;
                     e->retile(e->exact_tiles, &keep);
                     e->drop_graphs();
                     e->graph_dev = d;
@@ -662,6 +676,8 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         HIPCHK(hipEventRecord(e->ev[1], s));
         HIPCHK(hipStreamSynchronize(s));
         if (e->compacted) {  // outside mcp_solve the tiles always cover every instance
+            e->polA = polA0;
+            e->polT = polT0;
             e->retile(e->exact_tiles);
             e->drop_graphs();
             e->graph_dev = d;
