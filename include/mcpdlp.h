@@ -96,7 +96,9 @@ typedef struct mc_lp {
  *   the restart; the check compares T(z) (in place of the average) with z.  The
  *   primal weight at a restart is the geometric blend
  *   omega <- omega_move^(3/4) omega_res^(1/4) of the movement rule above and the
- *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)).
+ *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)),
+ *   with the exponents swapped (omega_move^(1/4) omega_res^(3/4)) while the primal and
+ *   dual residuals differ by more than 30x.
  * Termination, tolerances and the reported score are the reference's in all three. */
 #define MC_SCHEME_REFERENCE 0
 #define MC_SCHEME_KKT 1

@@ -65,7 +65,7 @@ typedef struct mcp_problem {
 
 /* Tile shape policy for the CSR-stream SpMV (autotuner knobs). */
 typedef struct mcp_tile_policy {
-    int32_t nnz_budget;     /* nonzeros staged in LDS per tile (<= 2048)        */
+    int32_t nnz_budget;     /* nonzeros per wave task (tile)                    */
     int32_t per_lane;       /* target nonzeros per lane when picking lanes/row  */
 } mcp_tile_policy;
 

@@ -594,11 +594,15 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
                     w = (th == 0.5) ? sqrt(s.omega * ratio) : pow(ratio, th) * pow(s.omega, 1.0 - th);
                     upd = true;
                 }
-                if (o.halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
+                if (o.halpern) {  // omega_move^(3/4) omega_res^(1/4), or ^(1/4) ^(3/4)
+                                  // when p/d is off by more than 30x; in square roots
                     double wres = s.omega;
-                    if (r.primal > 1e-12 && r.dual > 1e-12)
+                    bool lopsided = false;
+                    if (r.primal > 1e-12 && r.dual > 1e-12) {
                         wres = s.omega * sqrt(clampv(sqrt(r.primal / r.dual), 0.05, 20.0));
-                    w = sqrt(w * sqrt(w * wres));
+                        lopsided = r.primal > 30.0 * r.dual || r.dual > 30.0 * r.primal;
+                    }
+                    w = lopsided ? sqrt(wres * sqrt(w * wres)) : sqrt(w * sqrt(w * wres));
                     upd = true;
                 }
                 if (upd) s.omega = clampv(w, 1e-6, 1e6);

@@ -299,11 +299,15 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                     w = th == 0.5 ? std::sqrt(p.omega * ratio) : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
                     upd = true;
                 }
-                if (halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
+                if (halpern) {  // omega_move^(3/4) omega_res^(1/4), or ^(1/4) ^(3/4) when p/d
+                                // is off by more than 30x; in square roots
                     double wres = p.omega;
-                    if (cur.primal > 1e-12 && cur.dual > 1e-12)
+                    bool lopsided = false;
+                    if (cur.primal > 1e-12 && cur.dual > 1e-12) {
                         wres = p.omega * std::sqrt(std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0));
-                    w = std::sqrt(w * std::sqrt(w * wres));
+                        lopsided = cur.primal > 30.0 * cur.dual || cur.dual > 30.0 * cur.primal;
+                    }
+                    w = lopsided ? std::sqrt(wres * std::sqrt(w * wres)) : std::sqrt(w * std::sqrt(w * wres));
                     upd = true;
                 }
                 if (upd) p.omega = std::clamp(w, 1e-6, 1e6);
