@@ -96,9 +96,7 @@ typedef struct mc_lp {
  *   the restart; the check compares T(z) (in place of the average) with z.  The
  *   primal weight at a restart is the geometric blend
  *   omega <- omega_move^(3/4) omega_res^(1/4) of the movement rule above and the
- *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)),
- *   with the exponents swapped (omega_move^(1/4) omega_res^(3/4)) while the primal and
- *   dual residuals differ by more than 30x.
+ *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)).
  * Termination, tolerances and the reported score are the reference's in all three. */
 #define MC_SCHEME_REFERENCE 0
 #define MC_SCHEME_KKT 1
@@ -138,7 +136,10 @@ typedef struct mc_pdlp_options {
     const double* warm_primal;        /* optional [n] start point in the ORIGINAL space (NULL => cold x=proj(0)) */
     const double* warm_dual;          /* optional [m] start dual in the ORIGINAL space (reported sign convention) */
     int32_t fp32_stall_checks;        /* mixed: checks without a new best score before the FP64 switch (0 => 50) */
-    int32_t pad_;
+    int32_t detect_infeasibility;     /* 1: report MC_PDLP_INFEASIBLE_OR_UNBOUNDED when the iterate's movement
+                                         since the last restart is a primal or dual ray certificate (0: never,
+                                         as the reference) */
+    double infeasibility_tolerance;   /* certificate tolerance (0 => 1e-8) */
 } mc_pdlp_options;
 
 /* PdlpStatus (reference: pdlp.hpp:43). */

@@ -99,6 +99,9 @@ typedef struct mcp_solve_opts {
     int32_t fp32_stall_checks;  /* mixed: checks without a new best score before the FP64 switch; 0 => 50 */
     const double* warm_primal;  /* optional [n] host, original space */
     const double* warm_dual;    /* optional [m] host, original space (reported convention) */
+    int32_t detect_infeasibility;     /* 1: stop on a ray certificate (status infeasible_or_unbounded) */
+    int32_t pad2_;
+    double infeasibility_tolerance;   /* certificate tolerance; 0 => 1e-8 */
 } mcp_solve_opts;
 
 typedef struct mcp_solve_stats {

@@ -31,6 +31,7 @@ constexpr int32_t kRunning = 0;
 constexpr int32_t kOptimal = 1;
 constexpr int32_t kIterLimit = 2;
 constexpr int32_t kNumFail = 3;
+constexpr int32_t kInfeasible = 4;  // a primal or dual ray certificate was found
 // Arithmetic mode of an instance's iterations.
 constexpr int32_t kModeFp64 = 0;
 constexpr int32_t kModeFp32 = 1;
@@ -66,6 +67,7 @@ struct IterOpts {
     double restart_factor;
     double smoothing;
     double polish_threshold;
+    double infeas_tol;           // ray certificates: violation <= infeas_tol x |ray objective|
     int64_t restart_every;
     int64_t max_iterations;
     int32_t restart_strategy;    // 0 none, 1 adaptive, 2 fixed
@@ -75,7 +77,7 @@ struct IterOpts {
     int32_t kkt;                 // restart scheme: 0 reference, 1 KKT (Applegate et al.)
     int32_t stall_checks;        // mixed: checks without a new best before the FP64 switch
     int32_t halpern;             // Halpern scheme: blended primal weight at restarts
-    int32_t pad_;
+    int32_t detect;              // test the iterate's movement for infeasibility certificates
 };
 
 // Map (workgroup, wave) to a tile so that each XCD works on a contiguous

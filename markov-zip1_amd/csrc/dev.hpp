@@ -68,6 +68,15 @@ struct Dev {
     double* part_rows;  // [(c*ntA + tile)*3 + k]: max violation, dual-objective part, |y - ylast|^2
     double* part_cols;  // [(c*ntT + tile)*5 + k]: max violation, dual residual^2, primal obj,
                         //                          dual-objective part, |x - xlast|^2
+    // infeasibility detection: the candidate's movement since the last
+    // restart and its per-tile certificate partials
+    double* ray_x;      // [n]
+    double* ray_y;      // [m]
+    double* part_ray_rows;  // [tile*5 + k]: primal-ray row violation, dual-ray objective part and
+                            //   its magnitude, dual-ray row violation, |dy|_inf
+    double* part_ray_cols;  // [tile*7 + k]: primal-ray bound violation, c^T dx and its magnitude,
+                            //   dual-ray objective part and its magnitude, dual-ray column
+                            //   violation, |dx|_inf
     InstState* st;      // [B]
     int64_t* iter_base; // global iteration counter at the start of the current chunk
     int32_t* counters;  // [0] running instances, [1] running instances in FP32
@@ -87,6 +96,9 @@ void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s);
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
                            hipStream_t s);
 void launch_post_check(const Dev& d, bool kkt, hipStream_t s);
+// Infeasibility certificates from the candidate's movement since the last
+// restart (after the check, before the restart copy).
+void launch_ray_check(const Dev& d, const IterOpts& o, int t, hipStream_t s);
 void launch_advance(const Dev& d, int len, hipStream_t s);
 // Setup kernels (run on the current tiles).
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);

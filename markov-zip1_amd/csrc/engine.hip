@@ -204,6 +204,8 @@ struct mcp_engine {
         d.part_adx = buf<double>("part_adx", tA.size());
         d.part_rows = buf<double>("part_rows", 6 * tA.size());
         d.part_cols = buf<double>("part_cols", 10 * tT.size());
+        d.part_ray_rows = buf<double>("part_ray_rows", 5 * tA.size());
+        d.part_ray_cols = buf<double>("part_ray_cols", 7 * tT.size());
         d.tA = dA;
         d.tT = dT;
         d.tA_begin = bA;
@@ -288,6 +290,8 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.kkt = (o->restart_scheme == MC_SCHEME_KKT || o->restart_scheme == MC_SCHEME_HALPERN) ? 1 : 0;
     io.halpern = o->restart_scheme == MC_SCHEME_HALPERN ? 1 : 0;
     io.stall_checks = o->fp32_stall_checks > 0 ? o->fp32_stall_checks : 50;
+    io.detect = o->detect_infeasibility != 0 ? 1 : 0;
+    io.infeas_tol = o->infeasibility_tolerance > 0.0 ? o->infeasibility_tolerance : 1e-8;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
     // 0 => no score threshold: FP32 runs until it converges or stalls
     io.polish_threshold = o->polish_threshold > 0.0 ? o->polish_threshold : 0.0;
@@ -312,6 +316,7 @@ void enqueue_chunk(mcp_engine* e, const mcp::IterOpts& io, bool adaptive, bool e
         if ((it + 1) % io.restart_every == 0) {
             for (int cur = 0; cur <= io.kkt; ++cur) mcp::launch_resid(d, exact, cur != 0, s);
             mcp::launch_check_finalize(d, io, t, exact, false, s);
+            if (io.detect) mcp::launch_ray_check(d, io, t, s);
             mcp::launch_post_check(d, io.kkt != 0, s);
         }
     }
@@ -451,6 +456,8 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         d.y32 = e->buf<float>("y32", m);
         d.xlast = e->buf<double>("xlast", n);
         d.ylast = e->buf<double>("ylast", m);
+        d.ray_x = e->buf<double>("ray_x", n);
+        d.ray_y = e->buf<double>("ray_y", m);
         d.ax = e->buf<double>("ax", m);
         d.aty = e->buf<double>("aty", n);
         d.ax2 = e->buf<double>("ax2", m);
@@ -532,6 +539,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         const bool mixed = o->precision == MC_PRECISION_MIXED;
         require(!(exact && mixed), "reference reduction order requires FP64 precision");
         require(!exact || e->exact_tiles, "reference order needs mcp_setup(exact=1)");
+        require(!(exact && o->detect_infeasibility), "infeasibility detection runs in the fast reduction order");
         HIPCHK(hipSetDevice(e->device));
         auto& d = e->d;
         hipStream_t s = e->stream;
@@ -714,9 +722,10 @@ int mcp_fetch(mcp_engine* e, double* x, double* y, mcp_inst_result* res, double*
             for (int b = 0; b < e->B; ++b) {
                 const auto& q = st[b];
                 auto& r = res[b];
-                r.status = q.status == mcp::kOptimal    ? MC_PDLP_OPTIMAL
-                           : q.status == mcp::kNumFail  ? MC_PDLP_NUMERICAL_FAILURE
-                                                        : MC_PDLP_ITERATION_LIMIT;
+                r.status = q.status == mcp::kOptimal       ? MC_PDLP_OPTIMAL
+                           : q.status == mcp::kNumFail     ? MC_PDLP_NUMERICAL_FAILURE
+                           : q.status == mcp::kInfeasible ? MC_PDLP_INFEASIBLE_OR_UNBOUNDED
+                                                           : MC_PDLP_ITERATION_LIMIT;
                 r.mode = q.mode;
                 r.iterations = (uint64_t)q.done_iter;
                 r.fp32_iterations = (uint64_t)q.fp32_iters;

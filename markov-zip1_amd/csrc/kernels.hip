@@ -594,15 +594,11 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
                     w = (th == 0.5) ? sqrt(s.omega * ratio) : pow(ratio, th) * pow(s.omega, 1.0 - th);
                     upd = true;
                 }
-                if (o.halpern) {  // omega_move^(3/4) omega_res^(1/4), or ^(1/4) ^(3/4)
-                                  // when p/d is off by more than 30x; in square roots
+                if (o.halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
                     double wres = s.omega;
-                    bool lopsided = false;
-                    if (r.primal > 1e-12 && r.dual > 1e-12) {
+                    if (r.primal > 1e-12 && r.dual > 1e-12)
                         wres = s.omega * sqrt(clampv(sqrt(r.primal / r.dual), 0.05, 20.0));
-                        lopsided = r.primal > 30.0 * r.dual || r.dual > 30.0 * r.primal;
-                    }
-                    w = lopsided ? sqrt(wres * sqrt(w * wres)) : sqrt(w * sqrt(w * wres));
+                    w = sqrt(w * sqrt(w * wres));
                     upd = true;
                 }
                 if (upd) s.omega = clampv(w, 1e-6, 1e6);
@@ -655,6 +651,172 @@ __global__ __launch_bounds__(kBlock) void k_post_check(Dev d, int kkt) {
         if (kkt) l64[k] = v64[k];
         if (refresh32) d.y32[k] = (float)v64[k];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Infeasibility detection (extension; no reference counterpart: the
+// reference declares infeasible_or_unbounded but never reports it).  A
+// diverging PDHG iterate moves along a certificate ray (Applegate et al.
+// 2021, sec. 5).  At a check, the candidate's movement since the last
+// restart, (dx, dy), is tested in the unscaled space as
+//  * a dual ray certificate of primal infeasibility: the homogeneous dual
+//    objective D(dy) = sum_i -support_[lo_i,hi_i](dy_i) + sum_j (g_j > 0 ?
+//    g_j lo_j : g_j hi_j), g = A^T dy, is positive beyond rounding (more than
+//    1e-9 of the sum of its terms' magnitudes: for a feasible LP D <= 0 by
+//    weak duality), and the parts that need an infinite bound (|dy_i| or |g_j|
+//    toward it) sum to at most tol x D;
+//  * a primal ray certificate of dual infeasibility: c^T dx < 0 beyond
+//    rounding, and the recession violations of A dx against the row bounds
+//    and of dx against the variable bounds are at most tol x |c^T dx|.
+// Either stops the instance with status infeasible_or_unbounded.  A false
+// positive needs an optimal dual (primal) of norm above 1/tol.
+__global__ void k_ray_prep(Dev d) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < d.n) {
+        const InstState& s = d.st[d.col_inst[k]];
+        if (s.status != kRunning) return;
+        d.ray_x[k] = (s.cand_cur ? d.x[k] : d.xavg[k]) - d.xlast[k];
+    } else if (k < d.n + d.m) {
+        const int i = k - d.n;
+        const InstState& s = d.st[d.row_inst[i]];
+        if (s.status != kRunning) return;
+        d.ray_y[i] = (s.cand_cur ? d.y[i] : d.yavg[i]) - d.ylast[i];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ray_rows(Dev d) {
+    const int tile = wave_tile(d.ntA);
+    if (tile >= d.ntA) return;
+    const Tile tl = d.tA[tile];
+    if (d.st[tl.inst].status != kRunning) return;
+    const Lanes L = lanes_of(tl);
+    double vp = 0.0, dob = 0.0, dab = 0.0, vd = 0.0, ny = 0.0;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int i = base + (L.lane >> L.lg);
+        const bool own = i < tl.r1;
+        int p0 = 0, p1 = 0;
+        if (own) {
+            p0 = d.rp[i];
+            p1 = d.rp[i + 1];
+        }
+        const double a = row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.ray_x);
+        if (own && L.gl == 0) {
+            const double rs = d.row_scale[i];
+            const double lo = d.orow_lo[i], hi = d.orow_hi[i];
+            const double u = a / rs;  // (A dx)_i, unscaled
+            if (lo > -INFINITY && u < 0.0) vp = maxv(vp, -u);
+            if (hi < INFINITY && u > 0.0) vp = maxv(vp, u);
+            const double yu = d.ray_y[i] * rs;
+            double term = 0.0;
+            if (yu > 0.0) {
+                if (hi < INFINITY) term = -(yu * hi); else vd += yu;
+            } else if (yu < 0.0) {
+                if (lo > -INFINITY) term = -(yu * lo); else vd -= yu;
+            }
+            dob += term;
+            dab += fabs(term);
+            ny = maxv(ny, fabs(yu));
+        }
+    }
+    vp = wave_max(vp);
+    dob = wave_sum(dob);
+    dab = wave_sum(dab);
+    vd = wave_sum(vd);
+    ny = wave_max(ny);
+    if (L.lane == 0) {
+        double* out = d.part_ray_rows + 5 * tile;
+        out[0] = vp;
+        out[1] = dob;
+        out[2] = dab;
+        out[3] = vd;
+        out[4] = ny;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ray_cols(Dev d) {
+    const int tile = wave_tile(d.ntT);
+    if (tile >= d.ntT) return;
+    const Tile tl = d.tT[tile];
+    if (d.st[tl.inst].status != kRunning) return;
+    const Lanes L = lanes_of(tl);
+    const double sense = d.sense[tl.inst];
+    double vp = 0.0, cdx = 0.0, cab = 0.0, dob = 0.0, dab = 0.0, vd = 0.0, nx = 0.0;
+    for (int base = tl.r0; base < tl.r1; base += L.step) {
+        const int j = base + (L.lane >> L.lg);
+        const bool own = j < tl.r1;
+        int p0 = 0, p1 = 0;
+        if (own) {
+            p0 = d.cp[j];
+            p1 = d.cp[j + 1];
+        }
+        const double a = row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.ray_y);
+        if (own && L.gl == 0) {
+            const double cs = d.col_scale[j];
+            const double lo = d.ovar_lo[j], hi = d.ovar_hi[j];
+            const double g = a / cs;  // (A^T dy)_j, unscaled
+            double term = 0.0;
+            if (g > 0.0) {
+                if (lo > -INFINITY) term = g * lo; else vd += g;
+            } else if (g < 0.0) {
+                if (hi < INFINITY) term = g * hi; else vd -= g;
+            }
+            dob += term;
+            dab += fabs(term);
+            const double xu = d.ray_x[j] * cs;
+            if (lo > -INFINITY && xu < 0.0) vp = maxv(vp, -xu);
+            if (hi < INFINITY && xu > 0.0) vp = maxv(vp, xu);
+            const double cterm = sense * d.obj[j] * xu;
+            cdx += cterm;
+            cab += fabs(cterm);
+            nx = maxv(nx, fabs(xu));
+        }
+    }
+    vp = wave_max(vp);
+    cdx = wave_sum(cdx);
+    cab = wave_sum(cab);
+    dob = wave_sum(dob);
+    dab = wave_sum(dab);
+    vd = wave_sum(vd);
+    nx = wave_max(nx);
+    if (L.lane == 0) {
+        double* out = d.part_ray_cols + 7 * tile;
+        out[0] = vp;
+        out[1] = cdx;
+        out[2] = cab;
+        out[3] = dob;
+        out[4] = dab;
+        out[5] = vd;
+        out[6] = nx;
+    }
+}
+
+__global__ __launch_bounds__(kFinal) void k_ray_finalize(Dev d, IterOpts o, int t) {
+    const int b = blockIdx.x;
+    InstState& s = d.st[b];
+    __shared__ double red[kFinal / 64];
+    __shared__ int running;
+    if (threadIdx.x == 0) running = s.status == kRunning;
+    __syncthreads();
+    if (!running) return;
+    const double* pr = d.part_ray_rows;
+    const double* pc = d.part_ray_cols;
+    const int a0 = d.tA_begin[b], a1 = d.tA_begin[b + 1], t0 = d.tT_begin[b], t1 = d.tT_begin[b + 1];
+    constexpr int NW = kFinal / 64;
+    const double vp = block_max<NW>(maxv(max_strided(pr, a0, a1, 5), max_strided(pc, t0, t1, 7)), red);
+    const double cdx = block_sum<NW>(sum_strided(pc + 1, t0, t1, 7), red);
+    const double cab = block_sum<NW>(sum_strided(pc + 2, t0, t1, 7), red);
+    const double dob = block_sum<NW>(sum_strided(pr + 1, a0, a1, 5), red) +
+                       block_sum<NW>(sum_strided(pc + 3, t0, t1, 7), red);
+    const double dab = block_sum<NW>(sum_strided(pr + 2, a0, a1, 5), red) +
+                       block_sum<NW>(sum_strided(pc + 4, t0, t1, 7), red);
+    const double vd = block_sum<NW>(sum_strided(pr + 3, a0, a1, 5), red) +
+                      block_sum<NW>(sum_strided(pc + 5, t0, t1, 7), red);
+    const double ny = block_max<NW>(max_strided(pr + 4, a0, a1, 5), red);
+    const double nx = block_max<NW>(max_strided(pc + 6, t0, t1, 7), red);
+    if (threadIdx.x != 0) return;
+    const bool primal_infeasible = ny > 0.0 && dob > 1e-9 * dab && vd <= o.infeas_tol * dob;
+    const bool dual_infeasible = nx > 0.0 && cdx < -1e-9 * cab && vp <= o.infeas_tol * -cdx;
+    if (primal_infeasible || dual_infeasible) stop_instance(d, s, kInfeasible, *d.iter_base + t + 1);
 }
 
 __global__ void k_advance(Dev d, int len) {
@@ -1009,6 +1171,15 @@ void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, b
         else
             hipLaunchKernelGGL((k_check_finalize<false, false>), g, b, 0, s, d, o, t);
     }
+}
+
+void launch_ray_check(const Dev& d, const IterOpts& o, int t, hipStream_t s) {
+    const int64_t tot = (int64_t)d.m + d.n;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_ray_prep, dim3(elem_grid(tot)), dim3(kBlock), 0, s, d);
+    if (d.ntA > 0) hipLaunchKernelGGL(k_ray_rows, dim3(tile_grid(d.ntA)), dim3(kBlock), 0, s, d);
+    if (d.ntT > 0) hipLaunchKernelGGL(k_ray_cols, dim3(tile_grid(d.ntT)), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_ray_finalize, dim3(d.B), dim3(kFinal), 0, s, d, o, t);
 }
 
 void launch_post_check(const Dev& d, bool kkt, hipStream_t s) {

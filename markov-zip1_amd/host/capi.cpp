@@ -63,6 +63,8 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
                                                                : fo::RestartScheme::reference;
     p.polish_threshold = o.polish_threshold;
     p.fp32_stall_checks = o.fp32_stall_checks;
+    p.detect_infeasibility = o.detect_infeasibility != 0;
+    p.infeasibility_tolerance = o.infeasibility_tolerance > 0.0 ? o.infeasibility_tolerance : 1e-8;
     p.graph_chunk = o.graph_chunk;
     p.device = o.device;
     p.require_gpu = o.require_gpu != 0;

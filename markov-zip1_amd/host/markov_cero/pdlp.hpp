@@ -50,6 +50,11 @@ struct PdlpOptions {
     ReductionOrder reduction_order{ReductionOrder::fast};
     double polish_threshold{0.0};   // mixed: FP64 from this checked score (0 => only when FP32 stalls)
     int fp32_stall_checks{0};       // mixed: checks without a new best before FP64 (0 => 50)
+    // Report infeasible_or_unbounded when the iterate's movement since the
+    // last restart is a primal or dual ray certificate (off: the reference
+    // never reports that status).
+    bool detect_infeasibility{false};
+    double infeasibility_tolerance{1e-8};
     int graph_chunk{0};             // iterations per captured HIP graph (0 => 64, <0 => eager launches)
     int device{0};
     bool require_gpu{false};        // true: fail instead of the cpu_fallback path

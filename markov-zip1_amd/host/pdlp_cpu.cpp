@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 
 #include "markov_cero/pdlp.hpp"
 #include "markov_cero/scale.hpp"
@@ -112,6 +113,67 @@ Residuals evaluate(const model::Model& orig, const std::vector<double>& xs, cons
 
 }  // namespace
 
+// Infeasibility certificate test (extension) on the scaled movement (dx, dy)
+// of the candidate since the last restart, in the unscaled space: a dual ray
+// with positive homogeneous dual objective (primal infeasible) or a primal
+// ray with negative objective inside the recession cones (dual infeasible),
+// violations at most tol x |ray objective|, objective beyond 1e-9 of its terms'
+// magnitudes.  Same tests as the device engine's k_ray_* kernels.
+bool ray_certificate(const model::Model& orig, const Pdhg& p, const std::vector<double>& dx,
+                     const std::vector<double>& dy, const scale::RuizScalers& sc, bool ruiz, double tol) {
+    const std::size_t m = p.m, n = p.n;
+    const double sense = orig.objective_sense == model::ObjectiveSense::maximize ? -1.0 : 1.0;
+    std::vector<double> adx(m), atdy(n);
+    p.a_times(dx, adx);
+    p.at_times(dy, atdy);
+    double vp = 0.0, cdx = 0.0, cab = 0.0, dob = 0.0, dab = 0.0, vd = 0.0, nx = 0.0, ny = 0.0;
+    const double inf = std::numeric_limits<double>::infinity();
+    for (std::size_t i = 0; i < m; ++i) {
+        const double rs = ruiz ? sc.row_scale[i] : 1.0;
+        const double lo = orig.row_lower[i].kind == model::BoundKind::finite ? orig.row_lower[i].value : -inf;
+        const double hi = orig.row_upper[i].kind == model::BoundKind::finite ? orig.row_upper[i].value : inf;
+        const double u = adx[i] / rs;
+        if (lo > -inf && u < 0.0) vp = std::max(vp, -u);
+        if (hi < inf && u > 0.0) vp = std::max(vp, u);
+        const double yu = dy[i] * rs;
+        double term = 0.0;
+        if (yu > 0.0) {
+            if (hi < inf) term = -(yu * hi); else vd += yu;
+        } else if (yu < 0.0) {
+            if (lo > -inf) term = -(yu * lo); else vd -= yu;
+        }
+        dob += term;
+        dab += std::abs(term);
+        ny = std::max(ny, std::abs(yu));
+    }
+    for (std::size_t j = 0; j < n; ++j) {
+        const double cs = ruiz ? sc.col_scale[j] : 1.0;
+        const double lo =
+            orig.variable_lower[j].kind == model::BoundKind::finite ? orig.variable_lower[j].value : -inf;
+        const double hi =
+            orig.variable_upper[j].kind == model::BoundKind::finite ? orig.variable_upper[j].value : inf;
+        const double g = atdy[j] / cs;
+        double term = 0.0;
+        if (g > 0.0) {
+            if (lo > -inf) term = g * lo; else vd += g;
+        } else if (g < 0.0) {
+            if (hi < inf) term = g * hi; else vd -= g;
+        }
+        dob += term;
+        dab += std::abs(term);
+        const double xu = dx[j] * cs;
+        if (lo > -inf && xu < 0.0) vp = std::max(vp, -xu);
+        if (hi < inf && xu > 0.0) vp = std::max(vp, xu);
+        const double cterm = sense * orig.objective[j] * xu;
+        cdx += cterm;
+        cab += std::abs(cterm);
+        nx = std::max(nx, std::abs(xu));
+    }
+    const bool primal_infeasible = ny > 0.0 && dob > 1e-9 * dab && vd <= tol * dob;
+    const bool dual_infeasible = nx > 0.0 && cdx < -1e-9 * cab && vp <= tol * -cdx;
+    return primal_infeasible || dual_infeasible;
+}
+
 PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const char* backend_label) {
     const auto t0 = std::chrono::steady_clock::now();
     const std::size_t m = orig.matrix.row_count, n = orig.matrix.column_count;
@@ -186,7 +248,7 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     std::size_t iter = 0, k = 0, since = 0;
     double last_score = 1e300, prev_cand = 1e300;
     Residuals cur;
-    bool converged = false, use_cur = false;
+    bool converged = false, use_cur = false, infeasible = false;
     // Average's residuals; for the KKT scheme also the current iterate's, and
     // the better of the two becomes the candidate (cur, xu, yu).
     auto evaluate_candidates = [&] {
@@ -263,6 +325,17 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
             converged = true;
             break;
         }
+        if (opt.detect_infeasibility) {
+            const auto& xc = use_cur ? p.x : p.xavg;
+            const auto& yc = use_cur ? p.y : p.yavg;
+            std::vector<double> rdx(n), rdy(m);
+            for (std::size_t j = 0; j < n; ++j) rdx[j] = xc[j] - xlast[j];
+            for (std::size_t i = 0; i < m; ++i) rdy[i] = yc[i] - ylast[i];
+            if (ray_certificate(orig, p, rdx, rdy, sc, opt.ruiz_scaling, opt.infeasibility_tolerance)) {
+                infeasible = true;
+                break;
+            }
+        }
         bool restart;
         if (!kkt) {
             restart = opt.restart_strategy == RestartStrategy::fixed ||
@@ -299,15 +372,11 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                     w = th == 0.5 ? std::sqrt(p.omega * ratio) : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
                     upd = true;
                 }
-                if (halpern) {  // omega_move^(3/4) omega_res^(1/4), or ^(1/4) ^(3/4) when p/d
-                                // is off by more than 30x; in square roots
+                if (halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
                     double wres = p.omega;
-                    bool lopsided = false;
-                    if (cur.primal > 1e-12 && cur.dual > 1e-12) {
+                    if (cur.primal > 1e-12 && cur.dual > 1e-12)
                         wres = p.omega * std::sqrt(std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0));
-                        lopsided = cur.primal > 30.0 * cur.dual || cur.dual > 30.0 * cur.primal;
-                    }
-                    w = lopsided ? std::sqrt(wres * std::sqrt(w * wres)) : std::sqrt(w * std::sqrt(w * wres));
+                    w = std::sqrt(w * std::sqrt(w * wres));
                     upd = true;
                 }
                 if (upd) p.omega = std::clamp(w, 1e-6, 1e6);
@@ -325,10 +394,12 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
         since = 0;
         last_score = cur.score;
     }
-    if (!converged) evaluate_candidates();
+    if (!converged && !infeasible) evaluate_candidates();
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    res.status = converged ? PdlpStatus::optimal : PdlpStatus::iteration_limit;
-    res.message = converged ? "PDLP converged" : "iteration limit reached";
+    res.status = converged ? PdlpStatus::optimal
+                 : infeasible ? PdlpStatus::infeasible_or_unbounded
+                              : PdlpStatus::iteration_limit;
+    res.message = converged ? "PDLP converged" : infeasible ? "infeasibility certificate found" : "iteration limit reached";
     res.primal = std::move(xu);
     res.dual = std::move(yu);
     res.objective = cur.objective;

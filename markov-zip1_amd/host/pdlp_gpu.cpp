@@ -335,6 +335,8 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.primal_weight_smoothing = opt.primal_weight_smoothing;
     o.polish_threshold = opt.polish_threshold;
     o.fp32_stall_checks = opt.fp32_stall_checks;
+    o.detect_infeasibility = opt.detect_infeasibility ? 1 : 0;
+    o.infeasibility_tolerance = opt.infeasibility_tolerance;
     o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
     o.graph_chunk = opt.graph_chunk;
     o.restart_scheme = opt.restart_scheme == RestartScheme::kkt       ? MC_SCHEME_KKT
@@ -367,12 +369,14 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
         auto& r = out[live[b]];
         const auto& q = ir[b];
         r.backend_actually_used = "hip";
-        r.status = q.status == MC_PDLP_OPTIMAL            ? PdlpStatus::optimal
-                   : q.status == MC_PDLP_NUMERICAL_FAILURE ? PdlpStatus::numerical_failure
-                                                           : PdlpStatus::iteration_limit;
-        r.message = r.status == PdlpStatus::optimal           ? "PDLP converged"
-                    : r.status == PdlpStatus::numerical_failure ? "numerical failure (non-finite residual)"
-                                                                : "iteration limit reached";
+        r.status = q.status == MC_PDLP_OPTIMAL                   ? PdlpStatus::optimal
+                   : q.status == MC_PDLP_NUMERICAL_FAILURE        ? PdlpStatus::numerical_failure
+                   : q.status == MC_PDLP_INFEASIBLE_OR_UNBOUNDED ? PdlpStatus::infeasible_or_unbounded
+                                                                  : PdlpStatus::iteration_limit;
+        r.message = r.status == PdlpStatus::optimal                   ? "PDLP converged"
+                    : r.status == PdlpStatus::numerical_failure         ? "numerical failure (non-finite residual)"
+                    : r.status == PdlpStatus::infeasible_or_unbounded ? "infeasibility certificate found"
+                                                                        : "iteration limit reached";
         r.primal.assign(x.begin() + inst_col[b], x.begin() + inst_col[b + 1]);
         r.dual.assign(y.begin() + inst_row[b], y.begin() + inst_row[b + 1]);
         r.objective = q.objective;

@@ -66,7 +66,8 @@ class Options(C.Structure):
         ("polish_threshold", C.c_double),
         ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("restart_scheme", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
-        ("fp32_stall_checks", C.c_int32), ("pad_", C.c_int32),
+        ("fp32_stall_checks", C.c_int32), ("detect_infeasibility", C.c_int32),
+        ("infeasibility_tolerance", C.c_double),
     ]
 
 
@@ -132,6 +133,7 @@ class EngSolve(C.Structure):
         ("reduction_order", C.c_int32), ("graph_chunk", C.c_int32),
         ("restart_scheme", C.c_int32), ("fp32_stall_checks", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
+        ("detect_infeasibility", C.c_int32), ("pad2_", C.c_int32), ("infeasibility_tolerance", C.c_double),
     ]
 
 

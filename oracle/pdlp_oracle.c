@@ -474,15 +474,11 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                             wmove = (th == 0.5) ? sqrt(omega * ratio) : pow(ratio, th) * pow(omega, 1.0 - th);
                             upd = 1;
                         }
-                        if (halpern) { /* omega_move^(3/4) omega_res^(1/4), or ^(1/4) ^(3/4)
-                                          when p/d is off by more than 30x; in square roots */
+                        if (halpern) { /* omega_move^(3/4) omega_res^(1/4), in square roots */
                             double wres = omega;
-                            int lopsided = 0;
-                            if (cur.primal > 1e-12 && cur.dual > 1e-12) {
+                            if (cur.primal > 1e-12 && cur.dual > 1e-12)
                                 wres = omega * sqrt(or_clamp(sqrt(cur.primal / cur.dual), 0.05, 20.0));
-                                lopsided = cur.primal > 30.0 * cur.dual || cur.dual > 30.0 * cur.primal;
-                            }
-                            wmove = lopsided ? sqrt(wres * sqrt(wmove * wres)) : sqrt(wmove * sqrt(wmove * wres));
+                            wmove = sqrt(wmove * sqrt(wmove * wres));
                             upd = 1;
                         }
                         if (upd) {

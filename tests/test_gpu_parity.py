@@ -272,3 +272,30 @@ def test_repeated_solves_follow_their_own_options(gpu):
     m1 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1, **kw)
     m2 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1000000, **kw)
     assert m1["fp32_iterations"] < m2["fp32_iterations"]
+
+
+def _infeasibility_cases():
+    import test_host
+    return test_host._infeasibility_cases()
+
+
+@pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED])
+@pytest.mark.parametrize("scheme", [mp.SCHEME_KKT, mp.SCHEME_HALPERN], ids=["kkt", "halpern"])
+def test_gpu_infeasibility_detection(gpu, scheme, precision):
+    """Extension: the device ray-certificate kernels stop infeasible and
+    unbounded LPs with infeasible_or_unbounded; on a feasible LP they only
+    read the iterate (the solve is bit-identical to one without them); in a
+    batch each instance gets its own status."""
+    cases, base = _infeasibility_cases()
+    kw = dict(tolerance=1e-6, restart_scheme=scheme, precision=precision, max_iterations=100000)
+    for name, lp in cases.items():
+        print("infeasibility case", name, flush=True)
+        r = gpu_solve(lp, detect_infeasibility=1, **kw)
+        assert r["status"] == "infeasible_or_unbounded", (name, r["status"], r["iterations"])
+    a = gpu_solve(base, detect_infeasibility=1, **kw)
+    b = gpu_solve(base, **kw)
+    assert a["status"] == "optimal" and a["iterations"] == b["iterations"]
+    assert np.array_equal(a["primal"], b["primal"])
+    lps = [base] + list(cases.values()) + [small_uniform(seed=5)]
+    got = mp.solve_batch(lps, backend=mp.BACKEND_GPU, require_gpu=1, detect_infeasibility=1, **kw)
+    assert [g["status"] for g in got] == ["optimal"] + ["infeasible_or_unbounded"] * len(cases) + ["optimal"]

@@ -242,3 +242,40 @@ def test_cli_restart_scheme_flag():
     assert code == 6 and js["status"] == "IterationLimit"
     p = subprocess.run([mp.CLI, mps, "--restart-scheme", "bogus"], capture_output=True, text=True, timeout=60)
     assert p.returncode == 4 and "invalid restart scheme" in p.stderr
+
+
+def _infeasibility_cases():
+    inf = np.inf
+    base = mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11)
+    A, rl = base.dense(), base.row_lower["value"]
+    lo, hi = base.var_lower["value"], base.var_upper["value"]
+    # a copy of row 0 with its right-hand side moved by 1: no x satisfies both
+    contra = mp.LPData.from_dense(np.vstack([A, A[0]]), base.objective, np.append(rl, rl[0] + 1.0),
+                                  np.append(rl, rl[0] + 1.0), lo, hi)
+    # an extra column in no row, cost -1, no upper bound: unbounded along it
+    free = mp.LPData.from_dense(np.hstack([A, np.zeros((A.shape[0], 1))]), np.append(base.objective, -1.0),
+                                rl, rl, np.append(lo, 0.0), np.append(hi, inf))
+    return {
+        "x1+x2=1 and x1+x2=2": mp.LPData.from_dense([[1, 1], [1, 1]], [1, 1], [1, 2], [1, 2], [0, 0], [inf, inf]),
+        "min -x1, x1=x2, x>=0": mp.LPData.from_dense([[1, -1]], [-1, 0], [0], [0], [0, 0], [inf, inf]),
+        "planted + contradictory row": contra,
+        "planted + free column": free,
+    }, base
+
+
+@pytest.mark.parametrize("scheme", [mp.SCHEME_KKT, mp.SCHEME_HALPERN])
+def test_cpu_infeasibility_detection(scheme):
+    """Extension: with detect_infeasibility the host path stops infeasible and
+    unbounded LPs with infeasible_or_unbounded; on a feasible LP the test only
+    reads the iterate, so the solve is bit-identical to one without it."""
+    cases, base = _infeasibility_cases()
+    for name, lp in cases.items():
+        r = mp.solve(lp, tolerance=1e-6, restart_scheme=scheme, max_iterations=100000, detect_infeasibility=1)
+        assert r["status"] == "infeasible_or_unbounded", (name, r["status"], r["iterations"])
+        assert r["message"] == "infeasibility certificate found"
+        off = mp.solve(lp, tolerance=1e-6, restart_scheme=scheme, max_iterations=2000)
+        assert off["status"] == "iteration_limit"  # the reference's behaviour: no such status
+    a = mp.solve(base, tolerance=1e-6, restart_scheme=scheme, max_iterations=100000, detect_infeasibility=1)
+    b = mp.solve(base, tolerance=1e-6, restart_scheme=scheme, max_iterations=100000)
+    assert a["status"] == "optimal" and a["iterations"] == b["iterations"]
+    assert np.array_equal(a["primal"], b["primal"]) and np.array_equal(a["dual"], b["dual"])
