@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16, help="batch: instances solved one by one (FP64)")
     ap.add_argument("--train", type=int, default=384, help="warm: training instances")
     ap.add_argument("--test", type=int, default=64, help="warm: test instances")
+    ap.add_argument("--no-roofline", action="store_true",
+                    help="skip the per-kernel timing after the solve (profiling runs of the solve itself)")
     ap.add_argument("--profile-kernels", default=None, metavar="KINDS",
                     help="only time the listed fused kernels (comma-separated Engine.KERNELS names) on the "
                          "workload's layout and exit -- the short program rocprofv3 runs wrap")
@@ -157,13 +159,18 @@ def main():
         log(f"[bench] plain FP64: {pr['status']} it={pr['iterations']} wall={pwall:.1f}ms")
 
     # roofline of the fused iteration kernels (device events on the engine stream),
-    # on the layout the solve uses (rows/columns relabelled by length bucket)
-    kern, (ta, tt) = time_kernels(mp, lp, ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64",
-                                           "spmv_fp32", "spmv_t_fp32", "copy"), not args.no_autotune)
-    dom = max(("primal_fp32", "dual_fp32"), key=lambda k: kern[k]["us"])
-    ach = kern[dom]["GBps"]
-    roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, dom)}
+    # on the layout the solve uses (rows/columns relabelled by length bucket), in
+    # the precision most of the solve's iterations ran in
+    kern, roof, ta, tt = {}, None, None, None
+    if not args.no_roofline:
+        kern, (ta, tt) = time_kernels(mp, lp, ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64",
+                                               "spmv_fp32", "spmv_t_fp32", "copy"), not args.no_autotune)
+        fp32_major = res["fp32_iterations"] * 2 > res["iterations"]
+        kinds = ("primal_fp32", "dual_fp32") if fp32_major else ("primal_fp64", "dual_fp64")
+        dom = max(kinds, key=lambda k: kern[k]["us"])
+        ach = kern[dom]["GBps"]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, dom)}
 
     cpu = None
     if not args.no_cpu:
@@ -202,7 +209,7 @@ def main():
         "objective_rel_err_vs_planted": float(f"{obj_err:.3e}"),
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(plain["ms"] / ms, 3) if plain else None,
-        "kernels": kern, "tiles": {"rows": list(ta), "cols": list(tt)},
+        "kernels": kern, "tiles": {"rows": list(ta), "cols": list(tt)} if ta else None,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
