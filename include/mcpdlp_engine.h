@@ -69,7 +69,10 @@ typedef struct mcp_tile_policy {
     int32_t per_lane;       /* target nonzeros per lane when picking lanes/row  */
 } mcp_tile_policy;
 
-/* Upload + tiling.  Reuses device buffers when sizes allow. */
+/* Upload + tiling.  Reuses device buffers when sizes allow.  The default tile
+ * policy depends on the size: {128 nnz, 2 per lane} (short wave tasks, many
+ * waves) up to 2M nonzeros, {1024, 8} above; mcp_setup's autotuner or
+ * mcp_set_tile_policy may replace it. */
 int mcp_load(mcp_engine* e, const mcp_problem* p);
 
 typedef struct mcp_setup_opts {
@@ -112,6 +115,11 @@ typedef struct mcp_solve_stats {
     int32_t graphs_built;
     int32_t chunks;
 } mcp_solve_stats;
+/* Runs the iteration loop in captured 64-iteration HIP graphs (reused while the
+ * problem view and the options are unchanged).  In a batch, once half of the
+ * instances running at the last tiling have finished, the tiles are rebuilt
+ * over the running instances only (the latency policy once they hold <= 2M
+ * nonzeros); the full tiling is restored before returning. */
 int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats);
 
 /* Per-instance outcome. */

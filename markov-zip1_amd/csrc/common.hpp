@@ -128,6 +128,32 @@ __device__ __forceinline__ T block_sum(T v, T* scratch /* >= NW */) {
         for (int i = 0; i < NW; ++i) r += scratch[i];
     return r;
 }
+// One maximum and NS sums reduced together (one barrier pair): results valid
+// in thread 0, combined in wave order (deterministic).  scratch >= NW*(NS+1).
+template <int NW, int NS>
+__device__ __forceinline__ void block_max_sums(double& mx, double (&s)[NS], double* scratch) {
+    mx = wave_max(mx);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = wave_sum(s[k]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        scratch[w * (NS + 1)] = mx;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) scratch[w * (NS + 1) + 1 + k] = s[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s[k] = 0.0;
+        for (int i = 0; i < NW; ++i) {
+            mx = maxv(mx, scratch[i * (NS + 1)]);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) s[k] += scratch[i * (NS + 1) + 1 + k];
+        }
+    }
+}
 template <int NW, typename T>
 __device__ __forceinline__ T block_max(T v, T* scratch) {
     v = wave_max(v);

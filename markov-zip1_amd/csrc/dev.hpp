@@ -65,8 +65,8 @@ struct Dev {
     double* part_dxn;   // [ntT]
     double* part_adx;   // [ntA]
     // check partials per candidate c (0 average, 1 current iterate):
-    double* part_rows;  // [(c*ntA + tile)*3 + k]: max violation, dual-objective part, |y - ylast|^2
-    double* part_cols;  // [(c*ntT + tile)*5 + k]: max violation, dual residual^2, primal obj,
+    double* part_rows;  // [(c*3 + k)*ntA + tile]: max violation, dual-objective part, |y - ylast|^2
+    double* part_cols;  // [(c*5 + k)*ntT + tile]: max violation, dual residual^2, primal obj,
                         //                          dual-objective part, |x - xlast|^2
     // infeasibility detection: the candidate's movement since the last
     // restart and its per-tile certificate partials

@@ -318,11 +318,11 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
         viol = wave_max(viol);
         dob = wave_sum(dob);
         mv = wave_sum(mv);
-        if (L.lane == 0) {
-            double* out = d.part_rows + 3 * ((CUR ? d.ntA : 0) + tile);
+        if (L.lane == 0) {  // field-major: [field][tile], one block per candidate
+            double* out = d.part_rows + (CUR ? 3 * d.ntA : 0) + tile;
             out[0] = viol;
-            out[1] = dob;
-            out[2] = mv;
+            out[d.ntA] = dob;
+            out[2 * d.ntA] = mv;
         }
     }
 }
@@ -386,13 +386,13 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
         pob = wave_sum(pob);
         dob = wave_sum(dob);
         mv = wave_sum(mv);
-        if (L.lane == 0) {
-            double* out = d.part_cols + 5 * ((CUR ? d.ntT : 0) + tile);
+        if (L.lane == 0) {  // field-major: [field][tile], one block per candidate
+            double* out = d.part_cols + (CUR ? 5 * d.ntT : 0) + tile;
             out[0] = viol;
-            out[1] = dres;
-            out[2] = pob;
-            out[3] = dob;
-            out[4] = mv;
+            out[d.ntT] = dres;
+            out[2 * d.ntT] = pob;
+            out[3 * d.ntT] = dob;
+            out[4 * d.ntT] = mv;
         }
     }
 }
@@ -473,17 +473,28 @@ __device__ void serial_residuals(const Dev& d, int b, const double* xs, const do
 // whole block; the result is valid in thread 0.
 __device__ void partial_residuals(const Dev& d, int b, int cand, double c_scale, CandResid& out,
                                   double* red) {
-    const double* pr = d.part_rows + 3 * (cand ? d.ntA : 0);
-    const double* pc = d.part_cols + 5 * (cand ? d.ntT : 0);
+    // One coalesced pass over the field-major tile partials of this
+    // instance, every field accumulated in registers, one combined reduction.
+    const int ntA = d.ntA, ntT = d.ntT;
+    const double* pr = d.part_rows + (cand ? 3 * ntA : 0);
+    const double* pc = d.part_cols + (cand ? 5 * ntT : 0);
     const int a0 = d.tA_begin[b], a1 = d.tA_begin[b + 1], t0 = d.tT_begin[b], t1 = d.tT_begin[b + 1];
     constexpr int NW = kFinal / 64;
-    const double mp = block_max<NW>(maxv(max_strided(pr, a0, a1, 3), max_strided(pc, t0, t1, 5)), red);
-    const double dr = block_sum<NW>(sum_strided(pr + 1, a0, a1, 3), red);
-    const double ym = block_sum<NW>(sum_strided(pr + 2, a0, a1, 3), red);
-    const double dsq = block_sum<NW>(sum_strided(pc + 1, t0, t1, 5), red);
-    const double po = block_sum<NW>(sum_strided(pc + 2, t0, t1, 5), red);
-    const double dc = block_sum<NW>(sum_strided(pc + 3, t0, t1, 5), red);
-    const double xm = block_sum<NW>(sum_strided(pc + 4, t0, t1, 5), red);
+    double mp = 0.0, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dr, ym, dsq, po, dc, xm
+    for (int i = a0 + (int)threadIdx.x; i < a1; i += kFinal) {
+        mp = maxv(mp, pr[i]);
+        s[0] += pr[ntA + i];
+        s[1] += pr[2 * ntA + i];
+    }
+    for (int i = t0 + (int)threadIdx.x; i < t1; i += kFinal) {
+        mp = maxv(mp, pc[i]);
+        s[2] += pc[ntT + i];
+        s[3] += pc[2 * ntT + i];
+        s[4] += pc[3 * ntT + i];
+        s[5] += pc[4 * ntT + i];
+    }
+    block_max_sums<NW, 6>(mp, s, red);
+    const double dr = s[0], ym = s[1], dsq = s[2], po = s[3], dc = s[4], xm = s[5];
     out.primal = mp;
     out.dual = sqrt(dsq) / c_scale;
     out.objective = d.offset[b] + po;
@@ -514,7 +525,7 @@ template <bool EXACT, bool FINAL>
 __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, int t) {
     const int b = blockIdx.x;  // one workgroup per instance
     InstState& s = d.st[b];
-    __shared__ double red[kFinal / 64];
+    __shared__ double red[(kFinal / 64) * 7];
     __shared__ int running;
     if (threadIdx.x == 0) running = s.status == kRunning;
     __syncthreads();
