@@ -220,9 +220,10 @@ KERNEL_SYMBOL = {"primal_fp32": "k_primal", "dual_fp32": "k_dual", "primal_fp64"
 
 
 def time_kernels(mp, lp, kinds, autotune, reps=50):
-    """{kind: us/bytes/GBps} from HIP events on the engine's stream, and the tiling."""
+    """{kind: us/bytes/GBps} from HIP events on the engine's stream, and the tiling.
+    `lp` may be a list: the LPs are packed block-diagonally as in a batched solve."""
     eng = mp.Engine(0)
-    eng.load(mp.relabel_by_length(lp))
+    eng.load([mp.relabel_by_length(q) for q in lp] if isinstance(lp, list) else mp.relabel_by_length(lp))
     eng.setup(autotune=autotune)
     kern = {}
     for name in kinds:
@@ -254,15 +255,20 @@ def profile_kernels(args):
     """Time only the listed kernels on the workload's LP (for rocprofv3 runs)."""
     import mcpdlp as mp
     wl = WORKLOADS[args.workload]
-    if "family" not in wl:
-        raise SystemExit("--profile-kernels takes a single-LP workload (large, scale or cpu-lp)")
-    lp = mp.generate(wl["family"], **wl["spec"])
+    if args.workload == "batch":
+        lp = mp.generate_batch(args.batch_count)
+    elif "family" in wl:
+        lp = mp.generate(wl["family"], **wl["spec"])
+    else:
+        raise SystemExit("--profile-kernels takes the large, scale, cpu-lp or batch workload")
     kinds = [k for k in args.profile_kernels.split(",") if k]
     for k in kinds:
         if k not in mp.Engine.KERNELS:
             raise SystemExit(f"unknown kernel kind {k!r}; choose from {sorted(mp.Engine.KERNELS)}")
     kern, tiling = time_kernels(mp, lp, kinds, not args.no_autotune)
-    print(json.dumps({"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "kernels": kern,
+    lps = lp if isinstance(lp, list) else [lp]
+    print(json.dumps({"workload": wl["name"], "instances": len(lps), "m": sum(q.rows for q in lps),
+                      "n": sum(q.cols for q in lps), "nnz": sum(q.nnz for q in lps), "kernels": kern,
                       "tiles": {"rows": list(tiling[0]), "cols": list(tiling[1])}}), flush=True)
 
 
@@ -297,11 +303,48 @@ def bench_batch(args):
     ok = sum(r["status"] == "optimal" for r in res)
     log(f"[bench] batched mixed: {ok}/{len(res)} optimal, iterations mean={its.mean():.0f} max={its.max()} "
         f"wall={wall:.1f}ms kernel={res[0]['kernel_ms']:.1f}ms")
-    # independent FP64 re-score of a few instances
-    worst = 0.0
+    # independent FP64 re-score and objective error of a few instances
+    worst, worst_obj = 0.0, 0.0
     for k in range(0, len(lps), max(1, len(lps) // 8)):
         sc = orc.kkt(lps[k], res[k]["primal"], res[k]["dual"])["score"]
         worst = max(worst, sc)
+        worst_obj = max(worst_obj, abs(res[k]["objective"] - lps[k].planted_objective) /
+                        (1.0 + abs(lps[k].planted_objective)))
+    # roofline of the dominant fused kernel on the packed batch
+    fp32_major = sum(r["fp32_iterations"] for r in res) * 2 > int(its.sum())
+    kinds = ("primal_fp32", "dual_fp32") if fp32_major else ("primal_fp64", "dual_fp64")
+    kern, _ = time_kernels(mp, lps, kinds, True)
+    dom = max(kinds, key=lambda k: kern[k]["us"])
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(kern[dom]["GBps"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic("batch", dom)}
+    cpu = None
+    if not args.no_cpu:
+        # compiled reference on 1 core: ms/iteration of three instances (smallest, median,
+        # largest), fitted linearly in nnz and applied to every instance's GPU iteration count
+        use_ref = orc.ref_available()
+        fn = orc.ref_solve if use_ref else orc.solve
+        order_nnz = np.argsort([l.nnz for l in lps])
+        picks = [order_nnz[0], order_nnz[len(lps) // 2], order_nnz[-1]]
+        pts = []
+        for k in picks:
+            lo_it = 40
+            t0c = time.perf_counter()
+            fn(lps[k], tolerance=tol, max_iterations=lo_it)
+            t_lo = time.perf_counter() - t0c
+            per_guess = max(t_lo / (lo_it + 20), 1e-5)
+            hi_it = lo_it + max(40, int(args.cpu_seconds / 3 * 0.6 / per_guess) // 40 * 40)
+            t0c = time.perf_counter()
+            fn(lps[k], tolerance=tol, max_iterations=hi_it)
+            ms_it = (time.perf_counter() - t0c - t_lo) * 1e3 / (hi_it - lo_it)
+            pts.append((lps[k].nnz, ms_it))
+            log(f"[bench] cpu baseline sample nnz={lps[k].nnz}: {ms_it:.3f} ms/iteration")
+        a = sum(p[0] * p[1] for p in pts) / sum(p[0] * p[0] for p in pts)  # ms per iteration per nonzero
+        est_cpu = float(sum(a * l.nnz * r["iterations"] for l, r in zip(lps, res)))
+        cpu = {"value": round(est_cpu, 1), "unit": "ms", "cores": 1, "kind": "reference" if use_ref else "port",
+               "sample": "ms/iteration of the smallest, median and largest instance "
+                         + ", ".join(f"(nnz {n}: {v:.3f} ms)" for n, v in pts)
+                         + "; fitted linearly in nnz and summed over all instances x their GPU iteration counts "
+                           "(extrapolated)"}
     # plain FP64 GPU PDLP, one instance at a time, on an evenly spaced sample by size
     order = np.argsort([l.rows for l in lps])
     sample = order[np.linspace(0, len(lps) - 1, min(args.plain_sample, len(lps))).astype(int)]
@@ -319,11 +362,13 @@ def bench_batch(args):
            "plain_restart_scheme": args.plain_scheme},
           {"optimal": int(ok), "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
            "kkt_fp64_rescored_worst_of_sample": float(f"{worst:.4e}"),
+           "objective_rel_err_vs_planted_worst_of_sample": float(f"{worst_obj:.3e}"),
            "plain_fp64_per_instance": {"sampled": int(len(sample)), "mean_ms": round(float(np.mean(plain_ms)), 3),
                                        "estimated_total_ms": round(est, 1),
                                        "iterations_mean": float(np.mean(plain_it)),
                                        "note": "evenly spaced by size; total = mean x instances (extrapolated)"},
-           "speedup_vs_plain_fp64_per_instance": round(est / wall, 2)})
+           "speedup_vs_plain_fp64_per_instance": round(est / wall, 2),
+           "kernels": kern, "roofline": roof, "cpu_baseline": cpu})
 
 
 def bench_warm(args):
