@@ -10,14 +10,15 @@
 // plus, every 10th iteration, k_adx (A dx) and k_adapt_finalize (eta update,
 // :207-219), and every restart_every iterations the FP64 check: k_resid
 // (A x_c, A^T y_c and the residual terms on the unscaled model, :62-135),
-// k_check_finalize (termination, restart, primal weight, :229-260) and
-// k_post_check (restart copy / FP32->FP64 switch).
+// k_check_finalize (termination, restart, primal weight, :229-260),
+// optionally the infeasibility-certificate kernels k_ray_*, and k_post_check
+// (restart copy).
 //
 // Every row kernel is a set of wave tasks (one Tile per wave64, no LDS, no
 // workgroup barrier): a wave gates on its instance's status, picks FP32 or
-// FP64 from the instance's mode at run time (so mixed precision costs no
-// extra launches), sweeps its rows 64/G at a time and lets lane 0 of each
-// row group apply the update.  Step sizes are recomputed per element from
+// FP64 SpMV operands from the instance's mode at run time (so mixed precision
+// costs no extra launches; the iterate is FP64 in both), sweeps its rows 64/G
+// at a time and lets lane 0 of each row group apply the update.  Step sizes are recomputed per element from
 // (eta, omega) and the scaled L1 norms exactly as the reference's
 // update_step_sizes() forms them (tau = (eta/omega)/colnorm,
 // sigma = (eta*omega)/rownorm).  Built with -ffp-contract=off: every product
