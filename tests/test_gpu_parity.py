@@ -299,3 +299,31 @@ def test_gpu_infeasibility_detection(gpu, scheme, precision):
     lps = [base] + list(cases.values()) + [small_uniform(seed=5)]
     got = mp.solve_batch(lps, backend=mp.BACKEND_GPU, require_gpu=1, detect_infeasibility=1, **kw)
     assert [g["status"] for g in got] == ["optimal"] + ["infeasible_or_unbounded"] * len(cases) + ["optimal"]
+
+
+def _golden_runs():
+    import test_oracle
+    return test_oracle.INPUTS, test_oracle.GOLD["runs"]
+
+
+@pytest.mark.parametrize("scheme", [mp.SCHEME_REFERENCE, mp.SCHEME_KKT, mp.SCHEME_HALPERN],
+                         ids=["reference", "kkt", "halpern"])
+def test_golden_inputs_reference_order_bit_exact(gpu, scheme):
+    """Every input the compiled reference wrote golden fixtures for (maximize
+    sense, ranged rows, free variables, open '>=' rows, one-variable and
+    uniform/Zipf LPs), with each fixture's options: the GPU in reference order
+    reproduces the oracle bit for bit -- and, for the reference scheme, the
+    compiled reference's own iteration count and objective bits."""
+    inputs, runs = _golden_runs()
+    for run in runs:
+        lp = inputs[run["case"]]
+        kw = dict(run["options"], restart_scheme=scheme)
+        print("golden", run["case"], kw, flush=True)
+        want = orc.solve(lp, **kw)
+        got = gpu_solve(lp, reduction_order=mp.ORDER_REFERENCE, **kw)
+        assert got["status"] == want["status"] and got["iterations"] == want["iterations"], run["case"]
+        assert got["objective"] == want["objective"], run["case"]
+        assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
+        if scheme == mp.SCHEME_REFERENCE:
+            assert got["iterations"] == run["iterations"]
+            assert got["objective"] == float.fromhex(run["objective"])

@@ -279,3 +279,24 @@ def test_cpu_infeasibility_detection(scheme):
     b = mp.solve(base, tolerance=1e-6, restart_scheme=scheme, max_iterations=100000)
     assert a["status"] == "optimal" and a["iterations"] == b["iterations"]
     assert np.array_equal(a["primal"], b["primal"]) and np.array_equal(a["dual"], b["dual"])
+
+
+def test_relabel_by_length_is_a_permutation():
+    """mcpdlp.relabel_by_length (the host driver's length-bucket relabelling,
+    for Engine users): same LP up to a row/column permutation -- the relabelled
+    dense matrix is a permutation of the original, columns and rows come in
+    descending length bucket, every column's rows are strictly increasing, and
+    the permuted planted pair keeps its objective and KKT score."""
+    lp = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12)
+    q = mp.relabel_by_length(lp)
+    A, Q = lp.dense(), q.dense()
+    clen, rlen = np.diff(lp.column_start), np.bincount(lp.row_index, minlength=lp.rows)
+    cperm = np.argsort(-mp._length_bucket(clen), kind="stable")
+    rperm = np.argsort(-mp._length_bucket(rlen), kind="stable")
+    assert np.array_equal(Q, A[np.ix_(rperm, cperm)])
+    assert np.all(np.diff(mp._length_bucket(np.diff(q.column_start))) <= 0)
+    for j in range(q.cols):
+        assert np.all(np.diff(q.row_index[q.column_start[j]:q.column_start[j + 1]]) > 0)
+    x, y = lp.x_star[cperm], -lp.y_star[rperm]
+    assert np.isclose(q.objective @ x, lp.objective @ lp.x_star, rtol=1e-14)
+    assert orc.kkt(q, x, y)["score"] == pytest.approx(orc.kkt(lp, lp.x_star, -lp.y_star)["score"], abs=1e-12)
