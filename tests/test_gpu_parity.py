@@ -327,3 +327,25 @@ def test_golden_inputs_reference_order_bit_exact(gpu, scheme):
         if scheme == mp.SCHEME_REFERENCE:
             assert got["iterations"] == run["iterations"]
             assert got["objective"] == float.fromhex(run["objective"])
+
+
+def test_cli_gpu_backend(gpu):
+    """markov-cero-solve --engine pdlp --backend gpu: the JSON and the stderr
+    verdict name the HIP path; reference order reproduces the CPU backend's
+    answer to the bit (tests/golden/tiny_simplex.mps, ADR-P3-05 known answer)."""
+    import json
+    import os
+    import subprocess
+    mps = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tiny_simplex.mps")
+
+    def run(*extra):
+        p = subprocess.run([mp.CLI, mps, "--engine", "pdlp", "--tolerance", "1e-6", *extra],
+                           capture_output=True, text=True, timeout=120)
+        return p.returncode, json.loads(p.stdout.strip().splitlines()[-1]), p.stderr
+
+    code, js, err = run("--backend", "gpu", "--order", "reference")
+    assert code == 0 and js["status"] == "Optimal" and js["verified"]
+    assert js["backend_actually_used"] == "hip" and "backend_actually_used=hip" in err
+    code_c, js_c, _ = run("--backend", "cpu")
+    assert js["objective"] == js_c["objective"] and js["primal"] == js_c["primal"]
+    assert round(js["objective"], 7) == -9.0000013
