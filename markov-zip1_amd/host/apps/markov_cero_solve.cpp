@@ -180,7 +180,11 @@ int main(int argc, char** argv) {
        << "\",\"backend_actually_used\":\"" << esc(res.backend_actually_used) << "\",\"h2d_ms\":" << num(res.h2d_ms)
        << ",\"kernel_ms\":" << num(res.kernel_ms) << ",\"d2h_ms\":" << num(res.d2h_ms)
        << ",\"total_ms\":" << num(res.total_ms) << ",\"precision\":\"" << precision
-       << "\",\"fp32_iterations\":" << res.fp32_iterations;
+       << "\",\"fp32_iterations\":" << res.fp32_iterations << ",\"setup_ms\":" << num(res.setup_ms)
+       << ",\"restart_scheme\":\"" << scheme << "\",\"dual\":[";
+    // extension keys: the dual vector (reference sign convention, g = c + A^T y) and setup time
+    for (std::size_t i = 0; i < res.dual.size(); ++i) js << (i ? "," : "") << num(res.dual[i]);
+    js << "]";
     if (!error.empty()) js << ",\"error\":\"" << esc(error) << "\"";
     js << "}\n";
     std::cout << js.str();

@@ -134,6 +134,9 @@ def test_cli_tiny_simplex_adr():
     code, js, err = run_cli([os.path.join(GOLDEN, "tiny_simplex.mps"), "--engine", "pdlp", "--tolerance", "1e-6"])
     assert code == 0 and js["status"] == "Optimal" and js["verified"]
     assert round(js["objective"], 7) == -9.0000013 and js["backend"] == "cpu"
+    # extension key: the dual in the reference's convention g = c + A^T y; at the
+    # optimum (1, 3) both rows are tight and c + A^T y = 0 gives y = (1, 1)
+    assert js["dual"] == pytest.approx([1.0, 1.0], abs=1e-5)
     for key in ("relative_primal_residual", "relative_dual_residual", "relative_duality_gap", "h2d_ms", "kernel_ms",
                 "d2h_ms", "total_ms", "lp_iterations", "pdlp_tolerance", "primal"):
         assert key in js
