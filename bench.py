@@ -52,7 +52,7 @@ WORKLOADS = {
                        "linking rows (2 nnz/col), |a| log-uniform [1e-2,1e2]"),
     "batch": dict(tol=1e-6, name="256 planted LPs, m log-uniform [500,20000], n=2.5m, 4-8 nnz/col, "
                                  "box [0,U[1,100]], one batched solve"),
-    "warm": dict(tol=1e-6, name="LP family on one structure m=2000 n=5000 6 nnz/col, b/c/u perturbed "
+    "warm": dict(tol=1e-6, name="LP family on one structure ({rows}x{cols}, 6 nnz/col), b/c/u perturbed "
                                 "(lognormal 0.1, U[0.8,1.2]); learned warm start vs cold"),
 }
 
@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16, help="batch: instances solved one by one (FP64)")
     ap.add_argument("--train", type=int, default=384, help="warm: training instances")
     ap.add_argument("--test", type=int, default=64, help="warm: test instances")
+    ap.add_argument("--warm-rows", type=int, default=2000, help="warm: rows of the family's structure")
+    ap.add_argument("--warm-cols", type=int, default=5000, help="warm: columns of the family's structure")
     ap.add_argument("--no-roofline", action="store_true",
                     help="skip the per-kernel timing after the solve (profiling runs of the solve itself)")
     ap.add_argument("--profile-kernels", default=None, metavar="KINDS",
@@ -380,7 +382,7 @@ def bench_warm(args):
     tol = args.tolerance or 1e-6
     ntr, nte = args.train, args.test
     t = time.time()
-    base, fam = mp.generate_family(dict(rows=2000, cols=5000, nnz_min=6, nnz_max=6),
+    base, fam = mp.generate_family(dict(rows=args.warm_rows, cols=args.warm_cols, nnz_min=6, nnz_max=6),
                                    seeds=[0x4D4350444C50 + 4000 + k for k in range(ntr + nte)], sigma=0.1)
     train, test = fam[:ntr], fam[ntr:]
     log(f"[bench] family: {ntr} train + {nte} test instances of {base.rows}x{base.cols} "
@@ -409,7 +411,7 @@ def bench_warm(args):
     wi = np.array([r["iterations"] for r in warm])
     log(f"[bench] cold: it mean={ci.mean():.0f} max={ci.max()} {cold_ms:.0f}ms | warm: it mean={wi.mean():.0f} "
         f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference)")
-    _line(warm_ms, WORKLOADS["warm"]["name"],
+    _line(warm_ms, WORKLOADS["warm"]["name"].format(rows=base.rows, cols=base.cols),
           {"train": ntr, "test": nte, "m": base.rows, "n": base.cols, "nnz": base.nnz, "tolerance": tol,
            "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": args.scheme},
           {"cold_ms": round(cold_ms, 1), "warm_ms": round(warm_ms, 1),
