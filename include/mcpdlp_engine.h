@@ -151,7 +151,10 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
                     int32_t* passes, int32_t* converged);
 
 /* Micro-benchmark of one fused iteration kernel on the loaded, set-up problem,
- * launched `reps` times on the engine's stream between two HIP events.
+ * launched `reps` times on the engine's stream.  The fused steps (kinds 0-3)
+ * are timed as the solve runs them: each timed launch follows a launch of the
+ * other step (untimed), which streams the other matrix through the caches;
+ * the other kinds are timed back to back between two HIP events.
  * kind: 0 primal (A^T y + primal update) FP32, 1 dual (A x_bar + dual update)
  * FP32, 2 primal FP64, 3 dual FP64, 4 plain SpMV A x FP32, 5 plain A^T y FP32,
  * 6 plain streaming copy of the FP32 values of A (the calibration kernel for
@@ -166,6 +169,18 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
  * re-tile; the autotuner's choice is overwritten.  For tuning experiments and
  * for callers that know their matrix. */
 int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const mcp_tile_policy* cols_policy);
+
+/* Block tiles of the fused fast-order primal/dual steps (one 256-thread
+ * workgroup streams a contiguous run of rows holding at most `budget`
+ * nonzeros; the products are summed per row from LDS).  Budgets are 512,
+ * 1024 or 2048 nonzeros per tile; 0 for both runs those steps on the wave
+ * tiles of mcp_set_tile_policy instead.  Default: 512 up to 2M nonzeros,
+ * 1024 above; the autotuner may replace it.  Reference order always uses the
+ * wave tiles.  No counterpart in the reference (its SpMV is a serial loop,
+ * src/lp/first_order/pdlp.cpp:22-44): a tuning knob of this engine. */
+int mcp_set_block_budget(mcp_engine* e, int32_t rows_budget, int32_t cols_budget);
+int mcp_get_block_tiling(mcp_engine* e, int32_t* rows_budget, int32_t* cols_budget, int32_t* row_tiles,
+                         int32_t* col_tiles);
 
 /* Current tile policies (after mcp_setup / autotune) and tile counts. */
 int mcp_get_tiling(mcp_engine* e, mcp_tile_policy* rows_policy, mcp_tile_policy* cols_policy,

@@ -62,7 +62,7 @@ struct Dev {
     double* ax2;   // [m] reference-order mode, current-iterate candidate
     double* aty2;  // [n]
     // reduction partials (one slot group per tile)
-    double* part_dxn;   // [ntT]
+    double* part_dxn;   // [ntT], or [nbT] when the primal step runs on block tiles
     double* part_adx;   // [ntA]
     // check partials per candidate c (0 average, 1 current iterate):
     double* part_rows;  // [(c*3 + k)*ntA + tile]: max violation, dual-objective part, |y - ylast|^2
@@ -80,9 +80,18 @@ struct Dev {
     InstState* st;      // [B]
     int64_t* iter_base; // global iteration counter at the start of the current chunk
     int32_t* counters;  // [0] running instances, [1] running instances in FP32
+    // block tiles of the fused fast-order primal/dual kernels (stream.hip):
+    // one 256-thread workgroup per tile
+    const Tile* bA;
+    const Tile* bT;
+    const int32_t* bA_begin;   // [B+1] tile range of each instance in bA
+    const int32_t* bT_begin;   // [B+1]
+    int32_t nbA, nbT;
+    int32_t budA, budT;        // nonzeros per block tile: 512, 1024 or 2048
     int32_t halpern;    // reflected Halpern iterates: the average slots hold T(z), the
                         // last-restart slots the anchor z0
-    int32_t pad_;
+    int32_t blocks;     // 1: the fast-order primal/dual steps run on the block tiles
+                        // (part_dxn then holds one slot per tile of bT); 0: on tA / tT
 };
 
 // ---- launch helpers (kernels.hip) ----
@@ -115,5 +124,11 @@ void launch_unscale(const Dev& d, double* x_out, double* y_out, hipStream_t s);
 // tiles with one lane per row sum in the reference's serial order.
 void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const int32_t* idx,
                        const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
+// Block-tile forms (stream.hip): the fused fast-order primal/dual steps and a
+// plain SpMV over block tiles of `budget` (512, 1024 or 2048) nonzeros.
+void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s);
+void launch_dual_blk(const Dev& d, int t, hipStream_t s);
+void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
+                     const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
 
 }  // namespace mcp

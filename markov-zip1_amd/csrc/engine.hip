@@ -124,6 +124,64 @@ void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& in
     }
 }
 
+// Block tiles of the fused fast-order primal/dual steps (stream.hip): runs of
+// consecutive rows of one instance holding at most `budget` nonzeros and at
+// most kBlock rows, one workgroup each.  Lanes per row G = 2^lg: as many as
+// the workgroup has per row (G * rows <= kBlock), at most 64 and no more than
+// the tile's longest row needs.  A row longer than the budget is a tile of
+// its own (shape bit 3) that the whole workgroup strides.
+constexpr int kBlockBudget = 1024;
+constexpr int kBlockBudgetLatency = 512;
+
+int floor_log2(int v) {
+    int l = 0;
+    while ((2 << l) <= v) ++l;
+    return l;
+}
+
+void build_block_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& inst, int B, int budget,
+                       std::vector<mcp::Tile>& tiles, std::vector<int32_t>& begin,
+                       const std::vector<char>* keep = nullptr) {
+    tiles.clear();
+    begin.assign(B + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        begin[b] = (int32_t)tiles.size();
+        if (keep && !(*keep)[b]) continue;
+        int r = inst[b];
+        const int re = inst[b + 1];
+        while (r < re) {
+            const int len0 = ptr[r + 1] - ptr[r];
+            if (len0 > budget) {
+                tiles.push_back({r, r + 1, b, 8});
+                ++r;
+                continue;
+            }
+            const int r0 = r;
+            int nnz = 0, maxlen = 0;
+            while (r < re && (r - r0) < mcp::kBlock) {
+                const int len = ptr[r + 1] - ptr[r];
+                if (len > budget || nnz + len > budget) break;
+                nnz += len;
+                maxlen = std::max(maxlen, len);
+                ++r;
+            }
+            const int rows = r - r0;
+            int lg = std::min(6, floor_log2(mcp::kBlock / rows));
+            while (lg > 0 && (1 << (lg - 1)) >= maxlen) --lg;  // G no larger than the longest row needs
+            tiles.push_back({r0, r, b, lg});
+        }
+    }
+    begin[B] = (int32_t)tiles.size();
+    for (const auto& t : tiles) {  // invariants the block kernels rely on
+        const int lg = t.shape & 7;
+        const int nz = ptr[t.r1] - ptr[t.r0];
+        const bool lng = (t.shape & 8) != 0;
+        if (t.r1 <= t.r0 || (t.shape & ~15) != 0 || (lng && (t.r1 != t.r0 + 1 || lg != 0)) ||
+            (!lng && (nz > budget || ((t.r1 - t.r0) << lg) > mcp::kBlock || lg > 6)))
+            throw EngineError(MC_ERR_INTERNAL, "block tile invariant violated");
+    }
+}
+
 double ms_between(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, a, b));
@@ -147,6 +205,11 @@ struct mcp_engine {
     std::vector<mcp::Tile> tA, tT;
     std::vector<int32_t> tA_begin, tT_begin;
     mcp_tile_policy polA{1024, 8}, polT{1024, 8};
+    // block tiles of the fused fast-order steps; budget 0 => those steps run
+    // on the wave tiles
+    std::vector<mcp::Tile> bA, bT;
+    std::vector<int32_t> bA_begin, bT_begin;
+    int budA = kBlockBudget, budT = kBlockBudget;
     bool loaded = false, setup_done = false, exact_tiles = false;
     bool compacted = false;  // tiles cover only the instances still running (inside mcp_solve)
     int ruiz_passes_req = -1, ruiz_on_req = -1;
@@ -156,7 +219,11 @@ struct mcp_engine {
     mcp::IterOpts graph_io{};  //   the problem's device view and the solve options
     // autotuned tile policies per problem signature (instances, rows, columns,
     // nonzeros, pointer checksum), so repeated solves of one shape tune once
-    std::map<std::tuple<int, int, int, int64_t, uint64_t>, std::pair<mcp_tile_policy, mcp_tile_policy>> tuned;
+    struct Tuned {
+        mcp_tile_policy polA, polT;
+        int budA, budT;
+    };
+    std::map<std::tuple<int, int, int, int64_t, uint64_t>, Tuned> tuned;
     int32_t* pinned = nullptr;  // [2 slots x 2 counters]
     hipEvent_t ev[4]{};
 
@@ -191,16 +258,38 @@ struct mcp_engine {
     void upload_tiles() {
         auto* dA = buf<mcp::Tile>("tA", tA.size());
         auto* dT = buf<mcp::Tile>("tT", tT.size());
-        auto* bA = buf<int32_t>("tA_begin", B + 1);
-        auto* bT = buf<int32_t>("tT_begin", B + 1);
+        auto* tbA = buf<int32_t>("tA_begin", B + 1);
+        auto* tbT = buf<int32_t>("tT_begin", B + 1);
         if (!tA.empty())
             HIPCHK(hipMemcpyAsync(dA, tA.data(), tA.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
         if (!tT.empty())
             HIPCHK(hipMemcpyAsync(dT, tT.data(), tT.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(bA, tA_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(bT, tT_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(tbA, tA_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(tbT, tT_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        auto* dbA = buf<mcp::Tile>("bA", bA.size());
+        auto* dbT = buf<mcp::Tile>("bT", bT.size());
+        auto* bbA = buf<int32_t>("bA_begin", B + 1);
+        auto* bbT = buf<int32_t>("bT_begin", B + 1);
+        const bool blk = !bA_begin.empty();
+        if (!bA.empty())
+            HIPCHK(hipMemcpyAsync(dbA, bA.data(), bA.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
+        if (!bT.empty())
+            HIPCHK(hipMemcpyAsync(dbT, bT.data(), bT.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
+        if (blk) {
+            HIPCHK(hipMemcpyAsync(bbA, bA_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(bbT, bT_begin.data(), (B + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        }
+        d.bA = dbA;
+        d.bT = dbT;
+        d.bA_begin = bbA;
+        d.bT_begin = bbT;
+        d.nbA = (int32_t)bA.size();
+        d.nbT = (int32_t)bT.size();
+        d.budA = blk ? budA : 0;
+        d.budT = blk ? budT : 0;
+        d.blocks = blk ? 1 : 0;
         // partial-sum slots follow the tile counts
-        d.part_dxn = buf<double>("part_dxn", tT.size());
+        d.part_dxn = buf<double>("part_dxn", std::max(tT.size(), bT.size()));
         d.part_adx = buf<double>("part_adx", tA.size());
         d.part_rows = buf<double>("part_rows", 6 * tA.size());
         d.part_cols = buf<double>("part_cols", 10 * tT.size());
@@ -208,8 +297,8 @@ struct mcp_engine {
         d.part_ray_cols = buf<double>("part_ray_cols", 7 * tT.size());
         d.tA = dA;
         d.tT = dT;
-        d.tA_begin = bA;
-        d.tT_begin = bT;
+        d.tA_begin = tbA;
+        d.tT_begin = tbT;
         d.ntA = (int32_t)tA.size();
         d.ntT = (int32_t)tT.size();
         HIPCHK(hipStreamSynchronize(stream));
@@ -219,6 +308,15 @@ struct mcp_engine {
     void retile(bool exact, const std::vector<char>* keep = nullptr) {
         build_tiles(rp, inst_row, B, polA, exact, tA, tA_begin, keep);
         build_tiles(cp, inst_col, B, polT, exact, tT, tT_begin, keep);
+        if (!exact && budA > 0 && budT > 0) {
+            build_block_tiles(rp, inst_row, B, budA, bA, bA_begin, keep);
+            build_block_tiles(cp, inst_col, B, budT, bT, bT_begin, keep);
+        } else {
+            bA.clear();
+            bT.clear();
+            bA_begin.clear();
+            bT_begin.clear();
+        }
         exact_tiles = exact;
         compacted = keep != nullptr;
         upload_tiles();
@@ -270,6 +368,50 @@ struct mcp_engine {
             }
         }
         (rows ? polA : polT) = bestp;
+    }
+
+    // Same for the block-tile budget of the fused steps (plain block SpMV,
+    // FP32 values): candidates 512, 1024, 2048; the incumbent is the
+    // size-based default.  Budget 0 (block tiles off) is left alone.
+    void autotune_block(bool rows) {
+        int& bud = rows ? budA : budT;
+        if (bud <= 0) return;
+        const auto& ptr = rows ? rp : cp;
+        const auto& inst = rows ? inst_row : inst_col;
+        const int nrows = rows ? m : n;
+        const int ncols = rows ? n : m;
+        if (nrows == 0) return;
+        float* vin = buf<float>("tune_in", ncols);
+        float* vout = buf<float>("tune_out", nrows);
+        HIPCHK(hipMemsetAsync(vin, 0, sizeof(float) * std::max(ncols, 1), stream));
+        const int32_t* dptr = rows ? d.rp : d.cp;
+        const int32_t* didx = rows ? d.ci : d.ri;
+        const float* dval = rows ? d.av32 : d.tv32;
+        std::vector<mcp::Tile> tl;
+        std::vector<int32_t> tb;
+        double best = std::numeric_limits<double>::infinity();
+        int bestb = bud;
+        for (const int cand : {bud, 512, 1024, 2048}) {
+            build_block_tiles(ptr, inst, B, cand, tl, tb);
+            auto* dt = buf<mcp::Tile>("tune_tiles", tl.size());
+            HIPCHK(hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, stream));
+            mcp::launch_spmv_blk(dt, (int)tl.size(), cand, dptr, didx, dval, vin, vout, true, stream);
+            double t = std::numeric_limits<double>::infinity();
+            for (int trial = 0; trial < 3; ++trial) {
+                HIPCHK(hipEventRecord(ev[2], stream));
+                const int reps = 5;
+                for (int r = 0; r < reps; ++r)
+                    mcp::launch_spmv_blk(dt, (int)tl.size(), cand, dptr, didx, dval, vin, vout, true, stream);
+                HIPCHK(hipEventRecord(ev[3], stream));
+                HIPCHK(hipEventSynchronize(ev[3]));
+                t = std::min(t, ms_between(ev[2], ev[3]) / reps);
+            }
+            if (t < 0.98 * best) {
+                best = t;
+                bestb = cand;
+            }
+        }
+        bud = bestb;
     }
 };
 
@@ -469,8 +611,8 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         e->h2d_ms = ms_between(e->ev[0], e->ev[1]);
         e->loaded = true;
         e->setup_done = false;
-        e->polA = e->polT = mcp_tile_policy{1024, 8};
         e->polA = e->polT = e->nnz <= kLatencyNnz ? kLatencyPolicy : kThroughputPolicy;
+        e->budA = e->budT = e->nnz <= kLatencyNnz ? kBlockBudgetLatency : kBlockBudget;
         e->retile(false);  // setup kernels run on these tiles
     });
 }
@@ -517,12 +659,16 @@ int mcp_setup(mcp_engine* e, const mcp_setup_opts* o) {
             const auto key = std::make_tuple(e->B, e->m, e->n, e->nnz, h);
             auto it = e->tuned.find(key);
             if (it != e->tuned.end()) {
-                e->polA = it->second.first;
-                e->polT = it->second.second;
+                e->polA = it->second.polA;
+                e->polT = it->second.polT;
+                e->budA = it->second.budA;
+                e->budT = it->second.budT;
             } else {
                 e->autotune_one(true);
                 e->autotune_one(false);
-                e->tuned[key] = {e->polA, e->polT};
+                e->autotune_block(true);
+                e->autotune_block(false);
+                e->tuned[key] = {e->polA, e->polT, e->budA, e->budT};
             }
         }
         e->retile(o->exact != 0);
@@ -599,6 +745,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         // instances only, so finished instances stop costing waves.
         int32_t compact_at = e->B > 1 ? e->B / 2 : -1;
         const mcp_tile_policy polA0 = e->polA, polT0 = e->polT;
+        const int budA0 = e->budA, budT0 = e->budT;
         while (iter < max_it) {
             const int len = (int)std::min<int64_t>(chunk, max_it - iter);
             if (use_graphs && len == chunk) {
@@ -649,7 +796,10 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                     int64_t live_nnz = 0;
                     for (int b = 0; b < e->B; ++b)
                         if (keep[b]) live_nnz += e->rp[e->inst_row[b + 1]] - e->rp[e->inst_row[b]];
-                    if (live_nnz <= kLatencyNnz) e->polA = e->polT = kLatencyPolicy;
+                    if (live_nnz <= kLatencyNnz) {
+                        e->polA = e->polT = kLatencyPolicy;
+                        if (e->budA > 0 && e->budT > 0) e->budA = e->budT = kBlockBudgetLatency;
+                    }
                     e->retile(e->exact_tiles, &keep);
                     e->drop_graphs();
                     e->graph_dev = d;
@@ -686,6 +836,8 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         if (e->compacted) {  // outside mcp_solve the tiles always cover every instance
             e->polA = polA0;
             e->polT = polT0;
+            e->budA = budA0;
+            e->budT = budT0;
             e->retile(e->exact_tiles);
             e->drop_graphs();
             e->graph_dev = d;
@@ -753,8 +905,14 @@ int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
         const int nin = t ? e->m : e->n, nout = t ? e->n : e->m;
         std::vector<mcp::Tile> tl;
         std::vector<int32_t> tb;
-        build_tiles(t ? e->cp : e->rp, t ? e->inst_col : e->inst_row, e->B, t ? e->polT : e->polA,
-                    exact, tl, tb);
+        // fast order: the block tiles of the fused steps (when enabled)
+        const int bud = t ? e->budT : e->budA;
+        const bool blk = !exact && e->budA > 0 && e->budT > 0;
+        if (blk)
+            build_block_tiles(t ? e->cp : e->rp, t ? e->inst_col : e->inst_row, e->B, bud, tl, tb);
+        else
+            build_tiles(t ? e->cp : e->rp, t ? e->inst_col : e->inst_row, e->B, t ? e->polT : e->polA,
+                        exact, tl, tb);
         auto* dt = e->buf<mcp::Tile>("spmv_tiles", tl.size());
         if (!tl.empty())
             HIPCHK(hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(mcp::Tile), hipMemcpyHostToDevice, s));
@@ -774,10 +932,16 @@ int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
         const int32_t* ptr = t ? d.cp : d.rp;
         const int32_t* idx = t ? d.ri : d.ci;
         const void* val = f32 ? (const void*)(t ? d.tv32 : d.av32) : (const void*)(t ? d.tv64 : d.av64);
-        mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, s);
+        auto launch = [&] {
+            if (blk)
+                mcp::launch_spmv_blk(dt, (int)tl.size(), bud, ptr, idx, val, din, dout, f32, s);
+            else
+                mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, s);
+        };
+        launch();
         const int nrep = std::max(1, reps);
         HIPCHK(hipEventRecord(e->ev[0], s));
-        for (int r = 0; r < nrep; ++r) mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, s);
+        for (int r = 0; r < nrep; ++r) launch();
         HIPCHK(hipEventRecord(e->ev[1], s));
         HIPCHK(hipGetLastError());
         if (f32) {
@@ -868,12 +1032,35 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         };
         launch();
         const int nrep = std::max(1, reps);
-        HIPCHK(hipEventRecord(e->ev[0], s));
-        for (int r = 0; r < nrep; ++r) launch();
-        HIPCHK(hipEventRecord(e->ev[1], s));
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventSynchronize(e->ev[1]));
-        *avg_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / nrep;
+        if (kind <= 3) {
+            // The fused steps are timed as the solve runs them: each launch
+            // follows the other step, which streams the other matrix through
+            // the caches in between.
+            auto partner = [&] {
+                if (kind == 0 || kind == 2)
+                    mcp::launch_dual(d, 1, false, s);
+                else
+                    mcp::launch_primal(d, 1, false, false, s);
+            };
+            double tot = 0.0;
+            for (int r = 0; r < nrep; ++r) {
+                partner();
+                HIPCHK(hipEventRecord(e->ev[0], s));
+                launch();
+                HIPCHK(hipEventRecord(e->ev[1], s));
+                HIPCHK(hipEventSynchronize(e->ev[1]));
+                tot += ms_between(e->ev[0], e->ev[1]);
+            }
+            HIPCHK(hipGetLastError());
+            *avg_us = 1000.0 * tot / nrep;
+        } else {
+            HIPCHK(hipEventRecord(e->ev[0], s));
+            for (int r = 0; r < nrep; ++r) launch();
+            HIPCHK(hipEventRecord(e->ev[1], s));
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventSynchronize(e->ev[1]));
+            *avg_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / nrep;
+        }
         *bytes = b;
     });
 }
@@ -888,6 +1075,30 @@ int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const
         e->polA = *rows_policy;
         e->polT = *cols_policy;
         e->retile(e->exact_tiles);
+    });
+}
+
+int mcp_set_block_budget(mcp_engine* e, int32_t rows_budget, int32_t cols_budget) {
+    return guarded([&] {
+        require(e && e->loaded, "mcp_set_block_budget needs a loaded problem");
+        auto ok = [](int32_t b) { return b == 0 || b == 512 || b == 1024 || b == 2048; };
+        require(ok(rows_budget) && ok(cols_budget) && ((rows_budget == 0) == (cols_budget == 0)),
+                "block budgets must both be 0 or each one of 512, 1024, 2048");
+        HIPCHK(hipSetDevice(e->device));
+        e->budA = rows_budget;
+        e->budT = cols_budget;
+        e->retile(e->exact_tiles);
+    });
+}
+
+int mcp_get_block_tiling(mcp_engine* e, int32_t* rows_budget, int32_t* cols_budget, int32_t* row_tiles,
+                         int32_t* col_tiles) {
+    return guarded([&] {
+        require(e != nullptr, "null engine");
+        if (rows_budget) *rows_budget = e->budA;
+        if (cols_budget) *cols_budget = e->budT;
+        if (row_tiles) *row_tiles = (int32_t)e->bA.size();
+        if (col_tiles) *col_tiles = (int32_t)e->bT.size();
     });
 }
 

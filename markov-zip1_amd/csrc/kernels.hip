@@ -257,7 +257,8 @@ __global__ __launch_bounds__(kFinal) void k_adapt_finalize(Dev d) {
         for (int j = d.inst_col[b]; j < d.inst_col[b + 1]; ++j) dxn += d.cnorm[j] * d.dx[j] * d.dx[j];
         for (int i = d.inst_row[b]; i < d.inst_row[b + 1]; ++i) adxn += (d.ax[i] * d.ax[i]) / d.rnorm[i];
     } else {
-        dxn = block_sum<kFinal / 64>(sum_strided(d.part_dxn, d.tT_begin[b], d.tT_begin[b + 1], 1), red);
+        const int32_t* pb = d.blocks ? d.bT_begin : d.tT_begin;  // part_dxn slots: tiles of the primal step
+        dxn = block_sum<kFinal / 64>(sum_strided(d.part_dxn, pb[b], pb[b + 1], 1), red);
         adxn = block_sum<kFinal / 64>(sum_strided(d.part_adx, d.tA_begin[b], d.tA_begin[b + 1], 1), red);
         if (threadIdx.x != 0) return;
     }
@@ -1102,6 +1103,10 @@ static inline int tile_grid(int ntiles) {
 static inline int elem_grid(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s) {
+    if (!exact && d.blocks) {
+        launch_primal_blk(d, t, adapt, s);
+        return;
+    }
     if (d.ntT == 0) return;
     const dim3 g(tile_grid(d.ntT)), b(kBlock);
     if (exact) {
@@ -1118,6 +1123,10 @@ void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s) {
 }
 
 void launch_dual(const Dev& d, int t, bool exact, hipStream_t s) {
+    if (!exact && d.blocks) {
+        launch_dual_blk(d, t, s);
+        return;
+    }
     if (d.ntA == 0) return;
     const dim3 g(tile_grid(d.ntA)), b(kBlock);
     if (exact)

@@ -1,0 +1,344 @@
+// stream.hip -- block-tile forms of the fused fast-order iteration kernels.
+//
+// A block tile is the task of one 256-thread workgroup: a run of consecutive
+// rows of one instance holding at most NB nonzeros (512, 1024 or 2048).  The
+// workgroup streams the tile's nonzeros as one contiguous range -- thread k
+// takes entries k, k + 256, ... -- so every index and value load is
+// coalesced, and all of a thread's loads and gathers are in flight together
+// (two dependent memory round trips per tile, whatever the row lengths).  The
+// products go to LDS; then G = 2^lg threads per row sum that row's products
+// (lane gl takes every G-th, the group reduces by xor-shuffle) and the
+// group's first lane applies the row's update.  The update's own operands are
+// loaded before the products, so their latency hides under the gathers.  A
+// row longer than NB is a tile of its own that the whole workgroup strides.
+//
+// The random gathers of x_bar / y are what bounds these kernels on sparse
+// matrices without locality: each one is a separate L2 request.  The wave
+// tasks of kernels.hip spend several dependent round trips per row group on
+// top of that; this form does not.  Reference reduction order keeps the wave
+// tasks (one lane per row sums in index order).
+//
+// The updates are those of k_primal / k_dual in kernels.hip (reference
+// src/lp/first_order/pdlp.cpp:195-206, 220-227, and the Halpern form).
+#include "dev.hpp"
+
+namespace mcp {
+
+constexpr int kBT = kBlock;  // threads per block tile
+
+// Workgroup -> tile so that each XCD (workgroups are dealt round-robin to the
+// 8 XCDs) takes a contiguous range of tiles; grid = kXcd * ceil(nt / kXcd).
+__device__ __forceinline__ int block_tile_id(int ntiles) {
+    const int per = (ntiles + kXcd - 1) / kXcd;
+    const int b = blockIdx.x;
+    return (b % kXcd) * per + b / kXcd;
+}
+
+// Products val[p] * vec[idx[p]] of the nonzeros [p0, p0 + len) into prod[0, len),
+// len <= K * kBT.  All K index/value loads are issued, then all K gathers.
+template <int K, typename TS, typename TA, typename TV>
+__device__ __forceinline__ void stage_products(int p0, int len, const int32_t* __restrict__ idx,
+                                               const TA* __restrict__ val, const TV* __restrict__ vec,
+                                               TS* prod) {
+    int ii[K];
+    TA aa[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        const bool in = q < len;
+        ii[k] = in ? idx[p0 + q] : 0;
+        aa[k] = in ? val[p0 + q] : (TA)0;
+    }
+    TV vv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        vv[k] = q < len ? vec[ii[k]] : (TV)0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        if (q < len) prod[q] = (TS)aa[k] * (TS)vv[k];
+    }
+}
+
+// Sum of one row's staged products [q0, q1) by its G lanes.  Groups are
+// aligned runs of G <= 64 threads, so a group lies inside one wave.  Every
+// thread must call (the shuffles need their partners); a thread with no row
+// passes q0 == q1.
+template <typename TS>
+__device__ __forceinline__ TS group_sum(const TS* prod, int q0, int q1, int gl, int G) {
+    TS s = 0;
+    for (int q = q0 + gl; q < q1; q += G) s += prod[q];
+    for (int off = G >> 1; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+// Dot product of one long row [p0, p1) by the whole workgroup (four
+// accumulators per thread, fixed-order block reduction); valid in thread 0.
+template <typename TS, typename TA, typename TV>
+__device__ __forceinline__ double long_row_dot(int p0, int p1, const int32_t* __restrict__ idx,
+                                               const TA* __restrict__ val, const TV* __restrict__ vec,
+                                               double* red) {
+    TS s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int p = p0 + (int)threadIdx.x;
+    for (; p + 3 * kBT < p1; p += 4 * kBT) {
+        const int i0 = idx[p], i1 = idx[p + kBT], i2 = idx[p + 2 * kBT], i3 = idx[p + 3 * kBT];
+        const TA a0 = val[p], a1 = val[p + kBT], a2 = val[p + 2 * kBT], a3 = val[p + 3 * kBT];
+        s0 += (TS)a0 * (TS)vec[i0];
+        s1 += (TS)a1 * (TS)vec[i1];
+        s2 += (TS)a2 * (TS)vec[i2];
+        s3 += (TS)a3 * (TS)vec[i3];
+    }
+    for (; p < p1; p += kBT) s0 += (TS)val[p] * (TS)vec[idx[p]];
+    return block_sum<kBT / 64>((double)((s0 + s1) + (s2 + s3)), red);
+}
+
+// Row sum of a block tile in the instance's precision: FP32 values and
+// gathers with FP32 products and sums, or FP64.  Every thread must call.
+template <int NB, typename TA32, typename TV32>
+__device__ __forceinline__ double tile_row_sum(bool f32, int p0, int len, int q0, int q1, int gl, int G,
+                                               const int32_t* __restrict__ idx, const double* __restrict__ val64,
+                                               const double* __restrict__ vec64, const TA32* __restrict__ val32,
+                                               const TV32* __restrict__ vec32, double* prod) {
+    if (f32) {
+        float* pr = reinterpret_cast<float*>(prod);
+        stage_products<NB / kBT, float>(p0, len, idx, val32, vec32, pr);
+        __syncthreads();
+        return (double)group_sum<float>(pr, q0, q1, gl, G);
+    }
+    stage_products<NB / kBT, double>(p0, len, idx, val64, vec64, prod);
+    __syncthreads();
+    return group_sum<double>(prod, q0, q1, gl, G);
+}
+
+// ---------------------------------------------------------------------------
+// Primal step over block tiles of A^T: z = A^T y, x+ = proj(x - tau (c + z)),
+// x_bar = 2 x+ - x, running average (or Halpern update); with ADAPT also dx
+// and the tile's partial of |dx|^2_colnorm.
+template <int NB, bool ADAPT>
+__global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
+    __shared__ double prod[NB];
+    __shared__ double red[kBT / 64];
+    const int tile = block_tile_id(d.nbT);
+    if (tile >= d.nbT) return;
+    const Tile tl = d.bT[tile];
+    const InstState& s = d.st[tl.inst];
+    if (s.status != kRunning) return;  // uniform over the workgroup
+    const int64_t iter = *d.iter_base + t;
+    const double eta = s.eta, omega = s.omega;
+    const double kavg = (double)(iter - s.restart_iter + 1);
+    const bool f32 = s.mode == kModeFp32;
+    const bool halp = d.halpern != 0;
+    const double w = kavg / (kavg + 1.0);
+    const double* xsrc = halp ? d.xlast : d.xavg;
+    const int tid = threadIdx.x;
+    double part = 0.0;
+    double c = 0, cn = 1, lo = 0, hi = 0, xo = 0, xa = 0;
+    auto update = [&](int j, double aty) {
+        const double g = c + aty;
+        const double tau = (eta / omega) / cn;
+        const double xp = clampv(xo - tau * g, lo, hi);
+        const double dd = xp - xo;
+        if (ADAPT) {
+            d.dx[j] = dd;
+            if (f32) d.dx32[j] = (float)dd;
+            part += cn * dd * dd;
+        }
+        const double xb = 2.0 * xp - xo;
+        d.xbar[j] = xb;
+        if (f32) d.xbar32[j] = (float)xb;
+        if (halp) {
+            d.xavg[j] = xp;
+            d.x[j] = w * xb + (1.0 - w) * xa;
+        } else {
+            d.x[j] = xp;
+            d.xavg[j] = xa + (xp - xa) / kavg;
+        }
+    };
+    auto load = [&](int j) {
+        c = d.c[j];
+        cn = d.cnorm[j];
+        lo = d.slo[j];
+        hi = d.shi[j];
+        xo = d.x[j];
+        xa = xsrc[j];  // running average, or the Halpern anchor
+    };
+    if (tl.shape & 8) {  // one long column, strided by the whole workgroup
+        const int j = tl.r0;
+        const double aty = f32 ? long_row_dot<float>(d.cp[j], d.cp[j + 1], d.ri, d.tv32, d.y32, red)
+                               : long_row_dot<double>(d.cp[j], d.cp[j + 1], d.ri, d.tv64, d.y, red);
+        if (tid == 0) {
+            load(j);
+            update(j, aty);
+            if (ADAPT) d.part_dxn[tile] = part;
+        }
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int j = tl.r0 + (tid >> lg), gl = tid & (G - 1);
+    const bool own = j < tl.r1;
+    const int p0 = d.cp[tl.r0], len = d.cp[tl.r1] - p0;
+    int q0 = 0, q1 = 0;
+    if (own) {
+        q0 = d.cp[j] - p0;
+        q1 = d.cp[j + 1] - p0;
+        if (gl == 0) load(j);
+    }
+    const double aty = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ri, d.tv64, d.y, d.tv32, d.y32, prod);
+    if (own && gl == 0) update(j, aty);
+    if (ADAPT) {
+        part = block_sum<kBT / 64>(part, red);
+        if (tid == 0) d.part_dxn[tile] = part;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dual step over block tiles of A: A x_bar, Moreau prox, running average (or
+// Halpern update); FP32 mode also refreshes the FP32 copy of y.
+template <int NB>
+__global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
+    __shared__ double prod[NB];
+    __shared__ double red[kBT / 64];
+    const int tile = block_tile_id(d.nbA);
+    if (tile >= d.nbA) return;
+    const Tile tl = d.bA[tile];
+    const InstState& s = d.st[tl.inst];
+    if (s.status != kRunning) return;
+    const int64_t iter = *d.iter_base + t;
+    const double eta = s.eta, omega = s.omega;
+    const double kavg = (double)(iter - s.restart_iter + 1);
+    const bool f32 = s.mode == kModeFp32;
+    const bool halp = d.halpern != 0;
+    const double w = kavg / (kavg + 1.0);
+    const double* ysrc = halp ? d.ylast : d.yavg;
+    const int tid = threadIdx.x;
+    double rn = 1, bl = 0, bh = 0, yo = 0, ya = 0;
+    auto load = [&](int i) {
+        rn = d.rnorm[i];
+        bl = d.blo[i];
+        bh = d.bhi[i];
+        yo = d.y[i];
+        ya = ysrc[i];
+    };
+    auto update = [&](int i, double ax) {
+        const double sigma = (eta * omega) / rn;
+        const double v = yo + sigma * ax;
+        const double yn = v - sigma * clampv(v / sigma, bl, bh);
+        double ynext;
+        if (halp) {
+            d.yavg[i] = yn;
+            ynext = w * (2.0 * yn - yo) + (1.0 - w) * ya;
+        } else {
+            ynext = yn;
+            d.yavg[i] = ya + (yn - ya) / kavg;
+        }
+        d.y[i] = ynext;
+        if (f32) d.y32[i] = (float)ynext;
+    };
+    if (tl.shape & 8) {
+        const int i = tl.r0;
+        const double ax = f32 ? long_row_dot<float>(d.rp[i], d.rp[i + 1], d.ci, d.av32, d.xbar32, red)
+                              : long_row_dot<double>(d.rp[i], d.rp[i + 1], d.ci, d.av64, d.xbar, red);
+        if (tid == 0) {
+            load(i);
+            update(i, ax);
+        }
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int i = tl.r0 + (tid >> lg), gl = tid & (G - 1);
+    const bool own = i < tl.r1;
+    const int p0 = d.rp[tl.r0], len = d.rp[tl.r1] - p0;
+    int q0 = 0, q1 = 0;
+    if (own) {
+        q0 = d.rp[i] - p0;
+        q1 = d.rp[i + 1] - p0;
+        if (gl == 0) load(i);
+    }
+    const double ax = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.xbar, d.av32, d.xbar32, prod);
+    if (own && gl == 0) update(i, ax);
+}
+
+// ---------------------------------------------------------------------------
+// Plain SpMV over block tiles: out[r] = sum val * vec (tests, autotuning).
+template <typename T, int NB>
+__global__ __launch_bounds__(kBT) void k_spmv_blk(const Tile* tiles, int ntiles, const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                                  const T* __restrict__ vec, T* __restrict__ out) {
+    __shared__ T prod[NB];
+    __shared__ double red[kBT / 64];
+    const int tile = block_tile_id(ntiles);
+    if (tile >= ntiles) return;
+    const Tile tl = tiles[tile];
+    if (tl.shape & 8) {
+        const double v = long_row_dot<T>(ptr[tl.r0], ptr[tl.r0 + 1], idx, val, vec, red);
+        if (threadIdx.x == 0) out[tl.r0] = (T)v;
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int r = tl.r0 + ((int)threadIdx.x >> lg), gl = (int)threadIdx.x & (G - 1);
+    const bool own = r < tl.r1;
+    const int p0 = ptr[tl.r0], len = ptr[tl.r1] - p0;
+    const int q0 = own ? ptr[r] - p0 : 0, q1 = own ? ptr[r + 1] - p0 : 0;
+    stage_products<NB / kBT, T>(p0, len, idx, val, vec, prod);
+    __syncthreads();
+    const T v = group_sum<T>(prod, q0, q1, gl, G);
+    if (own && gl == 0) out[r] = v;
+}
+
+// ---------------------------------------------------------------------------
+static inline int block_grid(int ntiles) { return kXcd * ((ntiles + kXcd - 1) / kXcd); }
+
+template <bool ADAPT>
+static void primal_blk(const Dev& d, int t, hipStream_t s) {
+    const dim3 g(block_grid(d.nbT)), b(kBT);
+    if (d.budT == 512)
+        hipLaunchKernelGGL((k_primal_blk<512, ADAPT>), g, b, 0, s, d, t);
+    else if (d.budT == 2048)
+        hipLaunchKernelGGL((k_primal_blk<2048, ADAPT>), g, b, 0, s, d, t);
+    else
+        hipLaunchKernelGGL((k_primal_blk<1024, ADAPT>), g, b, 0, s, d, t);
+}
+
+void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s) {
+    if (d.nbT == 0) return;
+    if (adapt)
+        primal_blk<true>(d, t, s);
+    else
+        primal_blk<false>(d, t, s);
+}
+
+void launch_dual_blk(const Dev& d, int t, hipStream_t s) {
+    if (d.nbA == 0) return;
+    const dim3 g(block_grid(d.nbA)), b(kBT);
+    if (d.budA == 512)
+        hipLaunchKernelGGL((k_dual_blk<512>), g, b, 0, s, d, t);
+    else if (d.budA == 2048)
+        hipLaunchKernelGGL((k_dual_blk<2048>), g, b, 0, s, d, t);
+    else
+        hipLaunchKernelGGL((k_dual_blk<1024>), g, b, 0, s, d, t);
+}
+
+template <typename T>
+static void spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
+                     const T* val, const T* vec, T* out, hipStream_t s) {
+    const dim3 g(block_grid(ntiles)), b(kBT);
+    if (budget == 512)
+        hipLaunchKernelGGL((k_spmv_blk<T, 512>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+    else if (budget == 2048)
+        hipLaunchKernelGGL((k_spmv_blk<T, 2048>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+    else
+        hipLaunchKernelGGL((k_spmv_blk<T, 1024>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+}
+
+void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
+                     const void* val, const void* vec, void* out, bool fp32, hipStream_t s) {
+    if (ntiles == 0) return;
+    if (fp32)
+        spmv_blk<float>(tiles, ntiles, budget, ptr, idx, (const float*)val, (const float*)vec, (float*)out, s);
+    else
+        spmv_blk<double>(tiles, ntiles, budget, ptr, idx, (const double*)val, (const double*)vec, (double*)out, s);
+}
+
+}  // namespace mcp

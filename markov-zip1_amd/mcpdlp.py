@@ -208,6 +208,8 @@ def engine_lib():
         lib.mcp_get_tiling.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy), P_i32, P_i32]
         lib.mcp_time_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P_f64, P_f64]
         lib.mcp_set_tile_policy.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy)]
+        lib.mcp_set_block_budget.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        lib.mcp_get_block_tiling.argtypes = [C.c_void_p, P_i32, P_i32, P_i32, P_i32]
         _engine = lib
     return _engine
 
@@ -612,6 +614,17 @@ class Engine:
         na, nt = C.c_int32(), C.c_int32()
         _check_eng(self.lib.mcp_get_tiling(self.h, C.byref(a), C.byref(t), C.byref(na), C.byref(nt)))
         return (a.nnz_budget, a.per_lane, na.value), (t.nnz_budget, t.per_lane, nt.value)
+
+    def set_block_budget(self, rows=1024, cols=1024):
+        """Nonzeros per block tile of the fused fast-order steps (512/1024/2048),
+        or 0 for both to run those steps on the wave tiles."""
+        _check_eng(self.lib.mcp_set_block_budget(self.h, int(rows), int(cols)))
+
+    def block_tiling(self):
+        """((rows budget, row tiles), (cols budget, col tiles)) of the block tiles."""
+        ba, bt, na, nt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check_eng(self.lib.mcp_get_block_tiling(self.h, C.byref(ba), C.byref(bt), C.byref(na), C.byref(nt)))
+        return (ba.value, na.value), (bt.value, nt.value)
 
     KERNELS = {"primal_fp32": 0, "dual_fp32": 1, "primal_fp64": 2, "dual_fp64": 3,
                "spmv_fp32": 4, "spmv_t_fp32": 5, "copy": 6}

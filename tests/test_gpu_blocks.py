@@ -1,0 +1,132 @@
+"""Block tiles of the fused fast-order steps (one 256-thread workgroup streams
+a run of rows, the products are summed per row from LDS) against the oracle
+and against the wave-task form of the same steps.
+
+Bars:
+* SpMV over block tiles of every budget (512, 1024, 2048 nonzeros), including
+  rows longer than the budget in both orientations: FP64 within 1e-13 of the
+  oracle's serial sum, FP32 within 2e-6 of the row's absolute sum;
+* full solves (Halpern scheme, FP64 and mixed precision, single LPs and a
+  batch that is compacted as its instances finish) with the block tiles on
+  and off: all reach 1e-6, the returned pairs re-scored by the oracle's FP64
+  residuals are <= 1.5e-6, and the objectives agree to 1e-5 of the size of
+  the objective's terms (a KKT score of 1e-6 bounds residuals, not the
+  objective error itself); one Zipf LP, which needs far more iterations,
+  runs to 1e-4 with the bars scaled alike.
+"""
+import numpy as np
+import pytest
+
+import mcpdlp as mp
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def dense_lp(m, n, seed):
+    """Dense rows a_i x <= b_i with b = A x0 + slack for an interior x0, box
+    0 <= x <= 10, random costs: feasible and bounded by construction."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.1, 10.0, (m, n)) * rng.choice([-1.0, 1.0], (m, n))
+    x0 = rng.uniform(1.0, 9.0, n)
+    b = A @ x0 + rng.uniform(0.5, 2.0, m)
+    c = rng.standard_normal(n)
+    return mp.LPData.from_dense(A, c, [-np.inf] * m, b, [0.0] * n, [10.0] * n)
+
+
+def uniform():
+    return mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11)
+
+
+def mixed():
+    return mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12)
+
+
+def long_rows():  # every row of A holds 2500 nonzeros: longer than any budget
+    return dense_lp(3, 2500, 1)
+
+
+def long_cols():  # every column (row of A^T) holds 2500 nonzeros
+    return dense_lp(2500, 6, 2)
+
+
+MAKERS = [uniform, mixed, long_rows, long_cols]
+
+
+@pytest.mark.parametrize("budget", [512, 1024, 2048])
+@pytest.mark.parametrize("maker", MAKERS)
+def test_block_spmv_parity(budget, maker):
+    lp = maker()
+    print("block spmv", maker.__name__, budget, lp.rows, lp.cols, lp.nnz, flush=True)
+    e = mp.Engine(0)
+    e.load(lp)
+    e.setup(ruiz=False)
+    e.set_block_budget(budget, budget)
+    (ba, na), (bt, nt) = e.block_tiling()
+    assert ba == bt == budget and na > 0 and nt > 0
+    rng = np.random.default_rng(budget)
+    x = rng.standard_normal(lp.cols)
+    x[rng.random(lp.cols) < 0.2] = 0.0
+    y = rng.standard_normal(lp.rows)
+    ax, at = orc.spmv(lp, x), orc.spmv_t(lp, y)
+    f, _ = e.spmv(x)
+    ft, _ = e.spmv(y, transpose=True)
+    np.testing.assert_allclose(f, ax, rtol=1e-13, atol=1e-13 * np.abs(ax).max())
+    np.testing.assert_allclose(ft, at, rtol=1e-13, atol=1e-13 * np.abs(at).max())
+    A = np.abs(lp.dense())
+    s32, _ = e.spmv(x, fp32=True)
+    assert np.all(np.abs(s32 - ax) <= (A @ np.abs(x)) * 2e-6 + 1e-30)
+    t32, _ = e.spmv(y, transpose=True, fp32=True)
+    assert np.all(np.abs(t32 - at) <= (A.T @ np.abs(y)) * 2e-6 + 1e-30)
+    e.close()
+
+
+def engine_solve(lps, budget, precision, tol=1e-6):
+    e = mp.Engine(0)
+    e.load(lps)
+    e.setup()
+    e.set_block_budget(budget, budget)
+    e.solve(tolerance=tol, max_iterations=200000, precision=precision, restart_scheme=mp.SCHEME_HALPERN)
+    out, _ = e.fetch()
+    e.close()
+    return out
+
+
+def check_pair(lp, r, tol=1e-6):
+    assert r["status"] == "optimal", r["status"]
+    k = orc.kkt(lp, r["x"], r["y"])
+    assert k["score"] <= 1.5 * tol, k
+
+
+# The oracle (Halpern scheme, FP64) converges on these in 4640 (uniform, 1e-6),
+# 21280 (mixed, 1e-4), 3840 (long_rows, 1e-6) and 8320 (long_cols, 1e-6) iterations.
+SOLVE_CASES = [(uniform, 1e-6), (mixed, 1e-4), (long_rows, 1e-6), (long_cols, 1e-6)]
+
+
+@pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("maker,tol", SOLVE_CASES, ids=[c[0].__name__ for c in SOLVE_CASES])
+def test_block_and_wave_steps_reach_the_same_optimum(maker, tol, precision):
+    lp = maker()
+    print("block solve", maker.__name__, precision, flush=True)
+    runs = {b: engine_solve(lp, b, precision, tol)[0] for b in (0, 512, 1024)}
+    for r in runs.values():
+        check_pair(lp, r, tol)
+    scale = 1.0 + np.abs(lp.objective * runs[0]["x"]).sum()
+    for b in (512, 1024):
+        assert abs(runs[b]["objective"] - runs[0]["objective"]) <= 10 * tol * scale
+
+
+def test_block_batch_with_compaction():
+    """Instances of different sizes finish at different iterations, so the
+    batch is re-tiled over the running instances (block tiles included)
+    several times; each instance must still converge on its own."""
+    lps = ([mp.generate(mp.FAMILY_UNIFORM_EQ, rows=100 + 37 * k, cols=250 + 90 * k, seed=100 + k) for k in range(5)]
+           + [long_rows(), long_cols()])
+    print("block batch", len(lps), flush=True)
+    on = engine_solve(lps, 1024, mp.PRECISION_FP64)
+    off = engine_solve(lps, 0, mp.PRECISION_FP64)
+    for lp, a, b in zip(lps, on, off):
+        check_pair(lp, a)
+        check_pair(lp, b)
+        scale = 1.0 + np.abs(lp.objective * b["x"]).sum()
+        assert abs(a["objective"] - b["objective"]) <= 1e-5 * scale
