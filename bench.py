@@ -217,8 +217,9 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-KERNEL_SYMBOL = {"primal_fp32": "k_primal", "dual_fp32": "k_dual", "primal_fp64": "k_primal",
-                 "dual_fp64": "k_dual", "spmv_fp32": "k_spmv_plain", "spmv_t_fp32": "k_spmv_plain", "copy": "k_copy"}
+KERNEL_SYMBOL = {"primal_fp32": "k_primal_blk", "dual_fp32": "k_dual_blk", "primal_fp64": "k_primal_blk",
+                 "dual_fp64": "k_dual_blk", "spmv_fp32": "k_spmv_plain", "spmv_t_fp32": "k_spmv_plain",
+                 "copy": "k_copy"}
 
 
 def time_kernels(mp, lp, kinds, autotune, reps=50):
@@ -234,6 +235,8 @@ def time_kernels(mp, lp, kinds, autotune, reps=50):
         log(f"[bench] kernel {name} ({KERNEL_SYMBOL[name]}): {us:.2f} us/launch, {b / 1e6:.2f} MB algorithmic "
             f"-> {b / us / 1e3:.0f} GB/s")
     tiling = eng.tiling()
+    (ba, na), (bt, nt) = eng.block_tiling()
+    tiling = (tiling[0] + (ba, na), tiling[1] + (bt, nt))  # wave policy + tiles, block budget + tiles
     eng.close()
     return kern, tiling
 

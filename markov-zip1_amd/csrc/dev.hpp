@@ -63,7 +63,7 @@ struct Dev {
     double* aty2;  // [n]
     // reduction partials (one slot group per tile)
     double* part_dxn;   // [ntT], or [nbT] when the primal step runs on block tiles
-    double* part_adx;   // [ntA]
+    double* part_adx;   // [ntA], or [nbA] when A dx runs on block tiles
     // check partials per candidate c (0 average, 1 current iterate):
     double* part_rows;  // [(c*3 + k)*ntA + tile]: max violation, dual-objective part, |y - ylast|^2
     double* part_cols;  // [(c*5 + k)*ntT + tile]: max violation, dual residual^2, primal obj,
@@ -90,8 +90,9 @@ struct Dev {
     int32_t budA, budT;        // nonzeros per block tile: 512, 1024 or 2048
     int32_t halpern;    // reflected Halpern iterates: the average slots hold T(z), the
                         // last-restart slots the anchor z0
-    int32_t blocks;     // 1: the fast-order primal/dual steps run on the block tiles
-                        // (part_dxn then holds one slot per tile of bT); 0: on tA / tT
+    int32_t blocks;     // 1: the fast-order primal/dual steps and A dx run on the block
+                        // tiles (part_dxn / part_adx then hold one slot per tile of bT /
+                        // bA); 0: on tA / tT
 };
 
 // ---- launch helpers (kernels.hip) ----
@@ -128,6 +129,7 @@ void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const 
 // plain SpMV over block tiles of `budget` (512, 1024 or 2048) nonzeros.
 void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s);
 void launch_dual_blk(const Dev& d, int t, hipStream_t s);
+void launch_adx_blk(const Dev& d, int t, hipStream_t s);
 void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
                      const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
 

@@ -290,7 +290,7 @@ struct mcp_engine {
         d.blocks = blk ? 1 : 0;
         // partial-sum slots follow the tile counts
         d.part_dxn = buf<double>("part_dxn", std::max(tT.size(), bT.size()));
-        d.part_adx = buf<double>("part_adx", tA.size());
+        d.part_adx = buf<double>("part_adx", std::max(tA.size(), bA.size()));
         d.part_rows = buf<double>("part_rows", 6 * tA.size());
         d.part_cols = buf<double>("part_cols", 10 * tT.size());
         d.part_ray_rows = buf<double>("part_ray_rows", 5 * tA.size());

@@ -259,7 +259,8 @@ __global__ __launch_bounds__(kFinal) void k_adapt_finalize(Dev d) {
     } else {
         const int32_t* pb = d.blocks ? d.bT_begin : d.tT_begin;  // part_dxn slots: tiles of the primal step
         dxn = block_sum<kFinal / 64>(sum_strided(d.part_dxn, pb[b], pb[b + 1], 1), red);
-        adxn = block_sum<kFinal / 64>(sum_strided(d.part_adx, d.tA_begin[b], d.tA_begin[b + 1], 1), red);
+        const int32_t* pa = d.blocks ? d.bA_begin : d.tA_begin;  // part_adx slots: tiles of A dx
+        adxn = block_sum<kFinal / 64>(sum_strided(d.part_adx, pa[b], pa[b + 1], 1), red);
         if (threadIdx.x != 0) return;
     }
     adapt_decide(s, dxn, adxn);
@@ -1136,6 +1137,10 @@ void launch_dual(const Dev& d, int t, bool exact, hipStream_t s) {
 }
 
 void launch_adx(const Dev& d, int t, bool exact, hipStream_t s) {
+    if (!exact && d.blocks) {
+        launch_adx_blk(d, t, s);
+        return;
+    }
     if (d.ntA == 0) return;
     const dim3 g(tile_grid(d.ntA)), b(kBlock);
     if (exact)

@@ -1,4 +1,5 @@
-// stream.hip -- block-tile forms of the fused fast-order iteration kernels.
+// stream.hip -- block-tile forms of the fast-order iteration kernels (the
+// fused primal and dual steps and A dx of the adaptive step size).
 //
 // A block tile is the task of one 256-thread workgroup: a run of consecutive
 // rows of one instance holding at most NB nonzeros (512, 1024 or 2048).  The
@@ -261,6 +262,43 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
 }
 
 // ---------------------------------------------------------------------------
+// A dx for the adaptive step size (pdlp.cpp:208-210) over block tiles of A:
+// the tile's partial of |A dx|^2_{1/rownorm}.
+template <int NB>
+__global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
+    __shared__ double prod[NB];
+    __shared__ double red[kBT / 64];
+    const int tile = block_tile_id(d.nbA);
+    if (tile >= d.nbA) return;
+    const Tile tl = d.bA[tile];
+    const InstState& s = d.st[tl.inst];
+    if (s.status != kRunning) return;
+    const bool f32 = s.mode == kModeFp32;
+    const int tid = threadIdx.x;
+    if (tl.shape & 8) {
+        const int i = tl.r0;
+        const double a = f32 ? long_row_dot<float>(d.rp[i], d.rp[i + 1], d.ci, d.av32, d.dx32, red)
+                             : long_row_dot<double>(d.rp[i], d.rp[i + 1], d.ci, d.av64, d.dx, red);
+        if (tid == 0) d.part_adx[tile] = (a * a) / d.rnorm[i];
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int i = tl.r0 + (tid >> lg), gl = tid & (G - 1);
+    const bool own = i < tl.r1;
+    const int p0 = d.rp[tl.r0], len = d.rp[tl.r1] - p0;
+    int q0 = 0, q1 = 0;
+    double rn = 1.0;
+    if (own) {
+        q0 = d.rp[i] - p0;
+        q1 = d.rp[i + 1] - p0;
+        if (gl == 0) rn = d.rnorm[i];
+    }
+    const double a = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.dx, d.av32, d.dx32, prod);
+    const double part = block_sum<kBT / 64>((own && gl == 0) ? (a * a) / rn : 0.0, red);
+    if (tid == 0) d.part_adx[tile] = part;
+}
+
+// ---------------------------------------------------------------------------
 // Plain SpMV over block tiles: out[r] = sum val * vec (tests, autotuning).
 template <typename T, int NB>
 __global__ __launch_bounds__(kBT) void k_spmv_blk(const Tile* tiles, int ntiles, const int32_t* __restrict__ ptr,
@@ -318,6 +356,17 @@ void launch_dual_blk(const Dev& d, int t, hipStream_t s) {
         hipLaunchKernelGGL((k_dual_blk<2048>), g, b, 0, s, d, t);
     else
         hipLaunchKernelGGL((k_dual_blk<1024>), g, b, 0, s, d, t);
+}
+
+void launch_adx_blk(const Dev& d, int t, hipStream_t s) {
+    if (d.nbA == 0) return;
+    const dim3 g(block_grid(d.nbA)), b(kBT);
+    if (d.budA == 512)
+        hipLaunchKernelGGL((k_adx_blk<512>), g, b, 0, s, d, t);
+    else if (d.budA == 2048)
+        hipLaunchKernelGGL((k_adx_blk<2048>), g, b, 0, s, d, t);
+    else
+        hipLaunchKernelGGL((k_adx_blk<1024>), g, b, 0, s, d, t);
 }
 
 template <typename T>
