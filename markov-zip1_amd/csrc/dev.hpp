@@ -65,9 +65,10 @@ struct Dev {
     double* part_dxn;   // [ntT], or [nbT] when the primal step runs on block tiles
     double* part_adx;   // [ntA], or [nbA] when A dx runs on block tiles
     // check partials per candidate c (0 average, 1 current iterate):
-    double* part_rows;  // [(c*3 + k)*ntA + tile]: max violation, dual-objective part, |y - ylast|^2
-    double* part_cols;  // [(c*5 + k)*ntT + tile]: max violation, dual residual^2, primal obj,
-                        //                          dual-objective part, |x - xlast|^2
+    // (tiles = ntA / ntT, or nbA / nbT when the check runs on block tiles)
+    double* part_rows;  // [(c*3 + k)*tiles + tile]: max violation, dual-objective part, |y - ylast|^2
+    double* part_cols;  // [(c*5 + k)*tiles + tile]: max violation, dual residual^2, primal obj,
+                        //                            dual-objective part, |x - xlast|^2
     // infeasibility detection: the candidate's movement since the last
     // restart and its per-tile certificate partials
     double* ray_x;      // [n]
@@ -90,9 +91,10 @@ struct Dev {
     int32_t budA, budT;        // nonzeros per block tile: 512, 1024 or 2048
     int32_t halpern;    // reflected Halpern iterates: the average slots hold T(z), the
                         // last-restart slots the anchor z0
-    int32_t blocks;     // 1: the fast-order primal/dual steps and A dx run on the block
-                        // tiles (part_dxn / part_adx then hold one slot per tile of bT /
-                        // bA); 0: on tA / tT
+    int32_t blocks;     // 1: the fast-order primal/dual steps, A dx and the check's
+                        // residual terms run on the block tiles (part_dxn, part_adx,
+                        // part_rows, part_cols then hold slots per tile of bT / bA);
+                        // 0: on tA / tT
 };
 
 // ---- launch helpers (kernels.hip) ----
@@ -103,6 +105,9 @@ void launch_dual(const Dev& d, int t, bool exact, hipStream_t s);
 void launch_adx(const Dev& d, int t, bool exact, hipStream_t s);
 void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s);
 void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s);
+// The check's residual terms: the average and, with `both`, the current
+// iterate (block tiles in one pass when the engine uses them).
+void launch_check_resid(const Dev& d, bool exact, bool both, hipStream_t s);
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
                            hipStream_t s);
 void launch_post_check(const Dev& d, bool kkt, hipStream_t s);
@@ -130,6 +135,9 @@ void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const 
 void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s);
 void launch_dual_blk(const Dev& d, int t, hipStream_t s);
 void launch_adx_blk(const Dev& d, int t, hipStream_t s);
+// Residual terms of the check's candidates over block tiles in one pass:
+// the average only, or (both) the average and the current iterate.
+void launch_resid_blk(const Dev& d, bool both, hipStream_t s);
 void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
                      const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
 

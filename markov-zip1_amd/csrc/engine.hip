@@ -291,8 +291,8 @@ struct mcp_engine {
         // partial-sum slots follow the tile counts
         d.part_dxn = buf<double>("part_dxn", std::max(tT.size(), bT.size()));
         d.part_adx = buf<double>("part_adx", std::max(tA.size(), bA.size()));
-        d.part_rows = buf<double>("part_rows", 6 * tA.size());
-        d.part_cols = buf<double>("part_cols", 10 * tT.size());
+        d.part_rows = buf<double>("part_rows", 6 * std::max(tA.size(), bA.size()));
+        d.part_cols = buf<double>("part_cols", 10 * std::max(tT.size(), bT.size()));
         d.part_ray_rows = buf<double>("part_ray_rows", 5 * tA.size());
         d.part_ray_cols = buf<double>("part_ray_cols", 7 * tT.size());
         d.tA = dA;
@@ -456,7 +456,7 @@ void enqueue_chunk(mcp_engine* e, const mcp::IterOpts& io, bool adaptive, bool e
         }
         mcp::launch_dual(d, t, exact, s);
         if ((it + 1) % io.restart_every == 0) {
-            for (int cur = 0; cur <= io.kkt; ++cur) mcp::launch_resid(d, exact, cur != 0, s);
+            mcp::launch_check_resid(d, exact, io.kkt != 0, s);
             mcp::launch_check_finalize(d, io, t, exact, false, s);
             if (io.detect) mcp::launch_ray_check(d, io, t, s);
             mcp::launch_post_check(d, io.kkt != 0, s);
@@ -827,7 +827,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         const int32_t still_running = chunks > 0 ? pin[2 * last] : e->B;
         if (!finished && still_running > 0) {
             // iteration limit: residuals of the final average (pdlp.cpp:263-265)
-            for (int cur = 0; cur <= io.kkt; ++cur) mcp::launch_resid(d, exact, cur != 0, s);
+            mcp::launch_check_resid(d, exact, io.kkt != 0, s);
             mcp::launch_check_finalize(d, io, -1, exact, true, s);
             HIPCHK(hipGetLastError());
         }

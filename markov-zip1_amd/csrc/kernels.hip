@@ -478,10 +478,13 @@ __device__ void partial_residuals(const Dev& d, int b, int cand, double c_scale,
                                   double* red) {
     // One coalesced pass over the field-major tile partials of this
     // instance, every field accumulated in registers, one combined reduction.
-    const int ntA = d.ntA, ntT = d.ntT;
+    // slots per tile of the tiles the check ran on (block or wave tiles)
+    const int ntA = d.blocks ? d.nbA : d.ntA, ntT = d.blocks ? d.nbT : d.ntT;
+    const int32_t* ba = d.blocks ? d.bA_begin : d.tA_begin;
+    const int32_t* bt = d.blocks ? d.bT_begin : d.tT_begin;
     const double* pr = d.part_rows + (cand ? 3 * ntA : 0);
     const double* pc = d.part_cols + (cand ? 5 * ntT : 0);
-    const int a0 = d.tA_begin[b], a1 = d.tA_begin[b + 1], t0 = d.tT_begin[b], t1 = d.tT_begin[b + 1];
+    const int a0 = ba[b], a1 = ba[b + 1], t0 = bt[b], t1 = bt[b + 1];
     constexpr int NW = kFinal / 64;
     double mp = 0.0, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dr, ym, dsq, po, dc, xm
     for (int i = a0 + (int)threadIdx.x; i < a1; i += kFinal) {
@@ -1181,6 +1184,15 @@ void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s) {
         resid_pair<true>(d, exact, s);
     else
         resid_pair<false>(d, exact, s);
+}
+
+void launch_check_resid(const Dev& d, bool exact, bool both, hipStream_t s) {
+    if (!exact && d.blocks) {
+        launch_resid_blk(d, both, s);
+        return;
+    }
+    launch_resid(d, exact, false, s);
+    if (both) launch_resid(d, exact, true, s);
 }
 
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,

@@ -1,5 +1,6 @@
 // stream.hip -- block-tile forms of the fast-order iteration kernels (the
-// fused primal and dual steps and A dx of the adaptive step size).
+// fused primal and dual steps, A dx of the adaptive step size and the FP64
+// residual terms of the check).
 //
 // A block tile is the task of one 256-thread workgroup: a run of consecutive
 // rows of one instance holding at most NB nonzeros (512, 1024 or 2048).  The
@@ -299,6 +300,213 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
 }
 
 // ---------------------------------------------------------------------------
+// FP64 check over block tiles: the residual terms of NC candidates -- the
+// running average (Halpern: T(z)) and, with NC == 2, the current iterate --
+// from one pass over the matrix (one index/value stream, one gather per
+// candidate), on the unscaled model (pdlp.cpp:76-131).  The tile partials go
+// to the same field-major slots as the wave-task check kernels of kernels.hip,
+// [(c * fields + k) * tiles + tile], with tiles = nbA / nbT.
+
+// Products of the tile's nonzeros with NC gathered vectors into prod[c * NB + q].
+template <int K, int NC>
+__device__ __forceinline__ void stage_products_multi(int p0, int len, const int32_t* __restrict__ idx,
+                                                     const double* __restrict__ val, const double* const* vecs,
+                                                     double* prod) {
+    constexpr int NB = K * kBT;
+    int ii[K];
+    double aa[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        const bool in = q < len;
+        ii[k] = in ? idx[p0 + q] : 0;
+        aa[k] = in ? val[p0 + q] : 0.0;
+    }
+    double vv[NC][K];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int q = (int)threadIdx.x + k * kBT;
+            vv[c][k] = q < len ? vecs[c][ii[k]] : 0.0;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int q = (int)threadIdx.x + k * kBT;
+            if (q < len) prod[c * NB + q] = aa[k] * vv[c][k];
+        }
+}
+
+// Rows: A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
+template <int NB, int NC>
+__global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
+    __shared__ double prod[NC * NB];
+    __shared__ double red[(kBT / 64) * 3];
+    const int tile = block_tile_id(d.nbA);
+    if (tile >= d.nbA) return;
+    const Tile tl = d.bA[tile];
+    const InstState& s = d.st[tl.inst];
+    if (s.status != kRunning) return;
+    const double* xs[2] = {d.xavg, d.x};
+    const double* ys[2] = {d.yavg, d.y};
+    const int tid = threadIdx.x;
+    double rs = 1, lo = 0, hi = 0, yl = 0, yc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) yc[c] = 0.0;
+    auto load = [&](int i) {
+        rs = d.row_scale[i];
+        lo = d.orow_lo[i];
+        hi = d.orow_hi[i];
+        yl = d.ylast[i];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) yc[c] = ys[c][i];
+    };
+    // residual terms of one row for candidate c: (violation, dual-objective part, movement^2)
+    auto terms = [&](int c, double ax, double& viol, double& dob, double& mv) {
+        const double axu = ax / rs;
+        const double proj = clampv(axu, lo, hi);
+        const double r = fabs(axu - proj);
+        const double sc = 1.0 + maxv(fabs(axu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
+        viol = r / sc;
+        const double yu = yc[c] * rs;
+        dob = (yu > 0.0 && hi < 1e299) ? -(yu * hi) : ((yu < 0.0 && lo > -1e299) ? -(yu * lo) : 0.0);
+        const double dy = yc[c] - yl;
+        mv = dy * dy;
+    };
+    auto store = [&](int c, double viol, double dob, double mv) {
+        double* out = d.part_rows + c * 3 * d.nbA + tile;
+        out[0] = viol;
+        out[d.nbA] = dob;
+        out[2 * d.nbA] = mv;
+    };
+    if (tl.shape & 8) {
+        const int i = tl.r0;
+        double ax[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) ax[c] = long_row_dot<double>(d.rp[i], d.rp[i + 1], d.ci, d.av64, xs[c], red);
+        if (tid == 0) {
+            load(i);
+            for (int c = 0; c < NC; ++c) {
+                double v, ob, mv;
+                terms(c, ax[c], v, ob, mv);
+                store(c, v, ob, mv);
+            }
+        }
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int i = tl.r0 + (tid >> lg), gl = tid & (G - 1);
+    const bool own = i < tl.r1;
+    const int p0 = d.rp[tl.r0], len = d.rp[tl.r1] - p0;
+    int q0 = 0, q1 = 0;
+    if (own) {
+        q0 = d.rp[i] - p0;
+        q1 = d.rp[i + 1] - p0;
+        if (gl == 0) load(i);
+    }
+    stage_products_multi<NB / kBT, NC>(p0, len, d.ci, d.av64, xs, prod);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const double ax = group_sum<double>(prod + c * NB, q0, q1, gl, G);
+        double v = 0.0, ob[2] = {0.0, 0.0};
+        if (own && gl == 0) terms(c, ax, v, ob[0], ob[1]);
+        block_max_sums<kBT / 64, 2>(v, ob, red);
+        if (tid == 0) store(c, v, ob[0], ob[1]);
+    }
+}
+
+// Columns: A^T y_c, bound violation, dual residual, primal/dual objective
+// parts, |x_c - x_last|^2.
+template <int NB, int NC>
+__global__ __launch_bounds__(kBT) void k_resid_cols_blk(Dev d) {
+    __shared__ double prod[NC * NB];
+    __shared__ double red[(kBT / 64) * 5];
+    const int tile = block_tile_id(d.nbT);
+    if (tile >= d.nbT) return;
+    const Tile tl = d.bT[tile];
+    const InstState& s = d.st[tl.inst];
+    if (s.status != kRunning) return;
+    const double* xs[2] = {d.xavg, d.x};
+    const double* ys[2] = {d.yavg, d.y};
+    const double sense = d.sense[tl.inst];
+    const int tid = threadIdx.x;
+    double cs = 1, lo = 0, hi = 0, ob = 0, xl = 0, xc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) xc[c] = 0.0;
+    auto load = [&](int j) {
+        cs = d.col_scale[j];
+        lo = d.ovar_lo[j];
+        hi = d.ovar_hi[j];
+        ob = d.obj[j];
+        xl = d.xlast[j];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xc[c] = xs[c][j];
+    };
+    // (violation; dual residual^2, primal objective, dual-objective part, movement^2)
+    auto terms = [&](int c, double aty, double& viol, double (&t)[4]) {
+        const double xu = xc[c] * cs;
+        const double proj = clampv(xu, lo, hi);
+        const double r = fabs(xu - proj);
+        const double sc = 1.0 + maxv(fabs(xu), fabs(proj) < 1e299 ? fabs(proj) : 0.0);
+        viol = r / sc;
+        const double g = sense * ob + aty / cs;
+        const double xp = clampv(xu - g, lo, hi);
+        const double df = xu - xp;
+        t[0] = df * df;
+        t[1] = ob * xu;
+        t[2] = (g > 0.0 && lo > -1e299) ? g * lo : ((g < 0.0 && hi < 1e299) ? g * hi : 0.0);
+        const double dxm = xc[c] - xl;
+        t[3] = dxm * dxm;
+    };
+    auto store = [&](int c, double viol, const double (&t)[4]) {
+        double* out = d.part_cols + c * 5 * d.nbT + tile;
+        out[0] = viol;
+        out[d.nbT] = t[0];
+        out[2 * d.nbT] = t[1];
+        out[3 * d.nbT] = t[2];
+        out[4 * d.nbT] = t[3];
+    };
+    if (tl.shape & 8) {
+        const int j = tl.r0;
+        double aty[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) aty[c] = long_row_dot<double>(d.cp[j], d.cp[j + 1], d.ri, d.tv64, ys[c], red);
+        if (tid == 0) {
+            load(j);
+            for (int c = 0; c < NC; ++c) {
+                double v, t[4];
+                terms(c, aty[c], v, t);
+                store(c, v, t);
+            }
+        }
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int j = tl.r0 + (tid >> lg), gl = tid & (G - 1);
+    const bool own = j < tl.r1;
+    const int p0 = d.cp[tl.r0], len = d.cp[tl.r1] - p0;
+    int q0 = 0, q1 = 0;
+    if (own) {
+        q0 = d.cp[j] - p0;
+        q1 = d.cp[j + 1] - p0;
+        if (gl == 0) load(j);
+    }
+    stage_products_multi<NB / kBT, NC>(p0, len, d.ri, d.tv64, ys, prod);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const double aty = group_sum<double>(prod + c * NB, q0, q1, gl, G);
+        double v = 0.0, t[4] = {0.0, 0.0, 0.0, 0.0};
+        if (own && gl == 0) terms(c, aty, v, t);
+        block_max_sums<kBT / 64, 4>(v, t, red);
+        if (tid == 0) store(c, v, t);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Plain SpMV over block tiles: out[r] = sum val * vec (tests, autotuning).
 template <typename T, int NB>
 __global__ __launch_bounds__(kBT) void k_spmv_blk(const Tile* tiles, int ntiles, const int32_t* __restrict__ ptr,
@@ -367,6 +575,36 @@ void launch_adx_blk(const Dev& d, int t, hipStream_t s) {
         hipLaunchKernelGGL((k_adx_blk<2048>), g, b, 0, s, d, t);
     else
         hipLaunchKernelGGL((k_adx_blk<1024>), g, b, 0, s, d, t);
+}
+
+template <int NC>
+static void resid_blk(const Dev& d, hipStream_t s) {
+    const dim3 b(kBT);
+    if (d.nbA > 0) {
+        const dim3 g(block_grid(d.nbA));
+        if (d.budA == 512)
+            hipLaunchKernelGGL((k_resid_rows_blk<512, NC>), g, b, 0, s, d);
+        else if (d.budA == 2048)
+            hipLaunchKernelGGL((k_resid_rows_blk<2048, NC>), g, b, 0, s, d);
+        else
+            hipLaunchKernelGGL((k_resid_rows_blk<1024, NC>), g, b, 0, s, d);
+    }
+    if (d.nbT > 0) {
+        const dim3 g(block_grid(d.nbT));
+        if (d.budT == 512)
+            hipLaunchKernelGGL((k_resid_cols_blk<512, NC>), g, b, 0, s, d);
+        else if (d.budT == 2048)
+            hipLaunchKernelGGL((k_resid_cols_blk<2048, NC>), g, b, 0, s, d);
+        else
+            hipLaunchKernelGGL((k_resid_cols_blk<1024, NC>), g, b, 0, s, d);
+    }
+}
+
+void launch_resid_blk(const Dev& d, bool both, hipStream_t s) {
+    if (both)
+        resid_blk<2>(d, s);
+    else
+        resid_blk<1>(d, s);
 }
 
 template <typename T>
