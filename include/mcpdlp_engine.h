@@ -63,7 +63,9 @@ typedef struct mcp_problem {
     const double* offset;      /* [B] */
 } mcp_problem;
 
-/* Tile shape policy for the CSR-stream SpMV (autotuner knobs). */
+/* Tile shape policy of the wave tasks (autotuner knobs): the kernels of the
+ * reference reduction order, Ruiz and setup, the ray certificates, and the
+ * fused fast-order steps when block tiles are switched off. */
 typedef struct mcp_tile_policy {
     int32_t nnz_budget;     /* nonzeros per wave task (tile)                    */
     int32_t per_lane;       /* target nonzeros per lane when picking lanes/row  */
