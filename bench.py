@@ -6,18 +6,20 @@ Workload (default): the single large planted LP -- m = 200,000 rows (70 % '=',
 |a| log-uniform in [1e-2, 1e2], 0 <= x <= 10 -- solved to a relative KKT
 score <= 1e-6 (the reference's score: max of per-row relative primal
 violation, relative dual residual, relative gap; pdlp.cpp:62-135) through the
-product C-ABI mc_solve_pdlp with Backend::gpu, mixed precision (FP32 SpMV and
-updates, FP64 checks, FP64 polish) and the reflected Halpern restart
-scheme (--scheme; the plain FP64 comparison uses the classic PDLP restarts to
-the average, --plain-scheme kkt).
+product C-ABI mc_solve_pdlp with Backend::gpu, mixed precision (FP32 SpMVs
+with an FP64 iterate and FP64 updates until the FP32 phase stalls, then FP64
+SpMVs; FP64 checks throughout) and the reflected Halpern restart scheme
+(--scheme; the plain FP64 comparison uses the classic PDLP restarts to the
+average, --plain-scheme kkt).
 
 One "step" is one full solve: host packing + H2D + device Ruiz + PDHG
 iterations + D2H (the reference's total_ms also starts before Ruiz,
 pdlp.cpp:141).  The JSON line also carries: the returned pair re-scored in
 FP64 on the host, the objective error against the planted optimum, the
 roofline of the dominant fused kernel (device-event timing on the engine's
-stream, on the solve's layout; HBM traffic from the committed rocprofv3
-counter summary when present) and the CPU baseline (the compiled reference
+stream, each launch right after the other fused step as in the solve, on the
+solve's layout; HBM traffic from the committed rocprofv3 counter summary when
+present) and the CPU baseline (the compiled reference
 solve_pdlp, or the oracle, on one core: ms per iteration on this LP times the
 GPU's iteration count).  --plain adds the plain FP64 GPU solve of the same LP
 to the same tolerance (several more minutes on this LP).
