@@ -96,7 +96,9 @@ typedef struct mc_lp {
  *   the restart; the check compares T(z) (in place of the average) with z.  The
  *   primal weight at a restart is the geometric blend
  *   omega <- omega_move^(3/4) omega_res^(1/4) of the movement rule above and the
- *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)).
+ *   reference's residual-balance rule omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)),
+ *   except that it never moves against a residual imbalance of 100x or more: while
+ *   p > 100 d it does not decrease, while d > 100 p it does not increase.
  * Termination, tolerances and the reported score are the reference's in all three. */
 #define MC_SCHEME_REFERENCE 0
 #define MC_SCHEME_KKT 1

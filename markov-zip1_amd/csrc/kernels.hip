@@ -617,6 +617,11 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
                         wres = s.omega * sqrt(clampv(sqrt(r.primal / r.dual), 0.05, 20.0));
                     w = sqrt(w * sqrt(w * wres));
                     upd = true;
+                    // no move against a residual imbalance of 100x or more
+                    if (r.primal > 1e-12 && r.dual > 1e-12) {
+                        if (r.primal > 100.0 * r.dual && w < s.omega) w = s.omega;
+                        if (r.dual > 100.0 * r.primal && w > s.omega) w = s.omega;
+                    }
                 }
                 if (upd) s.omega = clampv(w, 1e-6, 1e6);
             }

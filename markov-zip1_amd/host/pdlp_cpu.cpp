@@ -378,6 +378,11 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                         wres = p.omega * std::sqrt(std::clamp(std::sqrt(cur.primal / cur.dual), 0.05, 20.0));
                     w = std::sqrt(w * std::sqrt(w * wres));
                     upd = true;
+                    // no move against a residual imbalance of 100x or more
+                    if (cur.primal > 1e-12 && cur.dual > 1e-12) {
+                        if (cur.primal > 100.0 * cur.dual && w < p.omega) w = p.omega;
+                        if (cur.dual > 100.0 * cur.primal && w > p.omega) w = p.omega;
+                    }
                 }
                 if (upd) p.omega = std::clamp(w, 1e-6, 1e6);
             }

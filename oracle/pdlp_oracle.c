@@ -480,6 +480,11 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                                 wres = omega * sqrt(or_clamp(sqrt(cur.primal / cur.dual), 0.05, 20.0));
                             wmove = sqrt(wmove * sqrt(wmove * wres));
                             upd = 1;
+                            /* no move against a residual imbalance of 100x or more */
+                            if (cur.primal > 1e-12 && cur.dual > 1e-12) {
+                                if (cur.primal > 100.0 * cur.dual && wmove < omega) wmove = omega;
+                                if (cur.dual > 100.0 * cur.primal && wmove > omega) wmove = omega;
+                            }
                         }
                         if (upd) {
                             omega = or_clamp(wmove, 1e-6, 1e6);
