@@ -398,6 +398,7 @@ def bench_warm(args):
     sols = []
     for k in range(0, ntr, 128):
         sols += mp.solve_batch(train[k:k + 128], **kw)
+        log(f"[bench] training solutions {len(sols)}/{ntr}: {(time.perf_counter() - t0):.1f}s")
     gen_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] training solutions: {sum(r['status'] == 'optimal' for r in sols)}/{ntr} optimal in {gen_ms:.0f}ms")
     ws = warmstart.WarmStart(base)
