@@ -5,9 +5,11 @@
 //   CSR(A)   rows = constraints: rp[m+1], ci[nnz] (int32, global column ids)
 //   CSR(A^T) rows = variables:   cp[n+1], ri[nnz] (int32, global row ids)
 //   values in FP64 (checks, FP64 iterations) and FP32 (mixed-precision iterations)
-// Work is cut into Tiles, each the task of ONE wave64: a run of consecutive
-// rows of one matrix and one instance, 2^lg lanes per row.  The host orders
-// rows by length bucket (fast order), so a tile's rows have similar lengths.
+// Work is cut into Tiles: a run of consecutive rows of one matrix and one
+// instance.  A wave tile is the task of ONE wave64 with 2^lg lanes per row; a
+// block tile (stream.hip) is the task of one 256-thread workgroup.  The host
+// orders rows by length bucket (fast order), so a tile's rows have similar
+// lengths.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -23,7 +25,8 @@ constexpr int kXcd = 8;            // MI355X: 8 XCDs, workgroups dealt round-rob
 struct Tile {
     int32_t r0, r1;  // rows [r0, r1)
     int32_t inst;    // instance id
-    int32_t shape;   // bits 0..2: lg = log2(lanes per row), 0..6
+    int32_t shape;   // bits 0..2: lg = log2(lanes per row), 0..6; bit 3 (block
+                     // tiles): one row longer than the budget, strided by the workgroup
 };
 
 // Instance status values.

@@ -14,11 +14,15 @@
 // optionally the infeasibility-certificate kernels k_ray_*, and k_post_check
 // (restart copy).
 //
-// Every row kernel is a set of wave tasks (one Tile per wave64, no LDS, no
-// workgroup barrier): a wave gates on its instance's status, picks FP32 or
-// FP64 SpMV operands from the instance's mode at run time (so mixed precision
-// costs no extra launches; the iterate is FP64 in both), sweeps its rows 64/G
-// at a time and lets lane 0 of each row group apply the update.  Step sizes are recomputed per element from
+// The kernels here are wave tasks (one Tile per wave64, no LDS, no workgroup
+// barrier): a wave gates on its instance's status, picks FP32 or FP64 SpMV
+// operands from the instance's mode at run time (so mixed precision costs no
+// extra launches; the iterate is FP64 in both), sweeps its rows 64/G at a time
+// and lets lane 0 of each row group apply the update.  They run the reference
+// reduction order (one lane per row), the setup, and the ray certificates; in
+// the fast order the launch helpers below hand the primal/dual steps, A dx and
+// the check's residual terms to the block-tile kernels of stream.hip when the
+// engine has block tiles.  Step sizes are recomputed per element from
 // (eta, omega) and the scaled L1 norms exactly as the reference's
 // update_step_sizes() forms them (tau = (eta/omega)/colnorm,
 // sigma = (eta*omega)/rownorm).  Built with -ffp-contract=off: every product
