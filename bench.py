@@ -401,22 +401,26 @@ def bench_warm(args):
         log(f"[bench] training solutions {len(sols)}/{ntr}: {(time.perf_counter() - t0):.1f}s")
     gen_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] training solutions: {sum(r['status'] == 'optimal' for r in sols)}/{ntr} optimal in {gen_ms:.0f}ms")
+    # the last nval training instances validate the start primal weight
+    nval = min(16, max(1, ntr // 8))
+    fit_n = ntr - nval
     ws = warmstart.WarmStart(base)
-    fit = ws.fit(train, [r["primal"] for r in sols], [r["dual"] for r in sols],
-                 omegas=[r["primal_weight"] for r in sols])
+    fit = ws.fit(train[:fit_n], [r["primal"] for r in sols[:fit_n]], [r["dual"] for r in sols[:fit_n]],
+                 omegas=[r["primal_weight"] for r in sols[:fit_n]],
+                 iterations=[r["iterations"] for r in sols[:fit_n]])
     log(f"[bench] MLP fit: {fit}")
+    tune = ws.tune_omega(train[fit_n:], **kw)
+    log(f"[bench] start primal weight on {nval} validation instances: {tune}")
     t1 = time.perf_counter()
     cold = mp.solve_batch(test, **kw)
     cold_ms = (time.perf_counter() - t1) * 1e3
     t2 = time.perf_counter()
-    pred = ws.predict(test)
-    warm = mp.solve_batch(test, warm_primal=np.concatenate([p[0] for p in pred]),
-                          warm_dual=np.concatenate([p[1] for p in pred]), initial_primal_weight=ws.omega, **kw)
+    warm, fallbacks = ws.solve_batch(test, **kw)
     warm_ms = (time.perf_counter() - t2) * 1e3
     ci = np.array([r["iterations"] for r in cold])
     wi = np.array([r["iterations"] for r in warm])
     log(f"[bench] cold: it mean={ci.mean():.0f} max={ci.max()} {cold_ms:.0f}ms | warm: it mean={wi.mean():.0f} "
-        f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference)")
+        f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference; {fallbacks} cold fallbacks)")
     _line(warm_ms, WORKLOADS["warm"]["name"].format(rows=base.rows, cols=base.cols),
           {"train": ntr, "test": nte, "m": base.rows, "n": base.cols, "nnz": base.nnz, "tolerance": tol,
            "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": args.scheme},
@@ -425,6 +429,7 @@ def bench_warm(args):
            "iterations_cold_max": int(ci.max()), "iterations_warm_max": int(wi.max()),
            "optimal_cold": int(sum(r["status"] == "optimal" for r in cold)),
            "optimal_warm": int(sum(r["status"] == "optimal" for r in warm)),
+           "warm_fallbacks": int(fallbacks), "start_primal_weight": tune,
            "speedup_warm_vs_cold": round(cold_ms / warm_ms, 3), "fit": fit,
            "training_solutions_ms": round(gen_ms, 1)})
 

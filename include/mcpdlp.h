@@ -135,7 +135,8 @@ typedef struct mc_pdlp_options {
     int32_t device;                   /* HIP device ordinal */
     int32_t require_gpu;              /* 1: a GPU request fails with MC_ERR_DEVICE instead of the cpu_fallback path */
     int32_t restart_scheme;           /* MC_SCHEME_REFERENCE (default), MC_SCHEME_KKT or MC_SCHEME_HALPERN */
-    const double* warm_primal;        /* optional [n] start point in the ORIGINAL space (NULL => cold x=proj(0)) */
+    const double* warm_primal;        /* optional [n] start point in the ORIGINAL space (NULL => cold x=proj(0));
+                                         a warm-started solve runs no FP32-SpMV phase */
     const double* warm_dual;          /* optional [m] start dual in the ORIGINAL space (reported sign convention) */
     int32_t fp32_stall_checks;        /* mixed: checks without a new best score before the FP64 switch (0 => 50) */
     int32_t detect_infeasibility;     /* 1: report MC_PDLP_INFEASIBLE_OR_UNBOUNDED when the iterate's movement

@@ -102,7 +102,8 @@ typedef struct mcp_solve_opts {
     int32_t graph_chunk;        /* iterations per HIP graph; 0 => default; <0 => no graphs */
     int32_t restart_scheme;     /* MC_SCHEME_* */
     int32_t fp32_stall_checks;  /* mixed: checks without a new best score before the FP64 switch; 0 => 50 */
-    const double* warm_primal;  /* optional [n] host, original space */
+    const double* warm_primal;  /* optional [n] host, original space; a warm start (either vector)
+                                   begins in FP64 SpMVs even with MC_PRECISION_MIXED */
     const double* warm_dual;    /* optional [m] host, original space (reported convention) */
     int32_t detect_infeasibility;     /* 1: stop on a ray certificate (status infeasible_or_unbounded) */
     int32_t pad2_;

@@ -713,10 +713,14 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
             wy = t;
         }
         mcp::launch_init_iterates(d, wx, wy, s);
-        mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, mixed, s);
+        // The FP32-SpMV phase serves the start far from the optimum; its
+        // rounding floor (1e-2 to 1e-4 relative KKT on planted LPs) lies above
+        // a warm start's accuracy, so a warm-started solve begins in FP64.
+        const bool fp32_start = mixed && !wx && !wy;
+        mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, fp32_start, s);
         int32_t* pin = e->pinned;
         pin[4] = e->B;
-        pin[5] = mixed ? e->B : 0;
+        pin[5] = fp32_start ? e->B : 0;
         HIPCHK(hipMemcpyAsync(d.counters, pin + 4, 2 * sizeof(int32_t), hipMemcpyHostToDevice, s));
         int64_t zero = 0;
         HIPCHK(hipMemcpyAsync(d.iter_base, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));

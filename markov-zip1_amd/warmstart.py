@@ -15,11 +15,16 @@ x = proj(0), y = 0; src/lp/first_order/pdlp.cpp:186-191).  Pipeline:
 3. a BF16 MLP on MFMA (mcpdlp.MLP: hand-written forward/backward/Adam kernels)
    maps standardized features to standardized coefficients;
 4. predict(): reconstruct z from the predicted coefficients, hand (x0, y0) to
-   solve_pdlp as warm_primal / warm_dual, together with the primal weight the
-   training solves ended at (median), since a start near the optimum with a
-   mismatched weight first drifts away from it.  Cold start remains the
-   fallback and the KKT score alone decides convergence, so a bad prediction
-   costs iterations, never correctness.
+   solve_pdlp as warm_primal / warm_dual;
+5. the start primal weight: the training solves' final weight (median), since
+   a start near the optimum with a mismatched weight first drifts away from
+   it, or the cold-start default -- tune_omega() picks the one that needs
+   fewer iterations on a held-out validation batch;
+6. solve_batch(): the warm-started batch with the cold start as the fallback.
+   An instance that has not reached the tolerance within twice the iterations
+   the hardest training solve needed is solved again from the cold start, so
+   a bad prediction costs iterations, never convergence; the KKT score alone
+   decides termination.
 """
 from __future__ import annotations
 
@@ -55,10 +60,17 @@ class WarmStart:
 
     @staticmethod
     def _pca(X: np.ndarray, k: int):
+        """Mean and the top-k principal directions of the rows of X.  Only
+        directions with a singular value above 1e-9 of the largest are kept:
+        with fewer samples than components the rest span rounding noise, on
+        which the training rows project to ~0 but a new row does not, and a
+        standardized feature on such a direction would be O(1e10).  Missing
+        directions are zero columns, which map every row to exactly 0."""
         mean = X.mean(axis=0)
-        _, _, vt = np.linalg.svd(X - mean, full_matrices=False)
-        basis = vt[:k].T
-        if basis.shape[1] < k:  # fewer samples than components: pad with zeros
+        _, sv, vt = np.linalg.svd(X - mean, full_matrices=False)
+        rank = int(np.sum(sv > 1e-9 * sv[0])) if sv.size and sv[0] > 0 else 0
+        basis = vt[:min(k, rank)].T
+        if basis.shape[1] < k:
             basis = np.pad(basis, ((0, 0), (0, k - basis.shape[1])))
         return mean, basis
 
@@ -67,7 +79,8 @@ class WarmStart:
         u = lp.var_upper["value"].astype(np.float64)
         return np.where((lp.var_upper["kind"] == mp.FINITE) & (u > 0), u, 1.0)
 
-    def fit(self, lps, xs, ys, omegas=None, epochs: int = 300, batch: int = 64, lr: float = 1e-3) -> dict:
+    def fit(self, lps, xs, ys, omegas=None, iterations=None, epochs: int = 300, batch: int = 64,
+            lr: float = 1e-3) -> dict:
         D = np.stack([self.perturbation(lp) for lp in lps])
         self.d_mean, self.d_basis = self._pca(D, self.n_features)
         F = (D - self.d_mean) @ self.d_basis
@@ -77,6 +90,8 @@ class WarmStart:
         Y = np.stack(ys)
         self.sy = float(np.sqrt(np.mean(Y * Y))) + 1e-12
         self.omega = float(np.median(omegas)) if omegas is not None and len(omegas) else 0.0
+        # warm-start iteration cap before the cold fallback (0: no cap)
+        self.cap = int(2 * max(iterations)) if iterations is not None and len(iterations) else 0
         Z = np.concatenate([X, Y / self.sy], axis=1)
         self.z_mean, self.z_basis = self._pca(Z, self.n_coeffs)
         Cf = (Z - self.z_mean) @ self.z_basis
@@ -87,7 +102,8 @@ class WarmStart:
                           capacity=max(32, batch), seed=self.seed, device=self.device)
         loss, ms = self.net.train(Fs, Cs, batch=batch, epochs=epochs, lr=lr, seed=self.seed + 1)
         return {"train_loss": loss, "train_ms": ms, "pca_explained": float(explained),
-                "samples": len(lps), "params": self.net.nparam, "primal_weight": self.omega}
+                "samples": len(lps), "params": self.net.nparam, "primal_weight": self.omega,
+                "iteration_cap": self.cap}
 
     def predict(self, lps):
         """[(x0, y0)] per instance; pass self.omega as initial_primal_weight."""
@@ -100,4 +116,39 @@ class WarmStart:
             lo = np.where(lp.var_lower["kind"] == mp.FINITE, lp.var_lower["value"], -np.inf)
             hi = np.where(lp.var_upper["kind"] == mp.FINITE, lp.var_upper["value"], np.inf)
             out.append((np.clip(x, lo, hi), z[n:] * self.sy))
+        return out
+
+    def solve_batch(self, lps, omega=None, **kw):
+        """Warm-started batched solve (mcpdlp.solve_batch keywords) with the
+        cold start as the fallback.  Returns (results, fallbacks): an instance
+        re-solved cold carries 'warm_fallback': True and the warm attempt's
+        iterations added to its own."""
+        pred = self.predict(lps)
+        w = self.omega if omega is None else omega
+        kw_w = dict(kw)
+        if self.cap > 0:
+            kw_w["max_iterations"] = min(int(kw.get("max_iterations", self.cap)), self.cap)
+        res = mp.solve_batch(lps, warm_primal=np.concatenate([p[0] for p in pred]),
+                             warm_dual=np.concatenate([p[1] for p in pred]), initial_primal_weight=w, **kw_w)
+        bad = [i for i, r in enumerate(res) if r["status"] != "optimal"]
+        if bad:
+            cold = mp.solve_batch([lps[i] for i in bad], **kw)
+            for i, r in zip(bad, cold):
+                r["iterations"] += res[i]["iterations"]
+                r["warm_fallback"] = True
+                res[i] = r
+        return res, len(bad)
+
+    def tune_omega(self, lps, **kw) -> dict:
+        """Choose the start primal weight on a validation batch: the training
+        solves' median final weight, or the cold-start default (0), whichever
+        needs fewer iterations in total (fallbacks included)."""
+        tried = {}
+        for w in (self.omega, 0.0):
+            res, nb = self.solve_batch(lps, omega=w, **kw)
+            tried[w] = (sum(r["iterations"] for r in res), nb)
+        best = min(tried, key=lambda w: tried[w][0])
+        out = {"median_final_weight": self.omega, "iterations_median_weight": tried[self.omega][0],
+               "iterations_default_weight": tried[0.0][0], "chosen": "median" if best == self.omega else "default"}
+        self.omega = best
         return out
