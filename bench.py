@@ -284,6 +284,8 @@ def _line(value_ms, workload, cfg, extra, steps=1, warmup=0):
             "warmup": warmup, "ms_per_step": round(value_ms, 3), "higher_is_better": False, "scaling": "strong",
             "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic (planted-feasible, seeded)",
             "config": {"workload": workload, **cfg, "parallelism": "single GPU"}}
+    if cfg.get("precision") == "fp64":
+        line["dtype"] = "fp64"
     line.update(extra)
     print(json.dumps(line), flush=True)
 
@@ -304,11 +306,12 @@ def bench_batch(args):
     kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1,
               max_iterations=args.max_iterations, graph_chunk=args.graph_chunk)
     t0 = time.perf_counter()
-    res = mp.solve_batch(lps, precision=mp.PRECISION_MIXED, restart_scheme=SCHEMES[args.scheme], **kw)
+    prec = mp.PRECISION_MIXED if args.precision == "mixed" else mp.PRECISION_FP64
+    res = mp.solve_batch(lps, precision=prec, restart_scheme=SCHEMES[args.scheme], **kw)
     wall = (time.perf_counter() - t0) * 1e3
     its = np.array([r["iterations"] for r in res])
     ok = sum(r["status"] == "optimal" for r in res)
-    log(f"[bench] batched mixed: {ok}/{len(res)} optimal, iterations mean={its.mean():.0f} max={its.max()} "
+    log(f"[bench] batched {args.precision}: {ok}/{len(res)} optimal, iterations mean={its.mean():.0f} max={its.max()} "
         f"wall={wall:.1f}ms kernel={res[0]['kernel_ms']:.1f}ms")
     # independent FP64 re-score and objective error of a few instances
     worst, worst_obj = 0.0, 0.0
@@ -365,7 +368,7 @@ def bench_batch(args):
     est = float(np.mean(plain_ms)) * len(lps)
     _line(wall, WORKLOADS["batch"]["name"],
           {"instances": len(lps), "sum_rows": int(sum(l.rows for l in lps)), "sum_nnz": int(sum(l.nnz for l in lps)),
-           "tolerance": tol, "precision": "mixed", "restart_scheme": args.scheme,
+           "tolerance": tol, "precision": args.precision, "restart_scheme": args.scheme,
            "plain_restart_scheme": args.plain_scheme},
           {"optimal": int(ok), "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
            "kkt_fp64_rescored_worst_of_sample": float(f"{worst:.4e}"),
