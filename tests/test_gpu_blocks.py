@@ -130,3 +130,21 @@ def test_block_batch_with_compaction():
         check_pair(lp, b)
         scale = 1.0 + np.abs(lp.objective * b["x"]).sum()
         assert abs(a["objective"] - b["objective"]) <= 1e-5 * scale
+
+
+def test_block_budget_arguments():
+    """Budgets are 512, 1024 or 2048 nonzeros, or 0 for both (block tiles
+    off); anything else is refused with an error and leaves the tiling as it
+    was."""
+    e = mp.Engine(0)
+    e.load(uniform())
+    e.setup(ruiz=False)
+    e.set_block_budget(2048, 512)
+    for rows, cols in [(300, 300), (0, 1024), (1024, 0), (4096, 1024), (-1, -1)]:
+        with pytest.raises(RuntimeError):
+            e.set_block_budget(rows, cols)
+    (ba, na), (bt, nt) = e.block_tiling()
+    assert (ba, bt) == (2048, 512) and na > 0 and nt > 0
+    e.set_block_budget(0, 0)
+    assert e.block_tiling() == ((0, 0), (0, 0))
+    e.close()
