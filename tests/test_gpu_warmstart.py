@@ -7,9 +7,9 @@ Bars:
 * every warm-started instance reaches 1e-6 (the cold fallback catches the
   ones that do not within the iteration cap), and its returned pair re-scored
   by the oracle's FP64 residuals is <= 1.5e-6;
-* the warm start's mean iteration count is within 1.25x of the cold start's
-  (the gain itself is printed: on this family it is small, since the 64 PCA
-  directions capture only part of how the solutions move);
+* what the trained network gains: the warm start's mean iteration count is
+  at least 10 % below the cold start's (measured on this family: 8,062 ->
+  5,652, -29.9 %; the gain is printed);
 * with an iteration cap too small for any instance, every instance falls back
   to the cold start and still converges.
 """
@@ -54,7 +54,7 @@ def test_warm_start_converges_and_does_not_lose(trained):
     wi = np.mean([r["iterations"] for r in warm])
     print(f"iterations to 1e-6: cold mean {ci:.0f}, warm mean {wi:.0f} ({100 * (wi / ci - 1):+.1f} %), "
           f"{fallbacks} cold fallbacks", flush=True)
-    assert wi <= 1.25 * ci
+    assert wi <= 0.9 * ci
 
 
 def test_cold_fallback_catches_every_unconverged_instance(trained):
