@@ -1051,17 +1051,32 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 else
                     mcp::launch_primal(d, 1, false, false, s);
             };
-            double tot = 0.0;
-            for (int r = 0; r < nrep; ++r) {
-                partner();
-                HIPCHK(hipEventRecord(e->ev[0], s));
-                launch();
-                HIPCHK(hipEventRecord(e->ev[1], s));
-                HIPCHK(hipEventSynchronize(e->ev[1]));
-                tot += ms_between(e->ev[0], e->ev[1]);
+            // All launches are queued before the one wait, with an event pair
+            // around each timed launch, so each launch is dispatched while the
+            // partner runs (as in the solve) rather than from an idle queue.
+            std::vector<hipEvent_t> evs(2 * (size_t)nrep, nullptr);
+            auto destroy = [&] {
+                for (auto& ev : evs)
+                    if (ev) (void)hipEventDestroy(ev);
+            };
+            try {
+                for (auto& ev : evs) HIPCHK(hipEventCreate(&ev));
+                for (int r = 0; r < nrep; ++r) {
+                    partner();
+                    HIPCHK(hipEventRecord(evs[2 * r], s));
+                    launch();
+                    HIPCHK(hipEventRecord(evs[2 * r + 1], s));
+                }
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventSynchronize(evs.back()));
+                double tot = 0.0;
+                for (int r = 0; r < nrep; ++r) tot += ms_between(evs[2 * r], evs[2 * r + 1]);
+                *avg_us = 1000.0 * tot / nrep;
+            } catch (...) {
+                destroy();
+                throw;
             }
-            HIPCHK(hipGetLastError());
-            *avg_us = 1000.0 * tot / nrep;
+            destroy();
         } else {
             HIPCHK(hipEventRecord(e->ev[0], s));
             for (int r = 0; r < nrep; ++r) launch();
