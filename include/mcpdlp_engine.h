@@ -157,8 +157,10 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
  * launched `reps` times on the engine's stream.  The fused steps (kinds 0-3)
  * are timed as the solve runs them: each timed launch follows a launch of the
  * other step (untimed), which streams the other matrix through the caches;
- * all launches are queued before one wait, with a HIP event pair around each
- * timed launch.  The other kinds are timed back to back between two events.
+ * all launches are queued before one wait, and each timed launch records a
+ * HIP event pair as part of its own dispatch (hipExtLaunchKernelGGL), so the
+ * mean is the kernel's own duration.  The other kinds are timed back to back
+ * between two events.
  * kind: 0 primal (A^T y + primal update) FP32, 1 dual (A x_bar + dual update)
  * FP32, 2 primal FP64, 3 dual FP64, 4 plain SpMV A x FP32, 5 plain A^T y FP32,
  * 6 plain streaming copy of the FP32 values of A (the calibration kernel for

@@ -1,12 +1,23 @@
 // dev.hpp -- device pointers of one packed problem + iterate state, passed
 // by value to every kernel, and the host-side launch helpers of kernels.hip.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "common.hpp"
 
 namespace mcp {
+
+// A kernel launch that, given events e0/e1, records them as part of the
+// kernel's dispatch packet (device timestamps of the kernel alone).
+#define MCP_LAUNCH_TIMED(K, g, b, s, e0, e1, ...)                        \
+    do {                                                                  \
+        if (e0)                                                           \
+            hipExtLaunchKernelGGL(K, g, b, 0, s, e0, e1, 0, __VA_ARGS__); \
+        else                                                              \
+            hipLaunchKernelGGL(K, g, b, 0, s, __VA_ARGS__);               \
+    } while (0)
 
 struct Dev {
     int32_t B;      // instances
@@ -100,8 +111,11 @@ struct Dev {
 // ---- launch helpers (kernels.hip) ----
 // Iteration kernels (each instance runs in its own FP32/FP64 mode).  `t` is
 // the iteration's offset inside the chunk; exact = reference reduction order.
-void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s);
-void launch_dual(const Dev& d, int t, bool exact, hipStream_t s);
+// With e0/e1 (timing only, never inside a captured graph) the launch records
+// the two events as part of the kernel's own dispatch (hipExtLaunchKernelGGL).
+void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
+void launch_dual(const Dev& d, int t, bool exact, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_adx(const Dev& d, int t, bool exact, hipStream_t s);
 void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s);
 void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s);
@@ -132,8 +146,9 @@ void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const 
                        const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
 // Block-tile forms (stream.hip): the fused fast-order primal/dual steps and a
 // plain SpMV over block tiles of `budget` (512, 1024 or 2048) nonzeros.
-void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s);
-void launch_dual_blk(const Dev& d, int t, hipStream_t s);
+void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr);
+void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_adx_blk(const Dev& d, int t, hipStream_t s);
 // Residual terms of the check's candidates over block tiles in one pass:
 // the average only, or (both) the average and the current iterate.

@@ -1051,9 +1051,9 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 else
                     mcp::launch_primal(d, 1, false, false, s);
             };
-            // All launches are queued before the one wait, with an event pair
-            // around each timed launch, so each launch is dispatched while the
-            // partner runs (as in the solve) rather than from an idle queue.
+            // All launches are queued before the one wait; each timed launch
+            // records its event pair as part of its own dispatch (no separate
+            // event packets between the kernels, as between graph nodes).
             std::vector<hipEvent_t> evs(2 * (size_t)nrep, nullptr);
             auto destroy = [&] {
                 for (auto& ev : evs)
@@ -1063,9 +1063,10 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 for (auto& ev : evs) HIPCHK(hipEventCreate(&ev));
                 for (int r = 0; r < nrep; ++r) {
                     partner();
-                    HIPCHK(hipEventRecord(evs[2 * r], s));
-                    launch();
-                    HIPCHK(hipEventRecord(evs[2 * r + 1], s));
+                    if (kind == 0 || kind == 2)
+                        mcp::launch_primal(d, 1, false, false, s, evs[2 * r], evs[2 * r + 1]);
+                    else
+                        mcp::launch_dual(d, 1, false, s, evs[2 * r], evs[2 * r + 1]);
                 }
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipEventSynchronize(evs.back()));

@@ -1115,37 +1115,37 @@ static inline int tile_grid(int ntiles) {
 }
 static inline int elem_grid(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
-void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s) {
+void launch_primal(const Dev& d, int t, bool adapt, bool exact, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (!exact && d.blocks) {
-        launch_primal_blk(d, t, adapt, s);
+        launch_primal_blk(d, t, adapt, s, e0, e1);
         return;
     }
     if (d.ntT == 0) return;
     const dim3 g(tile_grid(d.ntT)), b(kBlock);
     if (exact) {
         if (adapt)
-            hipLaunchKernelGGL((k_primal<true, true>), g, b, 0, s, d, t);
+            MCP_LAUNCH_TIMED((k_primal<true, true>), g, b, s, e0, e1, d, t);
         else
-            hipLaunchKernelGGL((k_primal<false, true>), g, b, 0, s, d, t);
+            MCP_LAUNCH_TIMED((k_primal<false, true>), g, b, s, e0, e1, d, t);
     } else {
         if (adapt)
-            hipLaunchKernelGGL((k_primal<true, false>), g, b, 0, s, d, t);
+            MCP_LAUNCH_TIMED((k_primal<true, false>), g, b, s, e0, e1, d, t);
         else
-            hipLaunchKernelGGL((k_primal<false, false>), g, b, 0, s, d, t);
+            MCP_LAUNCH_TIMED((k_primal<false, false>), g, b, s, e0, e1, d, t);
     }
 }
 
-void launch_dual(const Dev& d, int t, bool exact, hipStream_t s) {
+void launch_dual(const Dev& d, int t, bool exact, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (!exact && d.blocks) {
-        launch_dual_blk(d, t, s);
+        launch_dual_blk(d, t, s, e0, e1);
         return;
     }
     if (d.ntA == 0) return;
     const dim3 g(tile_grid(d.ntA)), b(kBlock);
     if (exact)
-        hipLaunchKernelGGL((k_dual<true>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_dual<true>), g, b, s, e0, e1, d, t);
     else
-        hipLaunchKernelGGL((k_dual<false>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_dual<false>), g, b, s, e0, e1, d, t);
 }
 
 void launch_adx(const Dev& d, int t, bool exact, hipStream_t s) {

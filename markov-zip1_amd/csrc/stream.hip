@@ -537,33 +537,33 @@ __global__ __launch_bounds__(kBT) void k_spmv_blk(const Tile* tiles, int ntiles,
 static inline int block_grid(int ntiles) { return kXcd * ((ntiles + kXcd - 1) / kXcd); }
 
 template <bool ADAPT>
-static void primal_blk(const Dev& d, int t, hipStream_t s) {
+static void primal_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const dim3 g(block_grid(d.nbT)), b(kBT);
     if (d.budT == 512)
-        hipLaunchKernelGGL((k_primal_blk<512, ADAPT>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_primal_blk<512, ADAPT>), g, b, s, e0, e1, d, t);
     else if (d.budT == 2048)
-        hipLaunchKernelGGL((k_primal_blk<2048, ADAPT>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_primal_blk<2048, ADAPT>), g, b, s, e0, e1, d, t);
     else
-        hipLaunchKernelGGL((k_primal_blk<1024, ADAPT>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_primal_blk<1024, ADAPT>), g, b, s, e0, e1, d, t);
 }
 
-void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s) {
+void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (d.nbT == 0) return;
     if (adapt)
-        primal_blk<true>(d, t, s);
+        primal_blk<true>(d, t, s, e0, e1);
     else
-        primal_blk<false>(d, t, s);
+        primal_blk<false>(d, t, s, e0, e1);
 }
 
-void launch_dual_blk(const Dev& d, int t, hipStream_t s) {
+void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (d.nbA == 0) return;
     const dim3 g(block_grid(d.nbA)), b(kBT);
     if (d.budA == 512)
-        hipLaunchKernelGGL((k_dual_blk<512>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<512>), g, b, s, e0, e1, d, t);
     else if (d.budA == 2048)
-        hipLaunchKernelGGL((k_dual_blk<2048>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<2048>), g, b, s, e0, e1, d, t);
     else
-        hipLaunchKernelGGL((k_dual_blk<1024>), g, b, 0, s, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<1024>), g, b, s, e0, e1, d, t);
 }
 
 void launch_adx_blk(const Dev& d, int t, hipStream_t s) {
