@@ -126,6 +126,30 @@ std::vector<int32_t> bucket_order(const std::vector<int64_t>& len) {
     return order;
 }
 
+// True when the rows of `lp` are local: the mean over non-empty rows of
+// (largest - smallest column index + 1) / columns is below 1/4.  Block-angular
+// LPs (each row inside its period's columns) are; rows with random columns
+// (span ~ all columns) are not.
+bool rows_are_local(const mc_lp& lp) {
+    const int64_t m = lp.rows, n = lp.cols;
+    if (m == 0 || n == 0) return false;
+    std::vector<int64_t> lo(m, n), hi(m, -1);
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t k = lp.column_start[j]; k < lp.column_start[j + 1]; ++k) {
+            const int64_t i = lp.row_index[k];
+            lo[i] = std::min(lo[i], j);
+            hi[i] = std::max(hi[i], j);
+        }
+    double span = 0.0;
+    int64_t rows = 0;
+    for (int64_t i = 0; i < m; ++i)
+        if (hi[i] >= 0) {
+            span += (double)(hi[i] - lo[i] + 1) / (double)n;
+            ++rows;
+        }
+    return rows > 0 && span / (double)rows < 0.25;
+}
+
 struct EngineDeleter {
     void operator()(mcp_engine* e) const { mcp_engine_destroy(e); }
 };
@@ -192,8 +216,12 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     std::vector<double> rv(nnz), cv(nnz), rlo(m), rhi(m), vlo(n), vhi(n), obj(n), sense(B), offset(B);
     // In the fast reduction order rows and columns are relabelled into length
     // buckets (row_perm[new] = old, col_perm[new] = old, per instance), so each
-    // SpMV tile holds rows of similar length.  The reference order keeps the
-    // model's labels: its sums must run in the reference's index order.
+    // SpMV tile holds rows of similar length -- unless the instance's rows
+    // already have column locality (rows_are_local): then a tile of
+    // consecutive rows gathers from a narrow window of x_bar, and the bucket
+    // order would interleave rows from all over the matrix.  The reference
+    // order keeps the model's labels: its sums must run in the reference's
+    // index order.
     const bool relabel = opt.reduction_order != ReductionOrder::reference;
     std::vector<int32_t> row_perm(m), col_perm(n);
     const double inf = std::numeric_limits<double>::infinity();
@@ -207,7 +235,7 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
             const int64_t lm = lp.rows, ln = lp.cols, lnz = lp.column_start[ln];
             const int th = B == 1 ? hw : 1;
             std::vector<int32_t> cperm(ln), rinv(lm);
-            if (relabel) {
+            if (relabel && !rows_are_local(lp)) {
                 std::vector<int64_t> clen(ln), rlen(lm, 0);
                 for (int64_t j = 0; j < ln; ++j) clen[j] = lp.column_start[j + 1] - lp.column_start[j];
                 for (int64_t k = 0; k < lnz; ++k) ++rlen[lp.row_index[k]];

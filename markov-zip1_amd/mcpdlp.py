@@ -481,11 +481,29 @@ def _length_bucket(lens: np.ndarray) -> np.ndarray:
     return out
 
 
+def rows_are_local(lp: LPData) -> bool:
+    """The host driver's locality test: the mean over non-empty rows of
+    (largest - smallest column index + 1) / columns is below 1/4."""
+    if lp.rows == 0 or lp.cols == 0 or lp.nnz == 0:
+        return False
+    col_of = np.repeat(np.arange(lp.cols), np.diff(lp.column_start))
+    lo = np.full(lp.rows, lp.cols, np.int64)
+    hi = np.full(lp.rows, -1, np.int64)
+    np.minimum.at(lo, lp.row_index, col_of)
+    np.maximum.at(hi, lp.row_index, col_of)
+    used = hi >= 0
+    return float(np.mean((hi[used] - lo[used] + 1) / lp.cols)) < 0.25
+
+
 def relabel_by_length(lp: LPData) -> LPData:
     """The same LP with rows and columns stably sorted by descending length
     bucket -- the relabelling the host driver applies before mcp_load in the
     fast reduction order, so Engine users can time kernels on the product's
-    layout.  Returns an LP without a planted pair."""
+    layout.  Like the host, it keeps the natural order (returns `lp` itself)
+    when the rows are local (rows_are_local).  Otherwise returns an LP without
+    a planted pair."""
+    if rows_are_local(lp):
+        return lp
     clen = np.diff(lp.column_start)
     rlen = np.bincount(lp.row_index, minlength=lp.rows)
     cperm = np.argsort(-_length_bucket(clen), kind="stable")  # new -> old

@@ -303,3 +303,33 @@ def test_relabel_by_length_is_a_permutation():
     x, y = lp.x_star[cperm], -lp.y_star[rperm]
     assert np.isclose(q.objective @ x, lp.objective @ lp.x_star, rtol=1e-14)
     assert orc.kkt(q, x, y)["score"] == pytest.approx(orc.kkt(lp, lp.x_star, -lp.y_star)["score"], abs=1e-12)
+
+
+def test_rows_are_local_matches_a_dense_span():
+    """mcpdlp.rows_are_local (the host driver's test for keeping the natural
+    order) against the span read off the dense matrix: block-angular LPs (rows
+    inside a period's columns) are local and relabel_by_length leaves them as
+    they are; random-column LPs are not; a hand-made matrix sits on each side
+    of the 1/4 threshold."""
+    for lp, want in [(mp.generate(mp.FAMILY_BLOCK_ANGULAR, rows=400, cols=1000, periods=8, link_rows=800), True),
+                     (mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12), False),
+                     (mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=3), False)]:
+        A = lp.dense() != 0
+        used = A.any(axis=1)
+        first = np.argmax(A, axis=1)
+        last = A.shape[1] - 1 - np.argmax(A[:, ::-1], axis=1)
+        span = ((last - first + 1) / A.shape[1])[used].mean()
+        assert (span < 0.25) == want == mp.rows_are_local(lp), span
+        if want:
+            assert mp.relabel_by_length(lp) is lp
+    # 4 rows x 8 columns: spans 1/8, 1/8, 1/8 and s/8 (row 3 covers columns 0..s-1)
+    for s_, want in [(4, True), (5, False)]:
+        rows = [[0], [1], [2], list(range(s_))]
+        cs, ri = [0], []
+        for j in range(8):
+            ri += [i for i, r in enumerate(rows) if j in r]
+            cs.append(len(ri))
+        lp = mp.LPData(rows=4, cols=8, column_start=np.array(cs, np.int64), row_index=np.array(ri, np.int64),
+                       value=np.ones(len(ri)), objective=np.zeros(8), var_lower=np.zeros(8),
+                       var_upper=np.ones(8), row_lower=np.zeros(4), row_upper=np.ones(4))
+        assert mp.rows_are_local(lp) == want  # mean span (3 + s) / 32
