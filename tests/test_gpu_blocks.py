@@ -132,6 +132,41 @@ def test_block_batch_with_compaction():
         assert abs(a["objective"] - b["objective"]) <= 1e-5 * scale
 
 
+def permute_columns(lp, perm):
+    """The same LP with its columns in the order `perm` (new j = old perm[j])."""
+    starts = lp.column_start
+    idx = np.concatenate([np.arange(starts[j], starts[j + 1]) for j in perm])
+    cs = np.concatenate([[0], np.cumsum(np.diff(starts)[perm])]).astype(starts.dtype)
+    return mp.LPData(lp.rows, lp.cols, cs, lp.row_index[idx], lp.value[idx], lp.objective[perm],
+                     lp.row_lower.copy(), lp.row_upper.copy(), lp.var_lower[perm], lp.var_upper[perm],
+                     lp.sense, lp.offset, None if lp.x_star is None else lp.x_star[perm], lp.y_star,
+                     lp.planted_objective)
+
+
+def test_local_rows_keep_their_order_and_reach_the_same_optimum():
+    """The host driver keeps the natural row and column order for an LP whose
+    rows are local (block-angular: each row inside one or two periods'
+    columns) and relabels by length bucket otherwise.  The same LP with its
+    columns shuffled is not local and takes the relabelled path; both solves
+    reach the tolerance on their own LP and agree on the objective, and the
+    natural-order solve matches the planted optimum."""
+    tol = 1e-4
+    lp = mp.generate(mp.FAMILY_BLOCK_ANGULAR, rows=200, cols=500, periods=8, link_rows=400, seed=21)
+    shuffled = permute_columns(lp, np.random.default_rng(4).permutation(lp.cols))
+    assert mp.rows_are_local(lp) and not mp.rows_are_local(shuffled)
+    kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=mp.SCHEME_HALPERN,
+              precision=mp.PRECISION_MIXED, max_iterations=400000)
+    print("local rows", lp.rows, lp.cols, lp.nnz, flush=True)
+    runs = [(q, mp.solve(q, **kw)) for q in (lp, shuffled)]
+    for q, r in runs:
+        print("iterations", r["iterations"], "objective", r["objective"], flush=True)
+        assert r["status"] == "optimal", r["status"]
+        assert orc.kkt(q, r["primal"], r["dual"])["score"] <= 1.5 * tol
+    scale = 1.0 + np.abs(lp.objective * runs[0][1]["primal"]).sum()
+    assert abs(runs[0][1]["objective"] - runs[1][1]["objective"]) <= 10 * tol * scale
+    assert abs(runs[0][1]["objective"] - lp.planted_objective) <= 10 * tol * scale
+
+
 def test_block_budget_arguments():
     """Budgets are 512, 1024 or 2048 nonzeros, or 0 for both (block tiles
     off); anything else is refused with an error and leaves the tiling as it
