@@ -124,13 +124,24 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
     __shared__ double red[kBT / 64];
     const int tile = block_tile_id(d.nbT);
     if (tile >= d.nbT) return;
+    // Instance state, iteration counter and the tile's row pointers are read
+    // together, and a stopped instance's tile is turned into an empty one
+    // rather than an early exit: a launch over a small instance is a chain of
+    // dependent round trips, and this keeps it at tile -> (state, row
+    // pointers) -> nonzeros -> gathers.
+    const int32_t* __restrict__ ptr = d.cp;
+    const int64_t* __restrict__ ib = d.iter_base;
     const Tile tl = d.bT[tile];
     const InstState& s = d.st[tl.inst];
-    if (s.status != kRunning) return;  // uniform over the workgroup
-    const int64_t iter = *d.iter_base + t;
+    const int32_t status = s.status, mode = s.mode;
     const double eta = s.eta, omega = s.omega;
-    const double kavg = (double)(iter - s.restart_iter + 1);
-    const bool f32 = s.mode == kModeFp32;
+    const int64_t r_iter = s.restart_iter;
+    const int64_t iter = *ib + t;
+    const int p0 = ptr[tl.r0], pe = ptr[tl.r1];  // both loaded whatever the status
+    const bool run = status == kRunning;  // uniform over the workgroup
+    const bool f32 = mode == kModeFp32;
+    const int p1 = run ? pe : p0;
+    const double kavg = (double)(iter - r_iter + 1);
     const bool halp = d.halpern != 0;
     const double w = kavg / (kavg + 1.0);
     const double* xsrc = halp ? d.xlast : d.xavg;
@@ -166,11 +177,11 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
         xo = d.x[j];
         xa = xsrc[j];  // running average, or the Halpern anchor
     };
-    if (tl.shape & 8) {  // one long column, strided by the whole workgroup
+    if (tl.shape & 8) {  // one long column (r1 = r0 + 1), strided by the whole workgroup
         const int j = tl.r0;
-        const double aty = f32 ? long_row_dot<float>(d.cp[j], d.cp[j + 1], d.ri, d.tv32, d.y32, red)
-                               : long_row_dot<double>(d.cp[j], d.cp[j + 1], d.ri, d.tv64, d.y, red);
-        if (tid == 0) {
+        const double aty = f32 ? long_row_dot<float>(p0, p1, d.ri, d.tv32, d.y32, red)
+                               : long_row_dot<double>(p0, p1, d.ri, d.tv64, d.y, red);
+        if (tid == 0 && run) {
             load(j);
             update(j, aty);
             if (ADAPT) d.part_dxn[tile] = part;
@@ -179,8 +190,8 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
     }
     const int lg = tl.shape & 7, G = 1 << lg;
     const int j = tl.r0 + (tid >> lg), gl = tid & (G - 1);
-    const bool own = j < tl.r1;
-    const int p0 = d.cp[tl.r0], len = d.cp[tl.r1] - p0;
+    const bool own = run && j < tl.r1;
+    const int len = p1 - p0;
     int q0 = 0, q1 = 0;
     if (own) {
         q0 = d.cp[j] - p0;
@@ -191,7 +202,7 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
     if (own && gl == 0) update(j, aty);
     if (ADAPT) {
         part = block_sum<kBT / 64>(part, red);
-        if (tid == 0) d.part_dxn[tile] = part;
+        if (tid == 0 && run) d.part_dxn[tile] = part;
     }
 }
 
@@ -204,13 +215,21 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
     __shared__ double red[kBT / 64];
     const int tile = block_tile_id(d.nbA);
     if (tile >= d.nbA) return;
+    // State, iteration counter and row pointers read together; a stopped
+    // instance's tile is empty (as in k_primal_blk).
+    const int32_t* __restrict__ ptr = d.rp;
+    const int64_t* __restrict__ ib = d.iter_base;
     const Tile tl = d.bA[tile];
     const InstState& s = d.st[tl.inst];
-    if (s.status != kRunning) return;
-    const int64_t iter = *d.iter_base + t;
+    const int32_t status = s.status, mode = s.mode;
     const double eta = s.eta, omega = s.omega;
-    const double kavg = (double)(iter - s.restart_iter + 1);
-    const bool f32 = s.mode == kModeFp32;
+    const int64_t r_iter = s.restart_iter;
+    const int64_t iter = *ib + t;
+    const int p0 = ptr[tl.r0], pe = ptr[tl.r1];  // both loaded whatever the status
+    const bool run = status == kRunning;
+    const bool f32 = mode == kModeFp32;
+    const int p1 = run ? pe : p0;
+    const double kavg = (double)(iter - r_iter + 1);
     const bool halp = d.halpern != 0;
     const double w = kavg / (kavg + 1.0);
     const double* ysrc = halp ? d.ylast : d.yavg;
@@ -238,11 +257,11 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
         d.y[i] = ynext;
         if (f32) d.y32[i] = (float)ynext;
     };
-    if (tl.shape & 8) {
+    if (tl.shape & 8) {  // one long row (r1 = r0 + 1)
         const int i = tl.r0;
-        const double ax = f32 ? long_row_dot<float>(d.rp[i], d.rp[i + 1], d.ci, d.av32, d.xbar32, red)
-                              : long_row_dot<double>(d.rp[i], d.rp[i + 1], d.ci, d.av64, d.xbar, red);
-        if (tid == 0) {
+        const double ax = f32 ? long_row_dot<float>(p0, p1, d.ci, d.av32, d.xbar32, red)
+                              : long_row_dot<double>(p0, p1, d.ci, d.av64, d.xbar, red);
+        if (tid == 0 && run) {
             load(i);
             update(i, ax);
         }
@@ -250,8 +269,8 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
     }
     const int lg = tl.shape & 7, G = 1 << lg;
     const int i = tl.r0 + (tid >> lg), gl = tid & (G - 1);
-    const bool own = i < tl.r1;
-    const int p0 = d.rp[tl.r0], len = d.rp[tl.r1] - p0;
+    const bool own = run && i < tl.r1;
+    const int len = p1 - p0;
     int q0 = 0, q1 = 0;
     if (own) {
         q0 = d.rp[i] - p0;
@@ -271,22 +290,28 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
     __shared__ double red[kBT / 64];
     const int tile = block_tile_id(d.nbA);
     if (tile >= d.nbA) return;
+    // State and row pointers read together; a stopped instance's tile is
+    // empty (as in k_primal_blk).
+    const int32_t* __restrict__ ptr = d.rp;
     const Tile tl = d.bA[tile];
     const InstState& s = d.st[tl.inst];
-    if (s.status != kRunning) return;
-    const bool f32 = s.mode == kModeFp32;
+    const int32_t status = s.status, mode = s.mode;
+    const int p0 = ptr[tl.r0], pe = ptr[tl.r1];  // both loaded whatever the status
+    const bool run = status == kRunning;
+    const bool f32 = mode == kModeFp32;
+    const int p1 = run ? pe : p0;
     const int tid = threadIdx.x;
-    if (tl.shape & 8) {
+    if (tl.shape & 8) {  // one long row (r1 = r0 + 1)
         const int i = tl.r0;
-        const double a = f32 ? long_row_dot<float>(d.rp[i], d.rp[i + 1], d.ci, d.av32, d.dx32, red)
-                             : long_row_dot<double>(d.rp[i], d.rp[i + 1], d.ci, d.av64, d.dx, red);
-        if (tid == 0) d.part_adx[tile] = (a * a) / d.rnorm[i];
+        const double a = f32 ? long_row_dot<float>(p0, p1, d.ci, d.av32, d.dx32, red)
+                             : long_row_dot<double>(p0, p1, d.ci, d.av64, d.dx, red);
+        if (tid == 0 && run) d.part_adx[tile] = (a * a) / d.rnorm[i];
         return;
     }
     const int lg = tl.shape & 7, G = 1 << lg;
     const int i = tl.r0 + (tid >> lg), gl = tid & (G - 1);
-    const bool own = i < tl.r1;
-    const int p0 = d.rp[tl.r0], len = d.rp[tl.r1] - p0;
+    const bool own = run && i < tl.r1;
+    const int len = p1 - p0;
     int q0 = 0, q1 = 0;
     double rn = 1.0;
     if (own) {
@@ -296,7 +321,7 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
     }
     const double a = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.dx, d.av32, d.dx32, prod);
     const double part = block_sum<kBT / 64>((own && gl == 0) ? (a * a) / rn : 0.0, red);
-    if (tid == 0) d.part_adx[tile] = part;
+    if (tid == 0 && run) d.part_adx[tile] = part;
 }
 
 // ---------------------------------------------------------------------------
