@@ -482,8 +482,9 @@ def _length_bucket(lens: np.ndarray) -> np.ndarray:
 
 
 def rows_are_local(lp: LPData) -> bool:
-    """The host driver's locality test: the mean over non-empty rows of
-    (largest - smallest column index + 1) / columns is below 1/4."""
+    """The host library's locality test (host/pdlp_gpu.cpp): the mean over
+    non-empty rows of (largest - smallest column index + 1) / columns is
+    below 1/4."""
     if lp.rows == 0 or lp.cols == 0 or lp.nnz == 0:
         return False
     col_of = np.repeat(np.arange(lp.cols), np.diff(lp.column_start))
@@ -497,7 +498,7 @@ def rows_are_local(lp: LPData) -> bool:
 
 def relabel_by_length(lp: LPData) -> LPData:
     """The same LP with rows and columns stably sorted by descending length
-    bucket -- the relabelling the host driver applies before mcp_load in the
+    bucket -- the relabelling the host library applies before mcp_load in the
     fast reduction order, so Engine users can time kernels on the product's
     layout.  Like the host, it keeps the natural order (returns `lp` itself)
     when the rows are local (rows_are_local).  Otherwise returns an LP without

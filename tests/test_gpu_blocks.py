@@ -144,7 +144,7 @@ def permute_columns(lp, perm):
 
 
 def test_local_rows_keep_their_order_and_reach_the_same_optimum():
-    """The host driver keeps the natural row and column order for an LP whose
+    """The host library keeps the natural row and column order for an LP whose
     rows are local (block-angular: each row inside one or two periods'
     columns) and relabels by length bucket otherwise.  The same LP with its
     columns shuffled is not local and takes the relabelled path; both solves

@@ -285,7 +285,7 @@ def test_cpu_infeasibility_detection(scheme):
 
 
 def test_relabel_by_length_is_a_permutation():
-    """mcpdlp.relabel_by_length (the host driver's length-bucket relabelling,
+    """mcpdlp.relabel_by_length (the host library's length-bucket relabelling,
     for Engine users): same LP up to a row/column permutation -- the relabelled
     dense matrix is a permutation of the original, columns and rows come in
     descending length bucket, every column's rows are strictly increasing, and
@@ -306,7 +306,7 @@ def test_relabel_by_length_is_a_permutation():
 
 
 def test_rows_are_local_matches_a_dense_span():
-    """mcpdlp.rows_are_local (the host driver's test for keeping the natural
+    """mcpdlp.rows_are_local (the host library's test for keeping the natural
     order) against the span read off the dense matrix: block-angular LPs (rows
     inside a period's columns) are local and relabel_by_length leaves them as
     they are; random-column LPs are not; a hand-made matrix sits on each side
