@@ -178,10 +178,11 @@ int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const
 
 /* Block tiles of the fused fast-order primal/dual steps (one 256-thread
  * workgroup streams a contiguous run of rows holding at most `budget`
- * nonzeros; the products are summed per row from LDS).  Budgets are 512,
- * 1024 or 2048 nonzeros per tile; 0 for both runs those steps on the wave
- * tiles of mcp_set_tile_policy instead.  Default: 512 up to 2M nonzeros,
- * 1024 above; the autotuner may replace it.  Reference order always uses the
+ * nonzeros; the products are summed per row from LDS).  Budgets are 256,
+ * 512, 1024 or 2048 nonzeros per tile; 0 for both runs those steps on the wave
+ * tiles of mcp_set_tile_policy instead.  Default: 256 up to 128k nonzeros,
+ * 512 up to 2M, 1024 above (a batch re-tiled as its instances finish takes
+ * the budget of the nonzeros still running); the autotuner may replace it.  Reference order always uses the
  * wave tiles.  No counterpart in the reference (its SpMV is a serial loop,
  * src/lp/first_order/pdlp.cpp:22-44): a tuning knob of this engine. */
 int mcp_set_block_budget(mcp_engine* e, int32_t rows_budget, int32_t cols_budget);

@@ -130,8 +130,18 @@ void build_tiles(const std::vector<int32_t>& ptr, const std::vector<int32_t>& in
 // the workgroup has per row (G * rows <= kBlock), at most 64 and no more than
 // the tile's longest row needs.  A row longer than the budget is a tile of
 // its own (shape bit 3) that the whole workgroup strides.
+// Budget by size (FP64 us per iteration, budgets 256 / 512 / 1024: 7.77 /
+// 8.18 / 9.08 at 25k nonzeros, 7.85 / 8.37 / 9.53 at 52k, 10.46 / 9.95 /
+// 10.34 at 209k): launches over small problems are latency chains, and
+// smaller tiles shorten them until the tile count outgrows the gain.
 constexpr int kBlockBudget = 1024;
-constexpr int kBlockBudgetLatency = 512;
+constexpr int kBlockBudgetLatency = 512;   // up to kLatencyNnz nonzeros
+constexpr int kBlockBudgetTiny = 256;      // up to kTinyNnz nonzeros
+constexpr int64_t kTinyNnz = 128 * 1024;
+
+int block_budget_for(int64_t nnz) {
+    return nnz <= kTinyNnz ? kBlockBudgetTiny : nnz <= kLatencyNnz ? kBlockBudgetLatency : kBlockBudget;
+}
 
 int floor_log2(int v) {
     int l = 0;
@@ -617,7 +627,7 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         e->loaded = true;
         e->setup_done = false;
         e->polA = e->polT = e->nnz <= kLatencyNnz ? kLatencyPolicy : kThroughputPolicy;
-        e->budA = e->budT = e->nnz <= kLatencyNnz ? kBlockBudgetLatency : kBlockBudget;
+        e->budA = e->budT = block_budget_for(e->nnz);
         e->retile(false);  // setup kernels run on these tiles
     });
 }
@@ -807,7 +817,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                         if (keep[b]) live_nnz += e->rp[e->inst_row[b + 1]] - e->rp[e->inst_row[b]];
                     if (live_nnz <= kLatencyNnz) {
                         e->polA = e->polT = kLatencyPolicy;
-                        if (e->budA > 0 && e->budT > 0) e->budA = e->budT = kBlockBudgetLatency;
+                        if (e->budA > 0 && e->budT > 0) e->budA = e->budT = block_budget_for(live_nnz);
                     }
                     e->retile(e->exact_tiles, &keep);
                     e->drop_graphs();
@@ -1106,9 +1116,9 @@ int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const
 int mcp_set_block_budget(mcp_engine* e, int32_t rows_budget, int32_t cols_budget) {
     return guarded([&] {
         require(e && e->loaded, "mcp_set_block_budget needs a loaded problem");
-        auto ok = [](int32_t b) { return b == 0 || b == 512 || b == 1024 || b == 2048; };
+        auto ok = [](int32_t b) { return b == 0 || b == 256 || b == 512 || b == 1024 || b == 2048; };
         require(ok(rows_budget) && ok(cols_budget) && ((rows_budget == 0) == (cols_budget == 0)),
-                "block budgets must both be 0 or each one of 512, 1024, 2048");
+                "block budgets must both be 0 or each one of 256, 512, 1024, 2048");
         HIPCHK(hipSetDevice(e->device));
         e->budA = rows_budget;
         e->budT = cols_budget;

@@ -564,7 +564,9 @@ static inline int block_grid(int ntiles) { return kXcd * ((ntiles + kXcd - 1) / 
 template <bool ADAPT>
 static void primal_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const dim3 g(block_grid(d.nbT)), b(kBT);
-    if (d.budT == 512)
+    if (d.budT == 256)
+        MCP_LAUNCH_TIMED((k_primal_blk<256, ADAPT>), g, b, s, e0, e1, d, t);
+    else if (d.budT == 512)
         MCP_LAUNCH_TIMED((k_primal_blk<512, ADAPT>), g, b, s, e0, e1, d, t);
     else if (d.budT == 2048)
         MCP_LAUNCH_TIMED((k_primal_blk<2048, ADAPT>), g, b, s, e0, e1, d, t);
@@ -583,7 +585,9 @@ void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s, hipEvent_
 void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (d.nbA == 0) return;
     const dim3 g(block_grid(d.nbA)), b(kBT);
-    if (d.budA == 512)
+    if (d.budA == 256)
+        MCP_LAUNCH_TIMED((k_dual_blk<256>), g, b, s, e0, e1, d, t);
+    else if (d.budA == 512)
         MCP_LAUNCH_TIMED((k_dual_blk<512>), g, b, s, e0, e1, d, t);
     else if (d.budA == 2048)
         MCP_LAUNCH_TIMED((k_dual_blk<2048>), g, b, s, e0, e1, d, t);
@@ -594,7 +598,9 @@ void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent
 void launch_adx_blk(const Dev& d, int t, hipStream_t s) {
     if (d.nbA == 0) return;
     const dim3 g(block_grid(d.nbA)), b(kBT);
-    if (d.budA == 512)
+    if (d.budA == 256)
+        hipLaunchKernelGGL((k_adx_blk<256>), g, b, 0, s, d, t);
+    else if (d.budA == 512)
         hipLaunchKernelGGL((k_adx_blk<512>), g, b, 0, s, d, t);
     else if (d.budA == 2048)
         hipLaunchKernelGGL((k_adx_blk<2048>), g, b, 0, s, d, t);
@@ -607,7 +613,9 @@ static void resid_blk(const Dev& d, hipStream_t s) {
     const dim3 b(kBT);
     if (d.nbA > 0) {
         const dim3 g(block_grid(d.nbA));
-        if (d.budA == 512)
+        if (d.budA == 256)
+            hipLaunchKernelGGL((k_resid_rows_blk<256, NC>), g, b, 0, s, d);
+        else if (d.budA == 512)
             hipLaunchKernelGGL((k_resid_rows_blk<512, NC>), g, b, 0, s, d);
         else if (d.budA == 2048)
             hipLaunchKernelGGL((k_resid_rows_blk<2048, NC>), g, b, 0, s, d);
@@ -616,7 +624,9 @@ static void resid_blk(const Dev& d, hipStream_t s) {
     }
     if (d.nbT > 0) {
         const dim3 g(block_grid(d.nbT));
-        if (d.budT == 512)
+        if (d.budT == 256)
+            hipLaunchKernelGGL((k_resid_cols_blk<256, NC>), g, b, 0, s, d);
+        else if (d.budT == 512)
             hipLaunchKernelGGL((k_resid_cols_blk<512, NC>), g, b, 0, s, d);
         else if (d.budT == 2048)
             hipLaunchKernelGGL((k_resid_cols_blk<2048, NC>), g, b, 0, s, d);
@@ -636,7 +646,9 @@ template <typename T>
 static void spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
                      const T* val, const T* vec, T* out, hipStream_t s) {
     const dim3 g(block_grid(ntiles)), b(kBT);
-    if (budget == 512)
+    if (budget == 256)
+        hipLaunchKernelGGL((k_spmv_blk<T, 256>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+    else if (budget == 512)
         hipLaunchKernelGGL((k_spmv_blk<T, 512>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
     else if (budget == 2048)
         hipLaunchKernelGGL((k_spmv_blk<T, 2048>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);

@@ -635,7 +635,7 @@ class Engine:
         return (a.nnz_budget, a.per_lane, na.value), (t.nnz_budget, t.per_lane, nt.value)
 
     def set_block_budget(self, rows=1024, cols=1024):
-        """Nonzeros per block tile of the fused fast-order steps (512/1024/2048),
+        """Nonzeros per block tile of the fused fast-order steps (256/512/1024/2048),
         or 0 for both to run those steps on the wave tiles."""
         _check_eng(self.lib.mcp_set_block_budget(self.h, int(rows), int(cols)))
 

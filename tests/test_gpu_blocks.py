@@ -3,7 +3,7 @@ a run of rows, the products are summed per row from LDS) against the oracle
 and against the wave-task form of the same steps.
 
 Bars:
-* SpMV over block tiles of every budget (512, 1024, 2048 nonzeros), including
+* SpMV over block tiles of every budget (256, 512, 1024, 2048 nonzeros), including
   rows longer than the budget in both orientations: FP64 within 1e-13 of the
   oracle's serial sum, FP32 within 2e-6 of the row's absolute sum;
 * full solves (Halpern scheme, FP64 and mixed precision, single LPs and a
@@ -53,7 +53,7 @@ def long_cols():  # every column (row of A^T) holds 2500 nonzeros
 MAKERS = [uniform, mixed, long_rows, long_cols]
 
 
-@pytest.mark.parametrize("budget", [512, 1024, 2048])
+@pytest.mark.parametrize("budget", [256, 512, 1024, 2048])
 @pytest.mark.parametrize("maker", MAKERS)
 def test_block_spmv_parity(budget, maker):
     lp = maker()
@@ -108,11 +108,11 @@ SOLVE_CASES = [(uniform, 1e-6), (mixed, 1e-4), (long_rows, 1e-6), (long_cols, 1e
 def test_block_and_wave_steps_reach_the_same_optimum(maker, tol, precision):
     lp = maker()
     print("block solve", maker.__name__, precision, flush=True)
-    runs = {b: engine_solve(lp, b, precision, tol)[0] for b in (0, 512, 1024)}
+    runs = {b: engine_solve(lp, b, precision, tol)[0] for b in (0, 256, 512, 1024)}
     for r in runs.values():
         check_pair(lp, r, tol)
     scale = 1.0 + np.abs(lp.objective * runs[0]["x"]).sum()
-    for b in (512, 1024):
+    for b in (256, 512, 1024):
         assert abs(runs[b]["objective"] - runs[0]["objective"]) <= 10 * tol * scale
 
 
@@ -168,14 +168,17 @@ def test_local_rows_keep_their_order_and_reach_the_same_optimum():
 
 
 def test_block_budget_arguments():
-    """Budgets are 512, 1024 or 2048 nonzeros, or 0 for both (block tiles
-    off); anything else is refused with an error and leaves the tiling as it
-    was."""
+    """Budgets are 256, 512, 1024 or 2048 nonzeros, or 0 for both (block
+    tiles off); anything else is refused with an error and leaves the tiling
+    as it was."""
     e = mp.Engine(0)
     e.load(uniform())
     e.setup(ruiz=False)
+    e.set_block_budget(256, 2048)
+    (ba, na), (bt, nt) = e.block_tiling()
+    assert (ba, bt) == (256, 2048) and na > 0 and nt > 0
     e.set_block_budget(2048, 512)
-    for rows, cols in [(300, 300), (0, 1024), (1024, 0), (4096, 1024), (-1, -1)]:
+    for rows, cols in [(300, 300), (0, 1024), (1024, 0), (4096, 1024), (128, 512), (-1, -1)]:
         with pytest.raises(RuntimeError):
             e.set_block_budget(rows, cols)
     (ba, na), (bt, nt) = e.block_tiling()
