@@ -41,6 +41,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "markov-zip1_amd"))
 
 HBM_PEAK_GBS = 8000.0
+T_START = time.time()
+BUDGET_S = [480.0]
+
+
+def left():
+    """Seconds left of the run's wall-clock budget."""
+    return BUDGET_S[0] - (time.time() - T_START)
 
 METRIC = "time to 1e-6 rel KKT + SpMV HBM GB/s; mixed/batched/warm vs plain PDLP, 1 GPU"
 
@@ -69,8 +76,15 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=1,
+                    help="full solves to time; fewer run when they would not fit --budget-s (at least one)")
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed warm-up solves, each capped at --warmup-iterations iterations")
+    ap.add_argument("--warmup-iterations", type=int, default=2000)
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock budget of the whole run; the JSON line is printed within it")
+    ap.add_argument("--no-fp64-compare", action="store_true",
+                    help="skip the FP64 solve under the same scheme (mixed vs FP64 precision)")
     ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
     ap.add_argument("--tolerance", type=float, default=None)
     ap.add_argument("--max-iterations", type=int, default=2_000_000)
@@ -97,6 +111,7 @@ def main():
                     help="only time the listed fused kernels (comma-separated Engine.KERNELS names) on the "
                          "workload's layout and exit -- the short program rocprofv3 runs wrap")
     args = ap.parse_args()
+    BUDGET_S[0] = args.budget_s
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the batched and "
               f"single-LP paths run on one MI355X -- rerun with --gpus 1", file=sys.stderr)
@@ -120,9 +135,9 @@ def main():
     log(f"[bench] generated {wl['name']}: m={lp.rows} n={lp.cols} nnz={lp.nnz} in {time.time() - t:.1f}s")
 
     prec = mp.PRECISION_MIXED if args.precision == "mixed" else mp.PRECISION_FP64
-    opts = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=prec,
-                      restart_scheme=SCHEMES[args.scheme], max_iterations=args.max_iterations,
-                      graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
+    common = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
+                  graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
+    opts = mp.options(precision=prec, max_iterations=args.max_iterations, **common)
 
     def run(o):
         t0 = time.perf_counter()
@@ -131,17 +146,44 @@ def main():
         assert r["backend_actually_used"] == "hip", r["backend_actually_used"]
         return r, wall
 
+    # Warm-up: iteration-capped solves of the same LP with the same options
+    # (autotune cache, captured graphs, device buffers); never the full solve.
+    wopts = mp.options(precision=prec, max_iterations=min(args.max_iterations, args.warmup_iterations), **common)
+    warm_done = 0
     for k in range(args.warmup):
-        r, wall = run(opts)
+        if left() < 0.6 * args.budget_s:
+            break
+        r, wall = run(wopts)
+        warm_done += 1
         log(f"[bench] warmup {k}: {r['status']} it={r['iterations']} wall={wall:.1f}ms")
+
+    # Timed solves: the first always runs; more run while they fit the budget,
+    # keeping room for the kernel timing, the CPU sample and (when it fits)
+    # one FP64 solve under the same scheme, which isolates the precision.
+    reserve = 25.0 + (0.0 if args.no_cpu else args.cpu_seconds * 1.3)
     walls, res = [], None
-    for k in range(args.steps):
+    fp64 = None
+    while len(walls) < max(1, args.steps):
+        if walls:
+            per = max(walls) / 1e3
+            want_fp64 = fp64 is None and prec == mp.PRECISION_MIXED and not args.no_fp64_compare
+            slack = left() - reserve - (1.3 * per if want_fp64 else 0.0)
+            if slack < 1.15 * per:
+                break
         res, wall = run(opts)
         walls.append(wall)
-        log(f"[bench] step {k}: {res['status']} it={res['iterations']} fp32_it={res['fp32_iterations']} "
+        log(f"[bench] step {len(walls) - 1}: {res['status']} it={res['iterations']} fp32_it={res['fp32_iterations']} "
             f"wall={wall:.1f}ms total={res['total_ms']:.1f} kernel={res['kernel_ms']:.1f} "
-            f"setup={res['setup_ms']:.1f} h2d={res['h2d_ms']:.1f} d2h={res['d2h_ms']:.1f}")
+            f"setup={res['setup_ms']:.1f} h2d={res['h2d_ms']:.1f} d2h={res['d2h_ms']:.1f} ({left():.0f}s left)")
     ms = float(np.mean(walls))
+
+    if prec == mp.PRECISION_MIXED and not args.no_fp64_compare and left() - reserve > 1.3 * max(walls) / 1e3:
+        fo = mp.options(precision=mp.PRECISION_FP64, max_iterations=args.max_iterations, **common)
+        fr, fwall = run(fo)
+        fp64 = {"ms": round(fwall, 3), "status": fr["status"], "iterations": fr["iterations"],
+                "restart_scheme": args.scheme, "kernel_ms": round(fr["kernel_ms"], 3),
+                "note": "same LP, scheme and tolerance with FP64 SpMVs throughout"}
+        log(f"[bench] FP64 (same scheme): {fr['status']} it={fr['iterations']} wall={fwall:.1f}ms")
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as orc  # the FP64 checker and the CPU baseline
@@ -152,7 +194,7 @@ def main():
         f"objective rel err vs planted {obj_err:.2e}")
 
     plain = None
-    if args.plain:
+    if args.plain and left() > reserve + 60:
         po = mp.options(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_FP64,
                         restart_scheme=SCHEMES[args.plain_scheme], max_iterations=args.max_iterations,
                         graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
@@ -168,60 +210,110 @@ def main():
     kern, roof, ta, tt = {}, None, None, None
     if not args.no_roofline:
         kern, (ta, tt) = time_kernels(mp, lp, ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64",
-                                               "spmv_fp32", "spmv_t_fp32", "copy"), not args.no_autotune)
+                                               "spmv_fp32", "spmv_t_fp32", "gather_fp64", "gather_local_fp64",
+                                               "stream_fp64", "copy"),
+                                      not args.no_autotune)
         fp32_major = res["fp32_iterations"] * 2 > res["iterations"]
         kinds = ("primal_fp32", "dual_fp32") if fp32_major else ("primal_fp64", "dual_fp64")
         dom = max(kinds, key=lambda k: kern[k]["us"])
         ach = kern[dom]["GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, dom)}
+    gather = gather_ceiling(kern, lp.nnz)
 
     cpu = None
     if not args.no_cpu:
-        use_ref = orc.ref_available()
-        fn = orc.ref_solve if use_ref else orc.solve
-        # ms per iteration from two short runs (Ruiz + setup cancel out)
-        lo_it = 40
-        t0 = time.perf_counter()
-        fn(lp, tolerance=tol, max_iterations=lo_it)
-        t_lo = time.perf_counter() - t0
-        per_guess = max(t_lo / (lo_it + 20), 1e-4)
-        hi_it = lo_it + max(40, int(args.cpu_seconds * 0.6 / per_guess) // 40 * 40)
-        t0 = time.perf_counter()
-        fn(lp, tolerance=tol, max_iterations=hi_it)
-        t_hi = time.perf_counter() - t0
-        ms_it = (t_hi - t_lo) * 1e3 / (hi_it - lo_it)
-        cpu = {"value": round(ms_it * res["iterations"], 1), "unit": "ms", "cores": 1,
-               "kind": "reference" if use_ref else "port",
-               "sample": f"{hi_it - lo_it} PDHG iterations of this LP (runs of {lo_it} and {hi_it} "
-                         f"iterations, difference) -> {ms_it:.2f} ms/iteration x {res['iterations']} GPU "
-                         f"iterations (extrapolated; the reference's own restart rule does not reach "
-                         f"{tol:g} on this LP)"}
-        log(f"[bench] cpu baseline: {ms_it:.2f} ms/iteration ({cpu['kind']})")
+        cpu = cpu_baseline(orc, lp, tol, res["iterations"], min(args.cpu_seconds, max(3.0, left() - 15.0)))
 
     line = {
         "metric": METRIC,
-        "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": len(walls), "warmup": warm_done,
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "fp32+fp64" if prec == mp.PRECISION_MIXED else "fp64", "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
                    "precision": args.precision, "restart_scheme": args.scheme, "parallelism": "single GPU"},
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
+        "warmup_iterations": args.warmup_iterations, "budget_s": args.budget_s,
+        "walls_ms": [round(w, 1) for w in walls],
         "status": res["status"], "iterations": res["iterations"], "fp32_iterations": res["fp32_iterations"],
         "kernel_ms": round(res["kernel_ms"], 3), "setup_ms": round(res["setup_ms"], 3),
         "h2d_ms": round(res["h2d_ms"], 3), "d2h_ms": round(res["d2h_ms"], 3),
         "kkt_fp64_rescored": {k: float(f"{kkt[k]:.4e}") for k in ("primal", "dual", "gap", "score")},
         "objective_rel_err_vs_planted": float(f"{obj_err:.3e}"),
+        "fp64_same_scheme": fp64,
+        "speedup_vs_fp64_same_scheme": round(fp64["ms"] / ms, 3) if fp64 else None,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(plain["ms"] / ms, 3) if plain else None,
         "kernels": kern, "tiles": {"rows": list(ta), "cols": list(tt)} if ta else None,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "gather_ceiling": gather, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
+def gather_ceiling(kern, nnz):
+    """The FP64 block SpMV A x_bar on the solve's tiles with random gathers, with
+    all gathers inside a 256-entry window, and with none (mcp_time_kernel kinds
+    7-9, each after a primal step as in the solve): the random-gather rate this
+    matrix allows, and the fused steps' time against that plain SpMV."""
+    need = ("gather_fp64", "gather_local_fp64", "stream_fp64", "dual_fp64", "primal_fp64")
+    if not all(k in kern for k in need):
+        return None
+    g = kern["gather_fp64"]["us"]
+    return {"spmv_random_gathers_us": g, "spmv_window_gathers_us": kern["gather_local_fp64"]["us"],
+            "spmv_no_gathers_us": kern["stream_fp64"]["us"],
+            "random_gathers_per_s": float(f"{nnz / g * 1e6:.4e}"),
+            "dual_fp64_vs_plain_spmv": round(g / kern["dual_fp64"]["us"], 4),
+            "primal_fp64_vs_plain_spmv": round(g / kern["primal_fp64"]["us"], 4),
+            "note": "fraction = plain FP64 SpMV time with the same random gathers / fused step time"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(orc, lp, tol, gpu_iterations, seconds):
+    """ms/iteration of the CPU oracle (the C restatement of the reference's
+    solve_pdlp, bit-identical to the compiled reference on the pinned cases)
+    on one pinned core, from two bounded runs of the same LP (Ruiz + setup
+    cancel in the difference), times the GPU's iteration count."""
+    aff = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+    core = min(aff) if aff else None
+    if core is not None:
+        os.sched_setaffinity(0, {core})
+    try:
+        lo_it = 40
+        t0 = time.perf_counter()
+        orc.solve(lp, tolerance=tol, max_iterations=lo_it)
+        t_lo = time.perf_counter() - t0
+        per_guess = max(t_lo / (lo_it + 20), 1e-4)
+        hi_it = lo_it + max(40, int(max(seconds - t_lo, 1.0) * 0.8 / per_guess) // 40 * 40)
+        t0 = time.perf_counter()
+        orc.solve(lp, tolerance=tol, max_iterations=hi_it)
+        t_hi = time.perf_counter() - t0
+    finally:
+        if core is not None:
+            os.sched_setaffinity(0, aff)
+    ms_it = max(t_hi - t_lo, 1e-9) * 1e3 / (hi_it - lo_it)
+    log(f"[bench] cpu baseline: {ms_it:.3f} ms/iteration (oracle, core {core}, {cpu_model()})")
+    return {"value": round(ms_it * gpu_iterations, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "cpu": cpu_model(), "pinned_core": core, "ms_per_iteration": round(ms_it, 4),
+            "sample": f"{hi_it - lo_it} PDHG iterations of this LP by the oracle (runs of {lo_it} and {hi_it} "
+                      f"iterations, difference; reference restart scheme, FP64, one pinned core) -> "
+                      f"{ms_it:.3f} ms/iteration x {gpu_iterations} GPU iterations (extrapolated; the "
+                      f"reference's own restart rule does not reach {tol:g} on this LP)"}
+
+
 KERNEL_SYMBOL = {"primal_fp32": "k_primal_blk", "dual_fp32": "k_dual_blk", "primal_fp64": "k_primal_blk",
                  "dual_fp64": "k_dual_blk", "spmv_fp32": "k_spmv_plain", "spmv_t_fp32": "k_spmv_plain",
-                 "copy": "k_copy"}
+                 "copy": "k_copy", "gather_fp64": "k_gather_probe<.,0>", "gather_local_fp64": "k_gather_probe<.,1>",
+                 "stream_fp64": "k_gather_probe<.,2>"}
 
 
 def time_kernels(mp, lp, kinds, autotune, reps=50):

@@ -81,6 +81,9 @@ struct IterOpts {
     int32_t stall_checks;        // mixed: checks without a new best before the FP64 switch
     int32_t halpern;             // Halpern scheme: blended primal weight at restarts
     int32_t detect;              // test the iterate's movement for infeasibility certificates
+    int32_t adaptive_step;       // A dx + eta update every 10 iterations (part of the captured
+                                 // kernel sequence, so a change must drop the cached graphs)
+    int32_t pad_;
 };
 
 // Map (workgroup, wave) to a tile so that each XCD works on a contiguous

@@ -155,5 +155,10 @@ void launch_adx_blk(const Dev& d, int t, hipStream_t s);
 void launch_resid_blk(const Dev& d, bool both, hipStream_t s);
 void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
                      const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
+// Gather probe (FP64 block SpMV, stream.hip): mode 0 gathers vec[idx], mode 1
+// vec[idx & mask] (mask + 1 entries, all cache hits), mode 2 no gather.
+void launch_gather_probe(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
+                         const double* val, const double* vec, double* out, int mode, int mask, hipStream_t s,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 }  // namespace mcp

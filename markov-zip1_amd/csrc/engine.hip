@@ -448,6 +448,7 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.halpern = o->restart_scheme == MC_SCHEME_HALPERN ? 1 : 0;
     io.stall_checks = o->fp32_stall_checks > 0 ? o->fp32_stall_checks : 50;
     io.detect = o->detect_infeasibility != 0 ? 1 : 0;
+    io.adaptive_step = o->adaptive_step_size != 0 ? 1 : 0;
     io.infeas_tol = o->infeasibility_tolerance > 0.0 ? o->infeasibility_tolerance : 1e-8;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
     // 0 => no score threshold: FP32 runs until it converges or stalls
@@ -738,7 +739,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
 
         const int64_t max_it = (int64_t)o->max_iterations;
         const int64_t R = io.restart_every;
-        const bool adaptive = o->adaptive_step_size != 0;
+        const bool adaptive = io.adaptive_step != 0;
         const int64_t period = adaptive ? (10 / gcd64(10, R)) * R : R;
         int chunk = o->graph_chunk == 0 ? 64 : std::abs(o->graph_chunk);
         const bool use_graphs = o->graph_chunk >= 0;
@@ -1000,12 +1001,14 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
 int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes) {
     return guarded([&] {
         require(e && e->setup_done, "mcp_time_kernel before mcp_setup");
-        require(kind >= 0 && kind <= 6, "unknown kernel kind");
+        require(kind >= 0 && kind <= 9, "unknown kernel kind");
+        const bool probe = kind >= 7;  // gather probes over the block tiles of A (FP64)
+        require(!probe || (e->d.blocks && e->d.nbA > 0 && e->n > 0), "gather probes need block tiles of A");
         HIPCHK(hipSetDevice(e->device));
         auto& d = e->d;
         hipStream_t s = e->stream;
         const double m = e->m, n = e->n, nnz = (double)e->nnz;
-        const bool f32 = kind == 0 || kind == 1 || kind >= 4;
+        const bool f32 = kind == 0 || kind == 1 || (kind >= 4 && kind <= 6);
         const double w = f32 ? 4.0 : 8.0;  // bytes per value
         // Matrix (index + value of width w) and pointers once, the gathered
         // vector (width w) once.  Per element, FP64 in both modes: the primal
@@ -1022,6 +1025,15 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
             b = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 4.0 * m;
         else if (kind == 5)
             b = 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 4.0 * n;
+        // kinds 7-9: the FP64 block SpMV A x_bar on the solve's tiles with random
+        // gathers (7), gathers confined to a window of mask + 1 <= 256 entries (8),
+        // no gathers (9); the stream is that of k_dual_blk in FP64 mode
+        int mask = 1;
+        while (2 * mask <= std::min(256, e->n)) mask *= 2;
+        mask -= 1;
+        if (kind == 7) b = 12.0 * nnz + 4.0 * (m + 1) + 8.0 * n + 8.0 * m;
+        if (kind == 8) b = 12.0 * nnz + 4.0 * (m + 1) + 8.0 * (mask + 1) + 8.0 * m;
+        if (kind == 9) b = 12.0 * nnz + 4.0 * (m + 1) + 8.0 * m;
         const int64_t ncopy = e->nnz / 4 * 4;  // kind 6: plain copy of the FP32 values of A
         float* copy_dst = kind == 6 ? e->buf<float>("copy_dst", ncopy) : nullptr;
         if (kind == 6) b = 8.0 * (double)ncopy;
@@ -1046,12 +1058,15 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 case 3: mcp::launch_dual(d, 1, false, s); break;
                 case 4: mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.xbar32, d.y32, true, s); break;
                 case 5: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.dx32, true, s); break;
-                default: mcp::launch_copy(d.av32, copy_dst, ncopy, s); break;
+                case 6: mcp::launch_copy(d.av32, copy_dst, ncopy, s); break;
+                default:
+                    mcp::launch_gather_probe(d.bA, d.nbA, d.budA, d.rp, d.ci, d.av64, d.xbar, d.ax, kind - 7, mask, s);
+                    break;
             }
         };
         launch();
         const int nrep = std::max(1, reps);
-        if (kind <= 3) {
+        if (kind <= 3 || probe) {
             // The fused steps are timed as the solve runs them: each launch
             // follows the other step, which streams the other matrix through
             // the caches in between.
@@ -1075,6 +1090,9 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                     partner();
                     if (kind == 0 || kind == 2)
                         mcp::launch_primal(d, 1, false, false, s, evs[2 * r], evs[2 * r + 1]);
+                    else if (probe)
+                        mcp::launch_gather_probe(d.bA, d.nbA, d.budA, d.rp, d.ci, d.av64, d.xbar, d.ax, kind - 7,
+                                                 mask, s, evs[2 * r], evs[2 * r + 1]);
                     else
                         mcp::launch_dual(d, 1, false, s, evs[2 * r], evs[2 * r + 1]);
                 }

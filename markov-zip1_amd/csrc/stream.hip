@@ -559,7 +559,98 @@ __global__ __launch_bounds__(kBT) void k_spmv_blk(const Tile* tiles, int ntiles,
 }
 
 // ---------------------------------------------------------------------------
+// Gather probe: the FP64 block SpMV above with its gathers redirected, to
+// measure what the random gathers of x_bar / y cost on a given tiling.
+// GM 0: vec[idx] (the SpMV itself); GM 1: vec[idx & mask] (a window of
+// mask + 1 <= 256 entries, so every gather hits in L1/L2); GM 2: no gather
+// (the product takes the index as its operand).  The matrix stream, the LDS
+// staging and the row sums are those of k_spmv_blk in all three.
+template <int GM>
+__device__ __forceinline__ double probe_operand(const double* __restrict__ vec, int i, int mask) {
+    if (GM == 0) return vec[i];
+    if (GM == 1) return vec[i & mask];
+    return (double)i;
+}
+
+template <int NB, int GM>
+__global__ __launch_bounds__(kBT) void k_gather_probe(const Tile* tiles, int ntiles, const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ idx,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ vec, double* __restrict__ out,
+                                                      int mask) {
+    constexpr int K = NB / kBT;
+    __shared__ double prod[NB];
+    __shared__ double red[kBT / 64];
+    const int tile = block_tile_id(ntiles);
+    if (tile >= ntiles) return;
+    const Tile tl = tiles[tile];
+    if (tl.shape & 8) {
+        double s = 0.0;
+        for (int p = ptr[tl.r0] + (int)threadIdx.x; p < ptr[tl.r0 + 1]; p += kBT)
+            s += val[p] * probe_operand<GM>(vec, idx[p], mask);
+        const double v = block_sum<kBT / 64>(s, red);
+        if (threadIdx.x == 0) out[tl.r0] = v;
+        return;
+    }
+    const int lg = tl.shape & 7, G = 1 << lg;
+    const int r = tl.r0 + ((int)threadIdx.x >> lg), gl = (int)threadIdx.x & (G - 1);
+    const bool own = r < tl.r1;
+    const int p0 = ptr[tl.r0], len = ptr[tl.r1] - p0;
+    const int q0 = own ? ptr[r] - p0 : 0, q1 = own ? ptr[r + 1] - p0 : 0;
+    int ii[K];
+    double aa[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        const bool in = q < len;
+        ii[k] = in ? idx[p0 + q] : 0;
+        aa[k] = in ? val[p0 + q] : 0.0;
+    }
+    double vv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        vv[k] = q < len ? probe_operand<GM>(vec, ii[k], mask) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = (int)threadIdx.x + k * kBT;
+        if (q < len) prod[q] = aa[k] * vv[k];
+    }
+    __syncthreads();
+    const double v = group_sum<double>(prod, q0, q1, gl, G);
+    if (own && gl == 0) out[r] = v;
+}
+
+// ---------------------------------------------------------------------------
 static inline int block_grid(int ntiles) { return kXcd * ((ntiles + kXcd - 1) / kXcd); }
+
+template <int GM>
+static void gather_probe(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
+                         const double* val, const double* vec, double* out, int mask, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1) {
+    const dim3 g(block_grid(ntiles)), b(kBT);
+    if (budget == 256)
+        MCP_LAUNCH_TIMED((k_gather_probe<256, GM>), g, b, s, e0, e1, tiles, ntiles, ptr, idx, val, vec, out, mask);
+    else if (budget == 512)
+        MCP_LAUNCH_TIMED((k_gather_probe<512, GM>), g, b, s, e0, e1, tiles, ntiles, ptr, idx, val, vec, out, mask);
+    else if (budget == 2048)
+        MCP_LAUNCH_TIMED((k_gather_probe<2048, GM>), g, b, s, e0, e1, tiles, ntiles, ptr, idx, val, vec, out, mask);
+    else
+        MCP_LAUNCH_TIMED((k_gather_probe<1024, GM>), g, b, s, e0, e1, tiles, ntiles, ptr, idx, val, vec, out, mask);
+}
+
+void launch_gather_probe(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
+                         const double* val, const double* vec, double* out, int mode, int mask, hipStream_t s,
+                         hipEvent_t e0, hipEvent_t e1) {
+    if (ntiles == 0) return;
+    if (mode == 1)
+        gather_probe<1>(tiles, ntiles, budget, ptr, idx, val, vec, out, mask, s, e0, e1);
+    else if (mode == 2)
+        gather_probe<2>(tiles, ntiles, budget, ptr, idx, val, vec, out, mask, s, e0, e1);
+    else
+        gather_probe<0>(tiles, ntiles, budget, ptr, idx, val, vec, out, mask, s, e0, e1);
+}
 
 template <bool ADAPT>
 static void primal_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {

@@ -645,8 +645,12 @@ class Engine:
         _check_eng(self.lib.mcp_get_block_tiling(self.h, C.byref(ba), C.byref(bt), C.byref(na), C.byref(nt)))
         return (ba.value, na.value), (bt.value, nt.value)
 
+    # gather_fp64 / gather_local_fp64 / stream_fp64: the FP64 block SpMV A x_bar on the
+    # solve's tiles with random gathers, with every gather inside a 256-entry window
+    # (cache hits), and with no gathers -- what the random gathers cost on this matrix
     KERNELS = {"primal_fp32": 0, "dual_fp32": 1, "primal_fp64": 2, "dual_fp64": 3,
-               "spmv_fp32": 4, "spmv_t_fp32": 5, "copy": 6}
+               "spmv_fp32": 4, "spmv_t_fp32": 5, "copy": 6,
+               "gather_fp64": 7, "gather_local_fp64": 8, "stream_fp64": 9}
 
     def time_kernel(self, kind: str, reps: int = 20):
         """(mean device us per launch, algorithmic bytes per launch) of one fused kernel."""

@@ -202,21 +202,24 @@ def test_kkt_scheme_reference_order_bit_exact(gpu, name, maker, kw, scheme):
 @pytest.mark.parametrize("precision", [mp.PRECISION_FP64, mp.PRECISION_MIXED])
 def test_kkt_scheme_converges_to_1e6(gpu, precision, scheme):
     """Fast order, FP64 or mixed: reaches 1e-6 on a planted LP; the returned pair
-    is re-scored independently by the oracle's FP64 residuals on the host, and its
-    objective matches the planted optimum c^T x* to 1e-5 relative to the size of
-    the objective's terms, 1 + sum |c_j x*_j| (a KKT score of 1e-6 bounds the
-    residuals, not |p - p*| itself; this LP's objective is ~0.1 while its terms
-    are O(1e3))."""
+    is re-scored independently by the oracle's FP64 residuals on the host.  The
+    objective is held to 1e-6 x (1 + |p*|) of the planted optimum p* = c^T x*
+    on a solve to 1e-8: a KKT score of 1e-6 bounds the residuals, not |p - p*|
+    itself, and this LP's objective (~0.12) is a sum of terms of size 1e3 (the
+    oracle's error at 1e-6 is 2.6e-6, at 1e-8 at most 4e-8)."""
     lp = small_uniform()  # the oracle converges on it in 8720 iterations (FP64, KKT scheme)
     got = gpu_solve(lp, tolerance=1e-6, restart_scheme=scheme, precision=precision,
                     max_iterations=200000)
     assert got["status"] == "optimal", got["message"]
     k = orc.kkt(lp, got["primal"], got["dual"])
     assert k["score"] <= 1.5e-6, k
-    scale = 1.0 + np.abs(lp.objective * lp.x_star).sum()
-    assert abs(got["objective"] - lp.planted_objective) <= 1e-5 * scale
     if precision == mp.PRECISION_MIXED:
         assert got["fp32_iterations"] > 0
+    tight = gpu_solve(lp, tolerance=1e-8, restart_scheme=scheme, precision=precision, max_iterations=200000)
+    assert tight["status"] == "optimal", tight["message"]
+    assert orc.kkt(lp, tight["primal"], tight["dual"])["score"] <= 1.5e-8
+    p_star = lp.planted_objective
+    assert abs(tight["objective"] - p_star) <= 1e-6 * (1.0 + abs(p_star)), (tight["objective"], p_star)
 
 
 # scripts/run_netlib.py:20-93 optima of the Netlib instances in tests/golden/netlib
@@ -272,6 +275,22 @@ def test_repeated_solves_follow_their_own_options(gpu):
     m1 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1, **kw)
     m2 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1000000, **kw)
     assert m1["fp32_iterations"] < m2["fp32_iterations"]
+
+
+def test_repeated_solves_follow_adaptive_step_option(gpu):
+    """adaptive_step_size decides whether the captured iteration chunks hold the
+    A dx / eta-update launches.  Flipping only that option between solves of one
+    LP on the same engine (1, 0, 1) must follow each solve's own setting: every
+    run is bit-identical to the oracle with the same options, in reference order."""
+    lp = small_uniform(seed=23)
+    for adaptive in (1, 0, 1):
+        kw = dict(tolerance=1e-6, max_iterations=700, adaptive_step_size=adaptive)
+        print("adaptive_step_size", adaptive, flush=True)
+        want = orc.solve(lp, **kw)
+        got = gpu_solve(lp, reduction_order=mp.ORDER_REFERENCE, **kw)
+        assert got["iterations"] == want["iterations"] and got["objective"] == want["objective"]
+        assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
+        assert got["step_size"] == want["step_size"]
 
 
 def _infeasibility_cases():
