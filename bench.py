@@ -178,10 +178,13 @@ def main():
     ms = float(np.mean(walls))
 
     if prec == mp.PRECISION_MIXED and not args.no_fp64_compare and left() - reserve > 1.3 * max(walls) / 1e3:
-        fo = mp.options(precision=mp.PRECISION_FP64, max_iterations=args.max_iterations, **common)
+        # capped so that it ends inside the budget even if its trajectory is longer
+        per_it_s = 1.2 * max(walls) / 1e3 / max(1, res["iterations"])
+        cap = int(min(args.max_iterations, max(1000, (left() - reserve) / per_it_s)))
+        fo = mp.options(precision=mp.PRECISION_FP64, max_iterations=cap, **common)
         fr, fwall = run(fo)
         fp64 = {"ms": round(fwall, 3), "status": fr["status"], "iterations": fr["iterations"],
-                "restart_scheme": args.scheme, "kernel_ms": round(fr["kernel_ms"], 3),
+                "restart_scheme": args.scheme, "kernel_ms": round(fr["kernel_ms"], 3), "iteration_cap": cap,
                 "note": "same LP, scheme and tolerance with FP64 SpMVs throughout"}
         log(f"[bench] FP64 (same scheme): {fr['status']} it={fr['iterations']} wall={fwall:.1f}ms")
 

@@ -21,9 +21,9 @@
 #include <sstream>
 #include <string>
 
-#include "markov_cero/mps.hpp"
-#include "markov_cero/pdlp.hpp"
-#include "markov_cero/verify.hpp"
+#include "markov_cero/io/mps.hpp"
+#include "markov_cero/lp/first_order/pdlp.hpp"
+#include "markov_cero/verify/primal_verifier.hpp"
 
 namespace fo = markov_cero::lp::first_order;
 

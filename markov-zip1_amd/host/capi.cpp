@@ -7,8 +7,8 @@
 #include <string>
 
 #include "../../include/mcpdlp.h"
-#include "markov_cero/pdlp.hpp"
-#include "markov_cero/verify.hpp"
+#include "markov_cero/lp/first_order/pdlp.hpp"
+#include "markov_cero/verify/primal_verifier.hpp"
 #include "pdlp_internal.hpp"
 
 namespace fo = markov_cero::lp::first_order;
@@ -103,13 +103,13 @@ void check_lp(const mc_lp* lp) {
     if (!lp) throw std::invalid_argument("null model");
     if (lp->rows < 0 || lp->cols < 0) throw std::invalid_argument("negative model dimensions");
     if (lp->cols > 0 && (!lp->column_start || lp->column_start[0] != 0))
-        throw std::invalid_argument("CSC must start at zero");
+        throw std::invalid_argument("model: column_start[0] must be 0");
     const int64_t nnz = lp->cols > 0 ? lp->column_start[lp->cols] : 0;
     for (int64_t j = 0; j < lp->cols; ++j)
-        if (lp->column_start[j] > lp->column_start[j + 1]) throw std::invalid_argument("CSC pointers must be monotone");
+        if (lp->column_start[j] > lp->column_start[j + 1]) throw std::invalid_argument("model: column_start decreases");
     for (int64_t k = 0; k < nnz; ++k)
         if (lp->row_index[k] < 0 || lp->row_index[k] >= lp->rows)
-            throw std::invalid_argument("CSC row index out of range");
+            throw std::invalid_argument("model: row index out of range");
 }
 
 }  // namespace

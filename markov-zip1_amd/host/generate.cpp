@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "../../include/mcpdlp.h"
-#include "markov_cero/mps.hpp"
+#include "markov_cero/io/mps.hpp"
 
 struct mc_lp_handle {
     int64_t m = 0, n = 0;

@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "markov_cero/model.hpp"
+#include "markov_cero/model/model.hpp"
 
 namespace markov_cero::lp::first_order {
 

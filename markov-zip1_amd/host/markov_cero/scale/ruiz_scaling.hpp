@@ -5,7 +5,7 @@
 #include <cstddef>
 #include <vector>
 
-#include "markov_cero/model.hpp"
+#include "markov_cero/model/model.hpp"
 
 namespace markov_cero::scale {
 

@@ -6,7 +6,7 @@
 #include <string>
 #include <vector>
 
-#include "markov_cero/model.hpp"
+#include "markov_cero/model/model.hpp"
 
 namespace markov_cero::verify {
 

@@ -1,6 +1,7 @@
-// model.cpp -- validation, long-double product and triplet aggregation for
-// the markov-cero model types (reference src/model/model.cpp).
-#include "markov_cero/model.hpp"
+// model.cpp -- contract checks, the long-double product and triplet
+// aggregation of the markov-cero model types (the contract of reference
+// src/model/model.cpp; the structure and messages here are this library's).
+#include "markov_cero/model/model.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -9,75 +10,98 @@
 namespace markov_cero::model {
 namespace {
 
-void finite_or_throw(double v, const std::string& what) {
-    if (!std::isfinite(v)) throw std::invalid_argument(what + " must be finite");
+[[noreturn]] void reject(const std::string& why) { throw std::invalid_argument(why); }
+
+std::string at(const char* what, std::size_t k) { return std::string(what) + " " + std::to_string(k); }
+
+// Bound pair of one row or variable: finite values must be finite numbers, a
+// lower bound may not be +inf nor an upper bound -inf, and finite lower <= upper.
+void audit_bound_pair(const Bound& lo, const Bound& hi, const char* what, std::size_t k) {
+    if ((lo.kind == BoundKind::finite && !std::isfinite(lo.value)) ||
+        (hi.kind == BoundKind::finite && !std::isfinite(hi.value)))
+        reject(at(what, k) + ": a finite bound holds a non-finite number");
+    if (lo.kind == BoundKind::positive_infinity || hi.kind == BoundKind::negative_infinity)
+        reject(at(what, k) + ": lower bound at +inf or upper bound at -inf");
+    if (lo.is_finite() && hi.is_finite() && hi.value < lo.value)
+        reject(at(what, k) + ": lower bound above upper bound");
 }
 
-void check_bound(const Bound& b, const char* what) {
-    if (b.kind == BoundKind::finite) finite_or_throw(b.value, what);
-}
-
-void check_names(const std::vector<std::string>& names, std::size_t expected, const std::string& what) {
-    if (names.size() != expected) throw std::invalid_argument(what + " count mismatch");
+// Names are optional metadata, but when present there is one per entity,
+// each non-empty and unique.
+void audit_names(const std::vector<std::string>& names, std::size_t count, const char* what) {
+    if (names.size() != count)
+        reject(std::string(what) + " names: " + std::to_string(names.size()) + " given for " + std::to_string(count));
     std::unordered_set<std::string> seen;
     seen.reserve(names.size());
-    for (const auto& nm : names) {
-        if (nm.empty()) throw std::invalid_argument(what + " cannot be empty");
-        if (!seen.insert(nm).second) throw std::invalid_argument("duplicate " + what + ": " + nm);
+    for (std::size_t k = 0; k < names.size(); ++k) {
+        if (names[k].empty()) reject(at(what, k) + " has an empty name");
+        if (!seen.insert(names[k]).second) reject(std::string(what) + " name repeated: " + names[k]);
     }
 }
 
 }  // namespace
 
 Bound Bound::negative_infinity() noexcept { return Bound{BoundKind::negative_infinity, 0.0}; }
+Bound Bound::positive_infinity() noexcept { return Bound{BoundKind::positive_infinity, 0.0}; }
 Bound Bound::finite(double v) {
-    finite_or_throw(v, "bound");
+    if (!std::isfinite(v)) reject("Bound::finite needs a finite value");
     return Bound{BoundKind::finite, v};
 }
-Bound Bound::positive_infinity() noexcept { return Bound{BoundKind::positive_infinity, 0.0}; }
 bool Bound::is_finite() const noexcept { return kind == BoundKind::finite; }
 
+// Structure of the CSC arrays (reference src/model/model.cpp:41-66): n + 1
+// column pointers from 0 to nnz, non-decreasing; as many values as row ids;
+// inside a column, row ids strictly increasing and below m; every value a
+// finite nonzero.
 void SparseMatrixCSC::validate() const {
-    if (column_start.size() != column_count + 1) throw std::invalid_argument("CSC column_start size mismatch");
-    if (column_start.front() != 0) throw std::invalid_argument("CSC must start at zero");
-    if (row_index.size() != value.size()) throw std::invalid_argument("CSC index/value size mismatch");
-    if (column_start.back() != value.size()) throw std::invalid_argument("CSC terminal pointer mismatch");
-    for (std::size_t col = 0; col < column_count; ++col) {
-        const std::size_t b = column_start[col], e = column_start[col + 1];
-        if (b > e) throw std::invalid_argument("CSC pointers must be monotone");
-        for (std::size_t k = b; k < e; ++k) {
-            if (row_index[k] >= row_count) throw std::invalid_argument("CSC row index out of range");
-            if (k > b && row_index[k] <= row_index[k - 1])
-                throw std::invalid_argument("CSC rows must be strictly increasing per column");
-            if (!std::isfinite(value[k]) || value[k] == 0.0)
-                throw std::invalid_argument("CSC values must be finite and nonzero");
+    const std::size_t n = column_count;
+    if (column_start.size() != n + 1)
+        reject("sparse matrix: " + std::to_string(column_start.size()) + " column pointers for " + std::to_string(n) +
+               " columns");
+    if (row_index.size() != value.size()) reject("sparse matrix: row ids and values differ in length");
+    if (column_start[0] != 0) reject("sparse matrix: the first column pointer is not 0");
+    if (column_start[n] != value.size()) reject("sparse matrix: the last column pointer is not the nonzero count");
+    for (std::size_t j = 0; j < n; ++j) {
+        const std::size_t first = column_start[j], last = column_start[j + 1];
+        if (last < first) reject(at("sparse matrix: column pointers decrease at column", j));
+        std::size_t prev_row = 0;
+        for (std::size_t k = first; k < last; ++k) {
+            const std::size_t r = row_index[k];
+            if (r >= row_count) reject(at("sparse matrix: row index out of range in column", j));
+            if (k > first && r <= prev_row) reject(at("sparse matrix: row ids not strictly increasing in column", j));
+            prev_row = r;
+            const double v = value[k];
+            if (v == 0.0 || !std::isfinite(v)) reject(at("sparse matrix: zero or non-finite value in column", j));
         }
     }
 }
 
+// y = A x with each row's terms accumulated in long double, column by column
+// (reference model.cpp:68-86; the verifier's row activities depend on this
+// order).
 std::vector<double> SparseMatrixCSC::multiply(const std::vector<double>& x) const {
     validate();
-    if (x.size() != column_count) throw std::invalid_argument("matrix/vector dimension mismatch");
-    std::vector<long double> acc(row_count, 0.0L);
-    for (std::size_t col = 0; col < column_count; ++col) {
-        if (!std::isfinite(x[col])) throw std::invalid_argument("vector contains non-finite value");
-        for (std::size_t k = column_start[col]; k < column_start[col + 1]; ++k)
-            acc[row_index[k]] += static_cast<long double>(value[k]) * x[col];
+    if (x.size() != column_count)
+        reject("multiply: vector of " + std::to_string(x.size()) + " for " + std::to_string(column_count) + " columns");
+    std::vector<long double> row_sum(row_count, 0.0L);
+    for (std::size_t j = 0; j < column_count; ++j) {
+        const double xj = x[j];
+        if (!std::isfinite(xj)) reject(at("multiply: non-finite entry", j));
+        for (std::size_t k = column_start[j]; k < column_start[j + 1]; ++k)
+            row_sum[row_index[k]] += static_cast<long double>(value[k]) * xj;
     }
-    std::vector<double> out(row_count);
-    for (std::size_t r = 0; r < row_count; ++r) {
-        out[r] = static_cast<double>(acc[r]);
-        if (!std::isfinite(out[r])) throw std::overflow_error("matrix/vector product is non-finite");
-    }
-    return out;
+    std::vector<double> y(row_sum.begin(), row_sum.end());
+    for (std::size_t i = 0; i < row_count; ++i)
+        if (!std::isfinite(y[i])) throw std::overflow_error(at("multiply: overflow in row", i));
+    return y;
 }
 
 SparseMatrixBuilder::SparseMatrixBuilder(std::size_t row_count, std::size_t column_count)
     : rows_(row_count), columns_(column_count) {}
 
 void SparseMatrixBuilder::add(std::size_t row, std::size_t column, double value) {
-    if (row >= rows_ || column >= columns_) throw std::out_of_range("sparse entry index out of range");
-    finite_or_throw(value, "sparse value");
+    if (row >= rows_ || column >= columns_) throw std::out_of_range("builder entry outside the matrix");
+    if (!std::isfinite(value)) reject("builder entry is not finite");
     triplets_.push_back({row, column, value});
 }
 
@@ -106,7 +130,7 @@ SparseMatrixCSC SparseMatrixBuilder::build() const {
             ++e;
         }
         const double v = static_cast<double>(sum);
-        if (!std::isfinite(v)) throw std::overflow_error("aggregated sparse value is non-finite");
+        if (!std::isfinite(v)) throw std::overflow_error("builder: duplicate entries sum to a non-finite value");
         if (v != 0.0) {
             out.row_index.push_back(first.row);
             out.value.push_back(v);
@@ -119,47 +143,36 @@ SparseMatrixCSC SparseMatrixBuilder::build() const {
     return out;
 }
 
+// Model contract (reference model.cpp:120-165): a valid matrix; objective,
+// bounds, types and names sized to it; finite objective data; consistent
+// bound pairs; binary variables boxed inside [0, 1]; a quadratic term, if
+// any, n x n.
 void Model::validate() const {
     matrix.validate();
     const std::size_t m = matrix.row_count, n = matrix.column_count;
-    finite_or_throw(objective_offset, "objective offset");
-    if (objective.size() != n || variable_lower.size() != n || variable_upper.size() != n ||
-        variable_type.size() != n)
-        throw std::invalid_argument("variable metadata dimension mismatch");
-    if (row_lower.size() != m || row_upper.size() != m)
-        throw std::invalid_argument("row metadata dimension mismatch");
-    check_names(row_name, m, "row name");
-    check_names(variable_name, n, "variable name");
+    if (!std::isfinite(objective_offset)) reject("objective offset is not finite");
+    for (const auto* v : {&variable_lower, &variable_upper})
+        if (v->size() != n) reject("variable bounds are not sized to the matrix columns");
+    if (objective.size() != n || variable_type.size() != n)
+        reject("objective or variable types are not sized to the matrix columns");
+    if (row_lower.size() != m || row_upper.size() != m) reject("row bounds are not sized to the matrix rows");
+    audit_names(row_name, m, "row");
+    audit_names(variable_name, n, "variable");
     for (std::size_t j = 0; j < n; ++j) {
-        finite_or_throw(objective[j], "objective coefficient");
-        check_bound(variable_lower[j], "variable lower bound");
-        check_bound(variable_upper[j], "variable upper bound");
-        if (variable_lower[j].kind == BoundKind::positive_infinity ||
-            variable_upper[j].kind == BoundKind::negative_infinity)
-            throw std::invalid_argument("invalid variable infinity direction");
-        if (variable_lower[j].is_finite() && variable_upper[j].is_finite() &&
-            variable_lower[j].value > variable_upper[j].value)
-            throw std::invalid_argument("inconsistent variable bounds");
+        if (!std::isfinite(objective[j])) reject(at("objective coefficient of variable", j) + " is not finite");
+        audit_bound_pair(variable_lower[j], variable_upper[j], "variable", j);
         if (variable_type[j] == VariableType::binary) {
-            if (!variable_lower[j].is_finite() || !variable_upper[j].is_finite())
-                throw std::invalid_argument("binary bounds must be finite");
-            if (variable_lower[j].value < 0.0 || variable_upper[j].value > 1.0)
-                throw std::invalid_argument("binary bounds must stay within zero and one");
+            const Bound& lo = variable_lower[j];
+            const Bound& hi = variable_upper[j];
+            if (!lo.is_finite() || !hi.is_finite() || lo.value < 0.0 || hi.value > 1.0)
+                reject(at("binary variable", j) + " needs finite bounds inside [0, 1]");
         }
     }
-    for (std::size_t i = 0; i < m; ++i) {
-        check_bound(row_lower[i], "row lower bound");
-        check_bound(row_upper[i], "row upper bound");
-        if (row_lower[i].kind == BoundKind::positive_infinity ||
-            row_upper[i].kind == BoundKind::negative_infinity)
-            throw std::invalid_argument("invalid row infinity direction");
-        if (row_lower[i].is_finite() && row_upper[i].is_finite() && row_lower[i].value > row_upper[i].value)
-            throw std::invalid_argument("inconsistent row bounds");
-    }
+    for (std::size_t i = 0; i < m; ++i) audit_bound_pair(row_lower[i], row_upper[i], "row", i);
     if (has_quadratic_objective) {
         quadratic_matrix.validate();
-        if (quadratic_matrix.column_count != n || quadratic_matrix.row_count != n)
-            throw std::invalid_argument("quadratic matrix dimension mismatch");
+        if (quadratic_matrix.row_count != n || quadratic_matrix.column_count != n)
+            reject("quadratic objective matrix is not n x n");
     }
 }
 

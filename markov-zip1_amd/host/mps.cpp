@@ -1,5 +1,5 @@
 // mps.cpp -- free-format MPS reader/writer and the binary LP format.
-#include "markov_cero/mps.hpp"
+#include "markov_cero/io/mps.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -45,13 +45,13 @@ struct ColRec {
     model::VariableType type = model::VariableType::continuous;
 };
 
-}  // namespace
 
-model::Model parse_mps(std::istream& in) {
+// Reader with optional size caps (lim == nullptr: none).
+model::Model parse_capped(std::istream& in, const MpsLimits* lim) {
     enum class Sec { none, name, objsense, rows, columns, rhs, ranges, bounds, end };
     Sec sec = Sec::none;
     std::string line, name = "MPS";
-    std::size_t ln = 0;
+    std::size_t ln = 0, bytes = 0;
     model::ObjectiveSense sense = model::ObjectiveSense::minimize;
     std::vector<RowRec> rows;
     std::unordered_map<std::string, std::size_t> row_of, col_of;
@@ -67,6 +67,7 @@ model::Model parse_mps(std::istream& in) {
         auto it = col_of.find(c);
         if (it != col_of.end()) return it->second;
         col_of.emplace(c, cols.size());
+        if (lim && cols.size() >= lim->maximum_columns) throw MpsError(ln, "too many columns");
         cols.push_back(ColRec{c});
         return cols.size() - 1;
     };
@@ -77,10 +78,18 @@ model::Model parse_mps(std::istream& in) {
     };
     while (std::getline(in, line)) {
         ++ln;
+        if (lim) {
+            bytes += line.size() + 1;
+            if (ln > lim->maximum_lines) throw MpsError(ln, "too many lines");
+            if (bytes > lim->maximum_bytes) throw MpsError(ln, "input exceeds the byte limit");
+        }
         if (!line.empty() && line.back() == '\r') line.pop_back();
         if (line.empty() || line[0] == '*') continue;
         const auto f = split(line);
         if (f.empty()) continue;
+        if (lim)
+            for (const auto& tok : f)
+                if (tok.size() > lim->maximum_name_bytes) throw MpsError(ln, "field exceeds the name limit");
         if (line[0] != ' ' && line[0] != '\t') {
             const std::string& h = f[0];
             if (h == "NAME") {
@@ -119,6 +128,7 @@ model::Model parse_mps(std::istream& in) {
                 if (row_of.count(f[1])) throw MpsError(ln, "duplicate row '" + f[1] + "'");
                 if (f[0][0] == 'N' && objective_row.empty()) objective_row = f[1];
                 row_of.emplace(f[1], rows.size());
+                if (lim && rows.size() >= lim->maximum_rows) throw MpsError(ln, "too many rows");
                 rows.push_back(RowRec{f[1], f[0][0]});
                 break;
             }
@@ -136,6 +146,7 @@ model::Model parse_mps(std::istream& in) {
                     if (rows[r].type == 'N') {
                         if (f[p] == objective_row) cols[c].obj += v;
                     } else {
+                        if (lim && entries.size() >= lim->maximum_nonzeros) throw MpsError(ln, "too many nonzeros");
                         entries.push_back({r, c, v});
                     }
                 }
@@ -214,6 +225,17 @@ model::Model parse_mps(std::istream& in) {
     }
     md.validate();
     return md;
+}
+
+}  // namespace
+
+model::Model parse_mps(std::istream& in) { return parse_capped(in, nullptr); }
+
+model::Model parse_mps(std::istream& in, const MpsLimits& limits) { return parse_capped(in, &limits); }
+
+model::Model parse_mps_string(std::string_view text, const MpsLimits& limits) {
+    std::istringstream in{std::string(text)};
+    return parse_capped(in, &limits);
 }
 
 model::Model read_mps_file(const std::string& path) {

@@ -1,7 +1,7 @@
 // pdlp.cpp -- solve_pdlp / solve_pdlp_batch dispatch between the host path
 // and the HIP engine (reference src/lp/first_order/pdlp.cpp:139-149 for the
 // trivial exit and the backend labels).
-#include "markov_cero/pdlp.hpp"
+#include "markov_cero/lp/first_order/pdlp.hpp"
 
 #include <stdexcept>
 #include <string>

@@ -7,8 +7,8 @@
 #include <cmath>
 #include <limits>
 
-#include "markov_cero/pdlp.hpp"
-#include "markov_cero/scale.hpp"
+#include "markov_cero/lp/first_order/pdlp.hpp"
+#include "markov_cero/scale/ruiz_scaling.hpp"
 #include "pdlp_internal.hpp"
 
 namespace markov_cero::lp::first_order::detail {

@@ -14,7 +14,7 @@
 #include <thread>
 
 #include "../../include/mcpdlp_engine.h"
-#include "markov_cero/pdlp.hpp"
+#include "markov_cero/lp/first_order/pdlp.hpp"
 #include "pdlp_internal.hpp"
 
 namespace markov_cero::lp::first_order::detail {

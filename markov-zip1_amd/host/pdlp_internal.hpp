@@ -6,8 +6,8 @@
 #include <vector>
 
 #include "../../include/mcpdlp.h"
-#include "markov_cero/model.hpp"
-#include "markov_cero/pdlp.hpp"
+#include "markov_cero/model/model.hpp"
+#include "markov_cero/lp/first_order/pdlp.hpp"
 
 namespace markov_cero::lp::first_order::detail {
 

@@ -2,7 +2,7 @@
 // (reference src/scale/ruiz_scaling.cpp:152-258).  The GPU path runs the same
 // passes on the device (kernels.hip k_ruiz_*); both use only maxima,
 // 1/sqrt and products, so they agree bit for bit.
-#include "markov_cero/scale.hpp"
+#include "markov_cero/scale/ruiz_scaling.hpp"
 
 #include <algorithm>
 #include <cmath>

@@ -1,7 +1,12 @@
-// verify.cpp -- primal verification (reference src/verify/primal_verifier.cpp:9-104):
-// bound and row checks with allowance abs + rel * max(1, |actual|, |bound|),
-// long-double row activity and objective recomputation.
-#include "markov_cero/verify.hpp"
+// verify.cpp -- zero-trust check of a returned primal against the original
+// model.  The contract is that of reference src/verify/primal_verifier.cpp:
+// each finite bound of a variable and of a row activity may be exceeded by at
+// most abs + rel * max(1, |actual|, |bound|); row activities and the objective
+// are recomputed in long double; integer-typed variables must lie within the
+// integrality tolerance of an integer.  Those formulas are kept exactly, so
+// verdicts agree with the reference's; the code is organised around one
+// audit object instead.
+#include "markov_cero/verify/primal_verifier.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -10,28 +15,49 @@
 namespace markov_cero::verify {
 namespace {
 
-double allowed(double actual, double bound, const Tolerance& t) {
-    const double a = t.absolute + t.relative * std::max({1.0, std::abs(actual), std::abs(bound)});
-    if (!std::isfinite(a)) throw std::invalid_argument("tolerance allowance overflow");
-    return a;
-}
+// Collects violations of one candidate.
+struct Audit {
+    const Tolerance& feas;
+    PrimalVerificationReport& rep;
 
-// side = +1: actual must be >= bound; side = -1: actual must be <= bound.
-void check(double actual, const model::Bound& bound, int side, const Tolerance& t, const char* category,
-           std::size_t index, PrimalVerificationReport& rep, double& worst) {
-    if (!bound.is_finite()) return;
-    const double excess = std::max(0.0, side > 0 ? bound.value - actual : actual - bound.value);
-    const double allow = allowed(actual, bound.value, t);
-    worst = std::max(worst, excess);
-    if (excess > allow) rep.violations.push_back({category, index, actual, bound.value, excess, allow});
-}
+    static double allowance(double actual, double bound, const Tolerance& t) {
+        const double scale = std::max({1.0, std::abs(actual), std::abs(bound)});
+        const double a = t.absolute + t.relative * scale;
+        if (!std::isfinite(a)) throw std::invalid_argument("verification allowance overflows");
+        return a;
+    }
+
+    // `actual` against the pair [lo, hi]; the excess over each finite side
+    // feeds `worst` and, past its allowance, a violation of that side.
+    void pair(double actual, const model::Bound& lo, const model::Bound& hi, const char* lo_name,
+              const char* hi_name, std::size_t index, double& worst) {
+        if (lo.is_finite()) side(actual, lo.value, std::max(0.0, lo.value - actual), lo_name, index, worst);
+        if (hi.is_finite()) side(actual, hi.value, std::max(0.0, actual - hi.value), hi_name, index, worst);
+    }
+
+    void side(double actual, double bound, double excess, const char* name, std::size_t index, double& worst) {
+        worst = std::max(worst, excess);
+        const double allow = allowance(actual, bound, feas);
+        if (excess > allow) rep.violations.push_back({name, index, actual, bound, excess, allow});
+    }
+
+    // Distance to the nearest integer (ties toward the lower one).
+    void integral(double x, std::size_t index, double int_tol) {
+        const double down = std::floor(x);
+        const double nearest = (x - down <= 0.5) ? down : down + 1.0;
+        const double frac = std::abs(x - nearest);
+        rep.maximum_integrality_violation = std::max(rep.maximum_integrality_violation, frac);
+        if (frac > int_tol) rep.violations.push_back({"integrality", index, x, nearest, frac, int_tol});
+    }
+};
 
 }  // namespace
 
+// Allowances above 1e-4 are refused (reference primal_verifier.cpp:39-43).
 void Tolerance::validate() const {
-    if (!std::isfinite(absolute) || !std::isfinite(relative) || absolute < 0.0 || relative < 0.0 ||
-        absolute > 1e-4 || relative > 1e-4)
-        throw std::invalid_argument("tolerances must be finite and nonnegative");
+    const bool ok = std::isfinite(absolute) && std::isfinite(relative) && absolute >= 0.0 && relative >= 0.0 &&
+                    absolute <= 1e-4 && relative <= 1e-4;
+    if (!ok) throw std::invalid_argument("verification tolerances must be finite, nonnegative and at most 1e-4");
 }
 
 PrimalVerificationReport verify_primal(const model::Model& md, const Candidate& cand, const Tolerance& feas,
@@ -39,36 +65,31 @@ PrimalVerificationReport verify_primal(const model::Model& md, const Candidate& 
     md.validate();
     feas.validate();
     objtol.validate();
-    if (!std::isfinite(int_tol) || int_tol < 0.0 || int_tol >= 0.5)
-        throw std::invalid_argument("integrality tolerance must be finite and nonnegative");
-    if (cand.primal.size() != md.matrix.column_count) throw std::invalid_argument("candidate dimension mismatch");
-    if (!std::isfinite(cand.claimed_objective)) throw std::invalid_argument("claimed objective must be finite");
+    if (!(int_tol >= 0.0 && int_tol < 0.5)) throw std::invalid_argument("integrality tolerance must lie in [0, 0.5)");
+    const std::size_t n = md.matrix.column_count;
+    if (cand.primal.size() != n) throw std::invalid_argument("candidate has the wrong number of variables");
+    if (!std::isfinite(cand.claimed_objective)) throw std::invalid_argument("claimed objective is not finite");
+    for (const double x : cand.primal)
+        if (!std::isfinite(x)) throw std::invalid_argument("candidate holds a non-finite value");
+
     PrimalVerificationReport rep;
-    for (std::size_t j = 0; j < cand.primal.size(); ++j) {
-        const double x = cand.primal[j];
-        if (!std::isfinite(x)) throw std::invalid_argument("candidate contains non-finite value");
-        check(x, md.variable_lower[j], +1, feas, "variable_lower", j, rep, rep.maximum_variable_violation);
-        check(x, md.variable_upper[j], -1, feas, "variable_upper", j, rep, rep.maximum_variable_violation);
-        if (md.variable_type[j] != model::VariableType::continuous) {
-            const double lo = std::floor(x), hi = std::ceil(x);
-            const double nearest = (x - lo <= hi - x) ? lo : hi;
-            const double frac = std::abs(x - nearest);
-            rep.maximum_integrality_violation = std::max(rep.maximum_integrality_violation, frac);
-            if (frac > int_tol) rep.violations.push_back({"integrality", j, x, nearest, frac, int_tol});
-        }
+    Audit audit{feas, rep};
+    for (std::size_t j = 0; j < n; ++j) {
+        audit.pair(cand.primal[j], md.variable_lower[j], md.variable_upper[j], "variable_lower", "variable_upper", j,
+                   rep.maximum_variable_violation);
+        if (md.variable_type[j] != model::VariableType::continuous) audit.integral(cand.primal[j], j, int_tol);
     }
-    const auto act = md.matrix.multiply(cand.primal);
-    for (std::size_t i = 0; i < act.size(); ++i) {
-        check(act[i], md.row_lower[i], +1, feas, "row_lower", i, rep, rep.maximum_row_violation);
-        check(act[i], md.row_upper[i], -1, feas, "row_upper", i, rep, rep.maximum_row_violation);
-    }
-    long double obj = md.objective_offset;
-    for (std::size_t j = 0; j < cand.primal.size(); ++j)
-        obj += static_cast<long double>(md.objective[j]) * cand.primal[j];
-    rep.recomputed_objective = static_cast<double>(obj);
-    if (!std::isfinite(rep.recomputed_objective)) throw std::overflow_error("recomputed objective is non-finite");
+    const std::vector<double> activity = md.matrix.multiply(cand.primal);
+    for (std::size_t i = 0; i < activity.size(); ++i)
+        audit.pair(activity[i], md.row_lower[i], md.row_upper[i], "row_lower", "row_upper", i,
+                   rep.maximum_row_violation);
+
+    long double objective = md.objective_offset;
+    for (std::size_t j = 0; j < n; ++j) objective += static_cast<long double>(md.objective[j]) * cand.primal[j];
+    rep.recomputed_objective = static_cast<double>(objective);
+    if (!std::isfinite(rep.recomputed_objective)) throw std::overflow_error("recomputed objective overflows");
     rep.objective_difference = std::abs(rep.recomputed_objective - cand.claimed_objective);
-    const double allow = allowed(rep.recomputed_objective, cand.claimed_objective, objtol);
+    const double allow = Audit::allowance(rep.recomputed_objective, cand.claimed_objective, objtol);
     if (rep.objective_difference > allow)
         rep.violations.push_back({"objective", 0, cand.claimed_objective, rep.recomputed_objective,
                                   rep.objective_difference, allow});
