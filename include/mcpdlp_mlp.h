@@ -35,6 +35,19 @@ int mcp_mlp_get_params(mcp_mlp* m, float* params);
 int mcp_mlp_set_params(mcp_mlp* m, const float* params);
 /* y[n x out] = net(x[n x in]) (row-major host arrays). */
 int mcp_mlp_forward(mcp_mlp* m, const float* x, int32_t n, float* y);
+/* Allocation-free forward pass.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  x_dev (n x in) and y_dev (n x out) are row-major
+ * FP32 arrays in device memory on the network's device, ready when the call
+ * is made; the call returns once y_dev is written and gives the same values
+ * as mcp_mlp_forward.  It calls no hipMalloc/hipFree: the padded input, the
+ * pre-activations and the activations live in a static 16 MiB device array
+ * of the library (one per device, kept until the process ends), which bounds
+ * n to 16 MiB / (8 x widest layer) rounded down to a multiple of 32 (8,192
+ * samples for the 256-wide network); a larger n is refused with
+ * MC_ERR_INVALID_ARGUMENT.  All calls on one device share that array, so
+ * calls that overlap on one device (from several threads) race. */
+int mcp_mlp_forward_noalloc(mcp_mlp* m, const float* x_dev, int32_t n, float* y_dev);
 /* Loss of one batch (n <= capacity) and, if grad != NULL, the gradient
  * (param_count floats) -- the device backward pass, for gradient checks. */
 int mcp_mlp_loss_grad(mcp_mlp* m, const float* x, const float* target, int32_t n, float* loss, float* grad);

@@ -21,6 +21,7 @@
 
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -150,6 +151,27 @@ __global__ void k_to_bf16(int64_t n, const float* src, uint16_t* dst) {
     if (k < n) dst[k] = f2bf(src[k]);
 }
 
+// BF16 copy of the first `valid` rows of x (rows x cols, row-major) with
+// zero padding rows up to `rows`.
+__global__ void k_rows_to_bf16(int valid, int rows, int cols, const float* x, uint16_t* dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)rows * cols) return;
+    dst[k] = (k / cols) < valid ? f2bf(x[k]) : (uint16_t)0;
+}
+
+__global__ void k_copy_f32(int64_t n, const float* src, float* dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+// Device scratch of mcp_mlp_forward_noalloc: the padded BF16 input, the
+// pre-activation of the current layer (FP32) and the BF16 activations, ping-
+// ponged between layers.  One copy per device, held for the life of the
+// process; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device.
+constexpr size_t kNoAllocScratchBytes = 16u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 thread_local std::string g_mlp_err;
 
 struct MlpError : std::runtime_error {
@@ -180,6 +202,25 @@ int mguard(F&& f) {
 }
 
 int blocks(int64_t n) { return (int)((n + 255) / 256); }
+
+// Device address of `scratch`, looked up once per device with that device
+// current (each device holds its own copy at its own address).
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+void* scratch_on(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        MCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw MlpError(MC_ERR_DEVICE, "no device address for the forward-pass scratch");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 }  // namespace
 
@@ -235,6 +276,24 @@ struct mcp_mlp {
             hipLaunchKernelGGL(k_bias_act, dim3(blocks((int64_t)rows * N)), dim3(256), 0, s, rows, N, Z[l],
                                p + boff[l], last ? nullptr : Abf[l + 1], last ? 0 : 1);
         }
+    }
+
+    // The forward pass of forward() with every intermediate in caller-given
+    // memory: a0/a1 BF16 activation ping-pong (rows x maxw), z FP32 (rows x
+    // maxw); x (valid x in, FP32) and y (valid x out, FP32) are device arrays.
+    void forward_into(const float* x, int valid, int rows, uint16_t* a0, uint16_t* a1, float* z, float* y) {
+        const int in = dims[0], out = dims.back();
+        hipLaunchKernelGGL(k_rows_to_bf16, dim3(blocks((int64_t)rows * in)), dim3(256), 0, s, valid, rows, in, x, a0);
+        for (int l = 0; l < layers(); ++l) {
+            const int K = dims[l], N = dims[l + 1];
+            gemm(rows, N, K, a0, K, 1, wbf + woff[l], N, 1, z, N, false);
+            const bool last = l == layers() - 1;
+            hipLaunchKernelGGL(k_bias_act, dim3(blocks((int64_t)rows * N)), dim3(256), 0, s, rows, N, z, p + boff[l],
+                               last ? nullptr : a1, last ? 0 : 1);
+            std::swap(a0, a1);
+        }
+        hipLaunchKernelGGL(k_copy_f32, dim3(blocks((int64_t)valid * out)), dim3(256), 0, s, (int64_t)valid * out, z,
+                           y);
     }
 
     // backward from dY (fp32 in dA, bf16 in dbf) -> grads in g
@@ -375,6 +434,31 @@ int mcp_mlp_forward(mcp_mlp* m, const float* x, int32_t n, float* y) {
                                 hipMemcpyDeviceToHost, m->s));
             MCHK(hipStreamSynchronize(m->s));
         }
+    });
+}
+
+int mcp_mlp_forward_noalloc(mcp_mlp* m, const float* x_dev, int32_t n, float* y_dev) {
+    return mguard([&] {
+        if (!m || n < 0 || (n > 0 && (!x_dev || !y_dev))) throw MlpError(MC_ERR_INVALID_ARGUMENT, "null argument");
+        MCHK(hipSetDevice(m->device));
+        char* base = static_cast<char*>(scratch_on(m->device));
+        if (n == 0) return;
+        const int rows = (n + 31) / 32 * 32;
+        const size_t act = align256((size_t)rows * m->maxw() * sizeof(uint16_t));
+        const size_t pre = align256((size_t)rows * m->maxw() * sizeof(float));
+        if (2 * act + pre > kNoAllocScratchBytes)
+            throw MlpError(MC_ERR_INVALID_ARGUMENT,
+                           "mcp_mlp_forward_noalloc: " + std::to_string(n) + " samples need " +
+                               std::to_string(2 * act + pre) + " bytes of scratch; the maximum is " +
+                               std::to_string(kNoAllocScratchBytes) + " (" +
+                               std::to_string(kNoAllocScratchBytes / (8 * (size_t)m->maxw()) / 32 * 32) +
+                               " samples at this width)");
+        uint16_t* a0 = reinterpret_cast<uint16_t*>(base);
+        uint16_t* a1 = reinterpret_cast<uint16_t*>(base + act);
+        float* z = reinterpret_cast<float*>(base + 2 * act);
+        m->forward_into(x_dev, n, rows, a0, a1, z, y_dev);
+        MCHK(hipGetLastError());
+        MCHK(hipStreamSynchronize(m->s));
     });
 }
 
