@@ -108,6 +108,10 @@ typedef struct mcp_solve_opts {
     int32_t detect_infeasibility;     /* 1: stop on a ray certificate (status infeasible_or_unbounded) */
     int32_t pad2_;
     double infeasibility_tolerance;   /* certificate tolerance; 0 => 1e-8 */
+    uint64_t power_iterations;        /* Halpern scheme with adaptive steps: power iterations that
+                                         estimate L = ||D_r^-1/2 A D_c^-1/2|| per instance, and the
+                                         step factor eta is capped at 0.95 / L instead of 0.99
+                                         (0 => 0.99; reference option pdlp.hpp:25, unread there) */
 } mcp_solve_opts;
 
 typedef struct mcp_solve_stats {

@@ -62,6 +62,8 @@ struct InstState {
     double ruiz_err_r, ruiz_err_c;  // setup: max |1 - norm| of the current pass
     int32_t ruiz_done, ruiz_passes, ruiz_converged, pad_;
     double b_norm_inf, c_norm_inf;  // setup: norms of the scaled b / c (floored at 1)
+    double eta_cap;          // largest step factor of the adaptive rule: 0.99, or 0.95 / L
+                             // (Halpern scheme, L the power-method norm of the scaled matrix)
 };
 
 // Options every iteration kernel reads (passed by value).

@@ -89,6 +89,7 @@ struct Dev {
     double* part_ray_cols;  // [tile*7 + k]: primal-ray bound violation, c^T dx and its magnitude,
                             //   dual-ray objective part and its magnitude, dual-ray column
                             //   violation, |dx|_inf
+    double* pw;         // [2B] step-cap power method: S_b = |v_b|^2, and 1 once S_b stopped being > 0
     InstState* st;      // [B]
     int64_t* iter_base; // global iteration counter at the start of the current chunk
     int32_t* counters;  // [0] running instances, [1] running instances in FP32
@@ -129,6 +130,11 @@ void launch_post_check(const Dev& d, bool kkt, hipStream_t s);
 // restart (after the check, before the restart copy).
 void launch_ray_check(const Dev& d, const IterOpts& o, int t, hipStream_t s);
 void launch_advance(const Dev& d, int len, hipStream_t s);
+// Halpern-scheme step cap: `iters` power iterations per instance on M^T M,
+// M = D_r^-1/2 A D_c^-1/2 (scaled A, D the floored L1 norms), then
+// st[b].eta_cap = 0.95 / L_b (0.99 if the estimate fails); exact => every sum
+// in the serial order of the oracle.  Uses ctmp, rtmp and aty2 as scratch.
+void launch_step_cap(const Dev& d, int64_t iters, bool exact, hipStream_t s);
 // Setup kernels (run on the current tiles).
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);
 void launch_setup_data(const Dev& d, hipStream_t s);

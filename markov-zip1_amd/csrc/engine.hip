@@ -621,6 +621,7 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         d.ax2 = e->buf<double>("ax2", m);
         d.aty2 = e->buf<double>("aty2", n);
         d.st = e->buf<mcp::InstState>("st", B);
+        d.pw = e->buf<double>("pw", 2 * (size_t)B);
         d.iter_base = e->buf<int64_t>("iter_base", 1);
         d.counters = e->buf<int32_t>("counters", 4);
         HIPCHK(hipStreamSynchronize(s));
@@ -729,6 +730,8 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         // a warm start's accuracy, so a warm-started solve begins in FP64.
         const bool fp32_start = mixed && !wx && !wy;
         mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, fp32_start, s);
+        if (io.halpern && io.adaptive_step && o->power_iterations > 0)
+            mcp::launch_step_cap(d, (int64_t)o->power_iterations, exact, s);
         int32_t* pin = e->pinned;
         pin[4] = e->B;
         pin[5] = fp32_start ? e->B : 0;

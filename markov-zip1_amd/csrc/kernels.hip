@@ -216,8 +216,8 @@ __device__ void adapt_decide(InstState& s, double dxn, double adxn) {
             const double target = 0.95 / L;
             if (target < s.eta)
                 s.eta = maxv(0.1, maxv(target, s.eta * 0.8));
-            else if (target > 1.05 * s.eta && s.eta < 0.99)
-                s.eta = minv(0.99, s.eta * 1.05);
+            else if (target > 1.05 * s.eta && s.eta < s.eta_cap)
+                s.eta = minv(s.eta_cap, s.eta * 1.05);
         }
     }
 }
@@ -612,7 +612,13 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
                 bool upd = false;
                 if (dX > 1e-10 && dY > 1e-10) {
                     const double th = o.smoothing, ratio = dY / dX;
-                    w = (th == 0.5) ? sqrt(s.omega * ratio) : pow(ratio, th) * pow(s.omega, 1.0 - th);
+                    if (o.halpern) {  // ratio^(127/128) omega^(1/128) in square roots
+                        double f = s.omega / ratio;
+                        for (int q = 0; q < 7; ++q) f = sqrt(f);
+                        w = ratio * f;
+                    } else {
+                        w = (th == 0.5) ? sqrt(s.omega * ratio) : pow(ratio, th) * pow(s.omega, 1.0 - th);
+                    }
                     upd = true;
                 }
                 if (o.halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots
@@ -1055,6 +1061,7 @@ __global__ void k_setup_state(Dev d, double initial_weight, double step_reductio
     s.omega = (initial_weight > 0.0) ? initial_weight
                                      : clampv(sqrt(s.c_norm_inf / s.b_norm_inf), 0.01, 100.0);
     s.eta = clampv(step_reduction, 0.1, 0.99);
+    s.eta_cap = 0.99;  // launch_step_cap may raise it (Halpern scheme)
     s.last_restart_score = 1e300;
     s.best_score = 1e300;
     s.restart_iter = 0;
@@ -1105,6 +1112,71 @@ __global__ __launch_bounds__(kBlock) void k_spmv_plain(const Tile* tiles, int nt
         const T v = row_dot<T>(own ? ptr[r] : 0, own ? ptr[r + 1] : 0, L.gl, L.G, idx, val, vec);
         if (own && L.gl == 0) out[r] = v;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Step cap of the Halpern scheme (the oracle's or_step_cap, per instance):
+// v_j = 1 + (j mod 7)/1000 (j local to the instance), S = |v|^2; per iteration
+// t = (v / sqrt(S)) / sqrt(cnorm), u = (A t) / rnorm, v = (A^T u) / sqrt(cnorm),
+// S = |v|^2; L = sqrt(sqrt(S)), cap = 0.95 / L.  v lives in ctmp, u in rtmp,
+// t in aty2; pw[b] = S, pw[B + b] = 1 once an S that starts an iteration was
+// not > 0 (the oracle stops there and keeps 0.99).
+__global__ void k_pw_init(Dev d) {  // grid covers max(n, B) threads
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < d.B) d.pw[d.B + j] = 0.0;
+    if (j >= d.n) return;
+    const int jl = j - d.inst_col[d.col_inst[j]];
+    d.ctmp[j] = 1.0 + (double)(jl % 7) / 1000.0;
+}
+
+// S_b = sum of v_j^2 over the instance's columns; `check`: S_b starts an
+// iteration, so a non-positive (or NaN) S_b marks the estimate failed.
+template <bool EXACT>
+__global__ __launch_bounds__(kFinal) void k_pw_sum(Dev d, int check) {
+    const int b = blockIdx.x;
+    const int j0 = d.inst_col[b], j1 = d.inst_col[b + 1];
+    double S = 0.0;
+    if (EXACT) {
+        if (threadIdx.x != 0) return;
+        for (int j = j0; j < j1; ++j) S += d.ctmp[j] * d.ctmp[j];
+    } else {
+        __shared__ double red[kFinal / 64];
+        double a0 = 0.0, a1 = 0.0;
+        int j = j0 + (int)threadIdx.x;
+        for (; j + kFinal < j1; j += 2 * kFinal) {
+            a0 += d.ctmp[j] * d.ctmp[j];
+            a1 += d.ctmp[j + kFinal] * d.ctmp[j + kFinal];
+        }
+        for (; j < j1; j += kFinal) a0 += d.ctmp[j] * d.ctmp[j];
+        S = block_sum<kFinal / 64>(a0 + a1, red);
+        if (threadIdx.x != 0) return;
+    }
+    d.pw[b] = S;
+    if (check && !(S > 0.0)) d.pw[d.B + b] = 1.0;
+}
+
+__global__ void k_pw_cols_in(Dev d) {  // t = (v / sqrt(S)) / sqrt(cnorm)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.n) return;
+    const double nv = sqrt(d.pw[d.col_inst[j]]);
+    d.aty2[j] = (d.ctmp[j] / nv) / sqrt(d.cnorm[j]);
+}
+
+__global__ void k_pw_rows(Dev d) {  // u = u / rnorm
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.m) d.rtmp[i] = d.rtmp[i] / d.rnorm[i];
+}
+
+__global__ void k_pw_cols_out(Dev d) {  // v = v / sqrt(cnorm)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < d.n) d.ctmp[j] = d.ctmp[j] / sqrt(d.cnorm[j]);
+}
+
+__global__ void k_pw_finish(Dev d) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.B) return;
+    const double L = sqrt(sqrt(d.pw[b]));
+    d.st[b].eta_cap = (d.pw[d.B + b] == 0.0 && L > 0.0 && L < 1e300) ? 0.95 / L : 0.99;
 }
 
 // ---------------------------------------------------------------------------
@@ -1265,6 +1337,28 @@ void launch_setup_state(const Dev& d, double initial_weight, double step_reducti
                         hipStream_t s) {
     hipLaunchKernelGGL(k_setup_state, dim3(elem_grid(d.B)), dim3(kBlock), 0, s, d, initial_weight,
                        step_reduction, mixed ? 1 : 0);
+}
+
+void launch_step_cap(const Dev& d, int64_t iters, bool exact, hipStream_t s) {
+    if (iters <= 0 || d.n == 0 || d.B == 0) return;
+    const dim3 b(kBlock), gn(elem_grid(d.n)), gm(elem_grid(d.m > 0 ? d.m : 1));
+    auto sum = [&](int check) {
+        if (exact)
+            hipLaunchKernelGGL((k_pw_sum<true>), dim3(d.B), dim3(kFinal), 0, s, d, check);
+        else
+            hipLaunchKernelGGL((k_pw_sum<false>), dim3(d.B), dim3(kFinal), 0, s, d, check);
+    };
+    hipLaunchKernelGGL(k_pw_init, dim3(elem_grid(d.n > d.B ? d.n : d.B)), b, 0, s, d);
+    sum(1);
+    for (int64_t k = 0; k < iters; ++k) {
+        hipLaunchKernelGGL(k_pw_cols_in, gn, b, 0, s, d);
+        launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av64, d.aty2, d.rtmp, false, s);
+        if (d.m) hipLaunchKernelGGL(k_pw_rows, gm, b, 0, s, d);
+        launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv64, d.rtmp, d.ctmp, false, s);
+        hipLaunchKernelGGL(k_pw_cols_out, gn, b, 0, s, d);
+        sum(k + 1 < iters ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_pw_finish, dim3(elem_grid(d.B)), b, 0, s, d);
 }
 
 void launch_unscale(const Dev& d, double* x_out, double* y_out, hipStream_t s) {

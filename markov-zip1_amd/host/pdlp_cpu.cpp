@@ -47,6 +47,32 @@ struct Pdhg {
             for (std::size_t k = A.column_start[j]; k < A.column_start[j + 1]; ++k) z[A.row_index[k]] += A.value[k] * vj;
         }
     }
+    // Halpern-scheme step cap: power iterations on M^T M, M = D_r^-1/2 A D_c^-1/2,
+    // from v_j = 1 + (j mod 7)/1000, give L <= ||M||; cap = 0.95 / L (0.99, the
+    // reference's cap, when the estimate fails).  Serial sums in index order.
+    double step_cap(std::size_t iters) const {
+        std::vector<double> v(n), t(n), u(m);
+        double s = 0.0;
+        for (std::size_t j = 0; j < n; ++j) {
+            v[j] = 1.0 + static_cast<double>(j % 7) / 1000.0;
+            s += v[j] * v[j];
+        }
+        std::size_t k = 0;
+        for (; k < iters && s > 0.0; ++k) {
+            const double nv = std::sqrt(s);
+            for (std::size_t j = 0; j < n; ++j) t[j] = (v[j] / nv) / std::sqrt(cnorm[j]);
+            a_times(t, u);
+            for (std::size_t i = 0; i < m; ++i) u[i] = u[i] / rnorm[i];
+            at_times(u, v);
+            s = 0.0;
+            for (std::size_t j = 0; j < n; ++j) {
+                v[j] = v[j] / std::sqrt(cnorm[j]);
+                s += v[j] * v[j];
+            }
+        }
+        const double L = std::sqrt(std::sqrt(s));
+        return (iters > 0 && k == iters && L > 0.0 && L < 1e300) ? 0.95 / L : 0.99;
+    }
     double tau(std::size_t j) const { return (eta / omega) / cnorm[j]; }
     double sigma(std::size_t i) const { return (eta * omega) / rnorm[i]; }
 };
@@ -243,6 +269,8 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
 
     const bool halpern = opt.restart_scheme == RestartScheme::halpern;
     const bool kkt = opt.restart_scheme == RestartScheme::kkt || halpern;
+    const double eta_cap =
+        (halpern && opt.adaptive_step_size && opt.power_iterations > 0) ? p.step_cap(opt.power_iterations) : 0.99;
     std::vector<double> yprev(halpern ? m : 0);
     std::vector<double> xlast = p.x, ylast = p.y, ax2(m), aty2(n), xu2, yu2;
     std::size_t iter = 0, k = 0, since = 0;
@@ -289,8 +317,8 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                     const double target = 0.95 / L;
                     if (target < p.eta)
                         p.eta = std::max(0.1, std::max(target, p.eta * 0.8));
-                    else if (target > 1.05 * p.eta && p.eta < 0.99)
-                        p.eta = std::min(0.99, p.eta * 1.05);
+                    else if (target > 1.05 * p.eta && p.eta < eta_cap)
+                        p.eta = std::min(eta_cap, p.eta * 1.05);
                 }
             }
         }
@@ -369,7 +397,13 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                 bool upd = false;
                 if (dX > 1e-10 && dY > 1e-10) {
                     const double th = opt.primal_weight_smoothing, ratio = dY / dX;
-                    w = th == 0.5 ? std::sqrt(p.omega * ratio) : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
+                    if (halpern) {  // ratio^(127/128) omega^(1/128) in square roots
+                        double f = p.omega / ratio;
+                        for (int q = 0; q < 7; ++q) f = std::sqrt(f);
+                        w = ratio * f;
+                    } else {
+                        w = th == 0.5 ? std::sqrt(p.omega * ratio) : std::pow(ratio, th) * std::pow(p.omega, 1.0 - th);
+                    }
                     upd = true;
                 }
                 if (halpern) {  // omega_move^(3/4) omega_res^(1/4), in square roots

@@ -134,6 +134,7 @@ class EngSolve(C.Structure):
         ("restart_scheme", C.c_int32), ("fp32_stall_checks", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
         ("detect_infeasibility", C.c_int32), ("pad2_", C.c_int32), ("infeasibility_tolerance", C.c_double),
+        ("power_iterations", C.c_uint64),
     ]
 
 
@@ -670,6 +671,7 @@ class Engine:
         o.adaptive_primal_weight, o.precision = 1, precision
         o.initial_primal_weight, o.primal_weight_smoothing = 0.0, 0.5
         o.polish_threshold, o.reduction_order, o.graph_chunk = 0.0, order, graph_chunk
+        o.power_iterations = d.power_iterations
         for k, v in kw.items():
             setattr(o, k, v)
         st = EngStats()

@@ -223,6 +223,74 @@ typedef struct or_trace {
     int32_t* restart;/* [capacity] 1 if the check restarted       */
 } or_trace;
 
+/* Step-size cap of the Halpern scheme (extension; no counterpart in the
+ * reference).  The reference's steps are tau_j = (eta/omega)/colnorm_j and
+ * sigma_i = (eta omega)/rownorm_i (pdlp.cpp:186-191) with eta capped at 0.99
+ * (:214-217): that assumes ||M||_2 <= 1 for M = D_r^-1/2 A D_c^-1/2 (D the L1
+ * norms), which always holds, while PDHG needs only eta ||M|| < 1.  K power
+ * iterations on M^T M from v_j = 1 + (j mod 7)/1000 estimate L <= ||M||
+ * (within 4 % after 20 iterations on the benchmark LPs), and the cap becomes
+ * 0.95 / L.  The reference's power_iterations option (pdlp.hpp:25, unread by
+ * the reference) gives K.  Every sum is serial in index order. */
+double or_step_cap(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, const double* val,
+                          const double* row_norms, const double* col_norms, int64_t iters) {
+    double* v = (double*)malloc(sizeof(double) * (size_t)n);
+    double* t = (double*)malloc(sizeof(double) * (size_t)n);
+    double* u = (double*)malloc(sizeof(double) * (size_t)m);
+    double s = 0.0, cap = 0.99;
+    for (int64_t j = 0; j < n; ++j) {
+        v[j] = 1.0 + (double)(j % 7) / 1000.0;
+        s += v[j] * v[j];
+    }
+    int64_t k = 0;
+    for (; k < iters && s > 0.0; ++k) {
+        const double nv = sqrt(s);
+        for (int64_t j = 0; j < n; ++j) t[j] = (v[j] / nv) / sqrt(col_norms[j]);
+        or_spmv(m, n, cs, ri, val, t, u);
+        for (int64_t i = 0; i < m; ++i) u[i] = u[i] / row_norms[i];
+        or_spmv_t(m, n, cs, ri, val, u, v);
+        s = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            v[j] = v[j] / sqrt(col_norms[j]);
+            s += v[j] * v[j];
+        }
+    }
+    const double L = sqrt(sqrt(s));
+    if (iters > 0 && k == iters && L > 0.0 && L < 1e300) cap = 0.95 / L;
+    free(v);
+    free(t);
+    free(u);
+    return cap;
+}
+
+/* Primal weight of the Halpern scheme at a restart (extension): from the
+ * movement since the last restart (dX = |x - x_last|, dY = |y - y_last|) the
+ * movement rule omega_move = ratio^(127/128) omega^(1/128), ratio = dY / dX
+ * (omega follows the primal/dual distance balance almost fully, the
+ * proportional gain 0.99 of cuPDLPx, Lu & Yang 2025); from the residuals
+ * omega_res = omega sqrt(clamp(sqrt(p/d), 0.05, 20)) (the reference's rule,
+ * pdlp.cpp:247-253); the result omega_move^(3/4) omega_res^(1/4), but never a
+ * move against a residual imbalance of 100x or more, clamped to [1e-6, 1e6].
+ * Every power is a chain of correctly rounded square roots, so a device
+ * evaluation in the same order is bit-identical. */
+double or_halpern_weight(double omega, double dX, double dY, double primal, double dual) {
+    double wmove = omega;
+    if (dX > 1e-10 && dY > 1e-10) {
+        const double ratio = dY / dX;
+        double f = omega / ratio;
+        for (int q = 0; q < 7; ++q) f = sqrt(f);
+        wmove = ratio * f;
+    }
+    double wres = omega;
+    if (primal > 1e-12 && dual > 1e-12) wres = omega * sqrt(or_clamp(sqrt(primal / dual), 0.05, 20.0));
+    double w = sqrt(wmove * sqrt(wmove * wres));
+    if (primal > 1e-12 && dual > 1e-12) {
+        if (primal > 100.0 * dual && w < omega) w = omega;
+        if (dual > 100.0 * primal && w > omega) w = omega;
+    }
+    return or_clamp(w, 1e-6, 1e6);
+}
+
 /* solve_pdlp, reference pdlp.cpp:139-276.  With o->restart_scheme ==
  * MC_SCHEME_KKT the restart decision follows the extension documented in
  * include/mcpdlp.h (Applegate et al. 2021 adaptive restarts); with
@@ -306,6 +374,10 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                        ? o->initial_primal_weight
                        : or_clamp(sqrt(c_norm_inf / b_norm_inf), 0.01, 100.0);
     double eta = or_clamp(o->step_size_reduction, 0.1, 0.99);
+    const double eta_cap = (o->restart_scheme == MC_SCHEME_HALPERN && o->adaptive_step_size && o->power_iterations > 0)
+                               ? or_step_cap(m, n, lp->column_start, lp->row_index, val, row_norms, col_norms,
+                                             (int64_t)o->power_iterations)
+                               : 0.99;
     double* tau = (double*)malloc(sizeof(double) * (size_t)n);
     double* sigma = (double*)malloc(sizeof(double) * (size_t)m);
 #define OR_UPDATE_STEPS()                                                         \
@@ -391,8 +463,8 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                     if (target < eta) {
                         eta = or_max(0.1, or_max(target, eta * 0.8));
                         OR_UPDATE_STEPS();
-                    } else if (target > 1.05 * eta && eta < 0.99) {
-                        eta = or_min(0.99, eta * 1.05);
+                    } else if (target > 1.05 * eta && eta < eta_cap) {
+                        eta = or_min(eta_cap, eta * 1.05);
                         OR_UPDATE_STEPS();
                     }
                 }
@@ -466,27 +538,14 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                         for (int64_t j = 0; j < n; ++j) dxs += (xc[j] - x_last[j]) * (xc[j] - x_last[j]);
                         for (int64_t i = 0; i < m; ++i) dys += (yc[i] - y_last[i]) * (yc[i] - y_last[i]);
                         const double dX = sqrt(dxs), dY = sqrt(dys);
-                        double wmove = omega;
-                        int upd = 0;
-                        if (dX > 1e-10 && dY > 1e-10) {
+                        if (halpern) {
+                            omega = or_halpern_weight(omega, dX, dY, cur.primal, cur.dual);
+                            OR_UPDATE_STEPS();
+                        } else if (dX > 1e-10 && dY > 1e-10) {
                             const double th = o->primal_weight_smoothing;
                             const double ratio = dY / dX;
-                            wmove = (th == 0.5) ? sqrt(omega * ratio) : pow(ratio, th) * pow(omega, 1.0 - th);
-                            upd = 1;
-                        }
-                        if (halpern) { /* omega_move^(3/4) omega_res^(1/4), in square roots */
-                            double wres = omega;
-                            if (cur.primal > 1e-12 && cur.dual > 1e-12)
-                                wres = omega * sqrt(or_clamp(sqrt(cur.primal / cur.dual), 0.05, 20.0));
-                            wmove = sqrt(wmove * sqrt(wmove * wres));
-                            upd = 1;
-                            /* no move against a residual imbalance of 100x or more */
-                            if (cur.primal > 1e-12 && cur.dual > 1e-12) {
-                                if (cur.primal > 100.0 * cur.dual && wmove < omega) wmove = omega;
-                                if (cur.dual > 100.0 * cur.primal && wmove > omega) wmove = omega;
-                            }
-                        }
-                        if (upd) {
+                            const double wmove =
+                                (th == 0.5) ? sqrt(omega * ratio) : pow(ratio, th) * pow(omega, 1.0 - th);
                             omega = or_clamp(wmove, 1e-6, 1e6);
                             OR_UPDATE_STEPS();
                         }

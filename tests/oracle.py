@@ -52,6 +52,10 @@ def oracle_lib():
         lib.or_verify_primal.argtypes = [C.POINTER(mp.LP), P_f64, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_double, C.POINTER(mp.VerifyReport)]
         lib.or_kkt.argtypes = [C.POINTER(mp.LP), P_f64, P_f64, P_f64]
+        lib.or_step_cap.argtypes = [C.c_int64, C.c_int64, P_i64, P_i64, P_f64, P_f64, P_f64, C.c_int64]
+        lib.or_step_cap.restype = C.c_double
+        lib.or_halpern_weight.argtypes = [C.c_double] * 5
+        lib.or_halpern_weight.restype = C.c_double
         _or = lib
     return _or
 
@@ -176,3 +180,22 @@ def kkt(lp: mp.LPData, x, y) -> dict:
     s = lp.struct()
     oracle_lib().or_kkt(C.byref(s), _p(x, C.c_double), _p(y, C.c_double), _p(out, C.c_double))
     return {"primal": out[0], "dual": out[1], "gap": out[2], "score": out[3], "objective": out[4]}
+
+
+def step_cap(lp: mp.LPData, iters: int) -> float:
+    """Halpern-scheme step cap 0.95 / L of the oracle, L the power-method estimate
+    of ||D_r^-1/2 A D_c^-1/2|| (D = L1 row / column norms of lp's matrix, floored
+    at 1 below 1e-12) after `iters` iterations."""
+    a = np.abs(lp.value)
+    cn = np.array([a[lp.column_start[j]:lp.column_start[j + 1]].sum() for j in range(lp.cols)])
+    rn = np.zeros(lp.rows)
+    np.add.at(rn, lp.row_index, a)
+    cn[cn < 1e-12] = 1.0
+    rn[rn < 1e-12] = 1.0
+    return oracle_lib().or_step_cap(lp.rows, lp.cols, _p(lp.column_start, C.c_int64), _p(lp.row_index, C.c_int64),
+                                    _p(lp.value, C.c_double), _p(rn, C.c_double), _p(cn, C.c_double), iters)
+
+
+def halpern_weight(omega, dx, dy, primal, dual) -> float:
+    """The Halpern scheme's primal weight after a restart (oracle)."""
+    return oracle_lib().or_halpern_weight(omega, dx, dy, primal, dual)

@@ -271,3 +271,87 @@ def test_halpern_iterates_by_hand():
         assert r["dual"][0] == pytest.approx(y, rel=1e-14)
     r = orc.solve(lp, restart_scheme=mp.SCHEME_KKT, max_iterations=3, **kw)
     assert r["primal"][0] != pytest.approx(1.08)
+
+
+def _csc_lp(A):
+    A = np.asarray(A, dtype=float)
+    m, n = A.shape
+    inf = np.inf
+    return mp.LPData.from_dense(A, np.zeros(n), [-inf] * m, [inf] * m, [-inf] * n, [inf] * n)
+
+
+def test_step_cap_closed_forms():
+    """or_step_cap: L = ||D_r^-1/2 A D_c^-1/2|| by the power method, cap 0.95 / L.
+    A diagonal matrix of any signs and sizes maps to a signed identity (L = 1);
+    [[1, 1], [1, -1]] maps to A / 2, of norm 1/sqrt(2); a positive rank-one
+    matrix a b^T maps to sqrt(a_i b_j) / sqrt(sum a sum b), of norm 1.  No
+    iterations (or an all-zero matrix) leaves the reference's cap 0.99."""
+    assert orc.step_cap(_csc_lp(np.diag([3.0, -0.25, 7.0, -40.0])), 20) == pytest.approx(0.95, rel=1e-15)
+    assert orc.step_cap(_csc_lp([[1.0, 1.0], [1.0, -1.0]]), 5) == pytest.approx(0.95 * np.sqrt(2.0), rel=1e-15)
+    a, b = np.array([1.0, 2.0, 5.0]), np.array([0.5, 3.0, 1.0, 4.0])
+    assert orc.step_cap(_csc_lp(np.outer(a, b)), 20) == pytest.approx(0.95, rel=1e-12)
+    assert orc.step_cap(_csc_lp([[1.0, 1.0], [1.0, -1.0]]), 0) == 0.99
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_step_cap_matches_the_singular_value(seed):
+    """On seeded LPs the estimate is the largest singular value of the
+    preconditioned matrix (dense SVD) from below: within 1e-6 after 400
+    iterations, within 10 % after 20 (the count the solve uses by default)."""
+    lp = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=60, cols=150, nnz_max=30, seed=seed)
+    A = lp.dense()
+    r = np.abs(A).sum(1)
+    c = np.abs(A).sum(0)
+    r[r < 1e-12] = 1.0
+    c[c < 1e-12] = 1.0
+    sigma = np.linalg.svd(A / np.sqrt(r)[:, None] / np.sqrt(c)[None, :], compute_uv=False)[0]
+    L400 = 0.95 / orc.step_cap(lp, 400)
+    L20 = 0.95 / orc.step_cap(lp, 20)
+    assert abs(L400 - sigma) <= 1e-6 * sigma
+    assert 0.9 * sigma <= L20 <= sigma * (1 + 1e-12)
+
+
+def test_halpern_step_size_respects_the_cap():
+    """Under the Halpern scheme eta grows up to 0.95 / L instead of 0.99 (the
+    other schemes keep the reference's 0.99), and never above it."""
+    lp = mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11)
+    q = orc.equilibrate(lp, 10)["lp"]
+    cap = orc.step_cap(q, 20)
+    assert cap > 0.99
+    h = orc.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, trace_cap=4096)
+    k = orc.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, trace_cap=4096)
+    assert h["status"] == k["status"] == "optimal"
+    assert 0.99 < h["trace"]["eta"].max() <= cap
+    assert k["trace"]["eta"].max() <= 0.99
+
+
+def _weight_closed_form(omega, dx, dy, p, d):
+    L = np.longdouble
+    om, ratio = L(omega), L(dy) / L(dx)
+    move = ratio ** (L(127) / L(128)) * om ** (L(1) / L(128)) if dx > 1e-10 and dy > 1e-10 else om
+    res = om * np.sqrt(np.clip(np.sqrt(L(p) / L(d)), L(0.05), L(20))) if p > 1e-12 and d > 1e-12 else om
+    return move ** L(0.75) * res ** L(0.25)
+
+
+@pytest.mark.parametrize("omega,dx,dy,p,d", [
+    (0.3, 2.0, 0.05, 1e-4, 3e-5), (5.0, 1e-3, 7.0, 2e-6, 8e-6), (1.0, 1.0, 1.0, 1e-5, 1e-5),
+    (0.01, 0.4, 0.4, 3e-3, 1e-4), (2.5, 1e-12, 3.0, 1e-6, 2e-6), (0.7, 3.0, 0.2, 0.0, 1e-6)])
+def test_halpern_weight_closed_form(omega, dx, dy, p, d):
+    """omega_move^(3/4) omega_res^(1/4) with omega_move = (dY/dX)^(127/128)
+    omega^(1/128) (omega when a movement is below 1e-10) and omega_res =
+    omega sqrt(clamp(sqrt(p/d), 0.05, 20)) (omega when a residual is below
+    1e-12), evaluated in long double from the powers themselves."""
+    want = float(_weight_closed_form(omega, dx, dy, p, d))
+    assert orc.halpern_weight(omega, dx, dy, p, d) == pytest.approx(want, rel=1e-14)
+
+
+def test_halpern_weight_guard_and_clamp():
+    """No move against a residual imbalance of 100x or more; [1e-6, 1e6]."""
+    # primal 1000x the dual and a movement that would lower omega: omega stays
+    assert orc.halpern_weight(0.5, 1.0, 1e-3, 1e-3, 1e-6) == 0.5
+    # dual 1000x the primal and a movement that would raise omega: omega stays
+    assert orc.halpern_weight(0.5, 1e-3, 1.0, 1e-9, 1e-6) == 0.5
+    # the same imbalance with the movement in its favour moves omega
+    assert orc.halpern_weight(0.5, 1e-3, 1.0, 1e-3, 1e-6) > 0.5
+    assert orc.halpern_weight(1e-6, 1.0, 1e-9, 1e-6, 1e-6) == 1e-6
+    assert orc.halpern_weight(1e6, 1e-9, 1.0, 1e-6, 1e-6) == 1e6
