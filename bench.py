@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--warm-cols", type=int, default=5000, help="warm: columns of the family's structure")
     ap.add_argument("--no-roofline", action="store_true",
                     help="skip the per-kernel timing after the solve (profiling runs of the solve itself)")
+    ap.add_argument("--profile-mlp", action="store_true",
+                    help="only run the warm-start network's MFMA kernels (forward + Adam training on synthetic "
+                         "data) and print their GEMM flop count, for rocprofv3 runs")
     ap.add_argument("--profile-kernels", default=None, metavar="KINDS",
                     help="only time the listed fused kernels (comma-separated Engine.KERNELS names) on the "
                          "workload's layout and exit -- the short program rocprofv3 runs wrap")
@@ -122,6 +125,8 @@ def main():
 
     if mp.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.profile_mlp:
+        return profile_mlp(args)
     if args.profile_kernels:
         return profile_kernels(args)
     if args.workload == "batch":
@@ -372,6 +377,40 @@ def profile_kernels(args):
     print(json.dumps({"workload": wl["name"], "instances": len(lps), "m": sum(q.rows for q in lps),
                       "n": sum(q.cols for q in lps), "nnz": sum(q.nnz for q in lps), "kernels": kern,
                       "tiles": {"rows": list(tiling[0]), "cols": list(tiling[1])}}), flush=True)
+
+
+def profile_mlp(args):
+    """The 256-256-256-256-64 BF16 network on MFMA: forward of 8192 samples and
+    20 epochs of Adam over 2048 samples in batches of 256 (synthetic data).
+    Prints the k_gemm_bf16 launches and flops of that work (each GEMM is
+    2 M N K flops; forward 4 per batch, backward 4 weight and 3 input
+    gradients per batch)."""
+    import numpy as np
+    import mcpdlp as mp
+    rng = np.random.default_rng(3)
+    dims = [256, 256, 256, 256, 64]
+    net = mp.MLP(in_dim=256, hidden=256, layers=3, out_dim=64, capacity=256, seed=1)
+    x = rng.standard_normal((2048, 256)).astype(np.float32)
+    t = rng.standard_normal((2048, 64)).astype(np.float32)
+    xf = rng.standard_normal((8192, 256)).astype(np.float32)
+    t0 = time.perf_counter()
+    net.forward(xf)
+    fwd_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss, ms = net.train(x, t, batch=256, epochs=20, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    net.close()
+    layer = [a * b for a, b in zip(dims[:-1], dims[1:])]
+    fwd_flops_per_row = 2 * sum(layer)
+    bwd_flops_per_row = 2 * sum(layer) + 2 * sum(layer[1:])  # weight gradients + input gradients (not layer 0)
+    batches = 20 * (2048 // 256)
+    gemm_fwd = 4 * (8192 // 256)  # forward in chunks of the capacity
+    gemm_train = batches * (4 + 4 + 3)
+    flops = 8192 * fwd_flops_per_row + batches * 256 * (fwd_flops_per_row + bwd_flops_per_row)
+    print(json.dumps({"k_gemm_bf16_launches": gemm_fwd + gemm_train, "gemm_flops": flops,
+                      "forward_s": round(fwd_s, 4), "train_s": round(train_s, 4), "final_loss": loss,
+                      "note": "achieved MFMA rate = gemm_flops / total k_gemm_bf16 time of the kernel trace"}),
+          flush=True)
 
 
 def _line(value_ms, workload, cfg, extra, steps=1, warmup=0):

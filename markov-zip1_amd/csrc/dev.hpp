@@ -103,6 +103,9 @@ struct Dev {
     int32_t budA, budT;        // nonzeros per block tile: 512, 1024 or 2048
     int32_t halpern;    // reflected Halpern iterates: the average slots hold T(z), the
                         // last-restart slots the anchor z0
+    int32_t nt_rows;    // 1: the FP64 streams of A (dual step, A dx, the check's rows) use
+                        // non-temporal loads, so that A^T stays in the 256 MB last-level
+                        // cache when the pair of FP64 matrices does not fit but one does
     int32_t blocks;     // 1: the fast-order primal/dual steps, A dx and the check's
                         // residual terms run on the block tiles (part_dxn, part_adx,
                         // part_rows, part_cols then hold slots per tile of bT / bA);

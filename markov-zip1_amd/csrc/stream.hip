@@ -36,9 +36,26 @@ __device__ __forceinline__ int block_tile_id(int ntiles) {
     return (b % kXcd) * per + b / kXcd;
 }
 
+// One index/value pair of the matrix stream; nt: non-temporal loads, which
+// leave the last-level cache (the 256 MB Infinity Cache) to the other
+// matrix (see Dev::nt_rows).
+// NT is a template parameter: the compiler merges a run-time choice between
+// the two load forms into one plain load.
+template <bool NT, typename TA>
+__device__ __forceinline__ void stream_pair(const int32_t* __restrict__ idx, const TA* __restrict__ val, int p,
+                                            int& i, TA& a) {
+    if (NT) {
+        i = __builtin_nontemporal_load(idx + p);
+        a = __builtin_nontemporal_load(val + p);
+    } else {
+        i = idx[p];
+        a = val[p];
+    }
+}
+
 // Products val[p] * vec[idx[p]] of the nonzeros [p0, p0 + len) into prod[0, len),
 // len <= K * kBT.  All K index/value loads are issued, then all K gathers.
-template <int K, typename TS, typename TA, typename TV>
+template <int K, typename TS, typename TA, typename TV, bool NT = false>
 __device__ __forceinline__ void stage_products(int p0, int len, const int32_t* __restrict__ idx,
                                                const TA* __restrict__ val, const TV* __restrict__ vec,
                                                TS* prod) {
@@ -47,9 +64,9 @@ __device__ __forceinline__ void stage_products(int p0, int len, const int32_t* _
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int q = (int)threadIdx.x + k * kBT;
-        const bool in = q < len;
-        ii[k] = in ? idx[p0 + q] : 0;
-        aa[k] = in ? val[p0 + q] : (TA)0;
+        ii[k] = 0;
+        aa[k] = (TA)0;
+        if (q < len) stream_pair<NT>(idx, val, p0 + q, ii[k], aa[k]);
     }
     TV vv[K];
 #pragma unroll
@@ -78,27 +95,38 @@ __device__ __forceinline__ TS group_sum(const TS* prod, int q0, int q1, int gl, 
 
 // Dot product of one long row [p0, p1) by the whole workgroup (four
 // accumulators per thread, fixed-order block reduction); valid in thread 0.
-template <typename TS, typename TA, typename TV>
+template <typename TS, bool NT = false, typename TA, typename TV>
 __device__ __forceinline__ double long_row_dot(int p0, int p1, const int32_t* __restrict__ idx,
                                                const TA* __restrict__ val, const TV* __restrict__ vec,
                                                double* red) {
     TS s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int p = p0 + (int)threadIdx.x;
     for (; p + 3 * kBT < p1; p += 4 * kBT) {
-        const int i0 = idx[p], i1 = idx[p + kBT], i2 = idx[p + 2 * kBT], i3 = idx[p + 3 * kBT];
-        const TA a0 = val[p], a1 = val[p + kBT], a2 = val[p + 2 * kBT], a3 = val[p + 3 * kBT];
+        int i0, i1, i2, i3;
+        TA a0, a1, a2, a3;
+        stream_pair<NT>(idx, val, p, i0, a0);
+        stream_pair<NT>(idx, val, p + kBT, i1, a1);
+        stream_pair<NT>(idx, val, p + 2 * kBT, i2, a2);
+        stream_pair<NT>(idx, val, p + 3 * kBT, i3, a3);
         s0 += (TS)a0 * (TS)vec[i0];
         s1 += (TS)a1 * (TS)vec[i1];
         s2 += (TS)a2 * (TS)vec[i2];
         s3 += (TS)a3 * (TS)vec[i3];
     }
-    for (; p < p1; p += kBT) s0 += (TS)val[p] * (TS)vec[idx[p]];
+    for (; p < p1; p += kBT) {
+        int i;
+        TA a;
+        stream_pair<NT>(idx, val, p, i, a);
+        s0 += (TS)a * (TS)vec[i];
+    }
     return block_sum<kBT / 64>((double)((s0 + s1) + (s2 + s3)), red);
 }
 
 // Row sum of a block tile in the instance's precision: FP32 values and
 // gathers with FP32 products and sums, or FP64.  Every thread must call.
-template <int NB, typename TA32, typename TV32>
+// NT64: the FP64 matrix stream uses non-temporal loads (FP32 streams never do:
+// the pair of FP32 matrices is small enough to share the cache).
+template <int NB, bool NT64 = false, typename TA32, typename TV32>
 __device__ __forceinline__ double tile_row_sum(bool f32, int p0, int len, int q0, int q1, int gl, int G,
                                                const int32_t* __restrict__ idx, const double* __restrict__ val64,
                                                const double* __restrict__ vec64, const TA32* __restrict__ val32,
@@ -109,7 +137,7 @@ __device__ __forceinline__ double tile_row_sum(bool f32, int p0, int len, int q0
         __syncthreads();
         return (double)group_sum<float>(pr, q0, q1, gl, G);
     }
-    stage_products<NB / kBT, double>(p0, len, idx, val64, vec64, prod);
+    stage_products<NB / kBT, double, double, double, NT64>(p0, len, idx, val64, vec64, prod);
     __syncthreads();
     return group_sum<double>(prod, q0, q1, gl, G);
 }
@@ -209,7 +237,7 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
 // ---------------------------------------------------------------------------
 // Dual step over block tiles of A: A x_bar, Moreau prox, running average (or
 // Halpern update); FP32 mode also refreshes the FP32 copy of y.
-template <int NB>
+template <int NB, bool NT>
 __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
     __shared__ double prod[NB];
     __shared__ double red[kBT / 64];
@@ -260,7 +288,7 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
     if (tl.shape & 8) {  // one long row (r1 = r0 + 1)
         const int i = tl.r0;
         const double ax = f32 ? long_row_dot<float>(p0, p1, d.ci, d.av32, d.xbar32, red)
-                              : long_row_dot<double>(p0, p1, d.ci, d.av64, d.xbar, red);
+                              : long_row_dot<double, NT>(p0, p1, d.ci, d.av64, d.xbar, red);
         if (tid == 0 && run) {
             load(i);
             update(i, ax);
@@ -277,14 +305,14 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
         q1 = d.rp[i + 1] - p0;
         if (gl == 0) load(i);
     }
-    const double ax = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.xbar, d.av32, d.xbar32, prod);
+    const double ax = tile_row_sum<NB, NT>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.xbar, d.av32, d.xbar32, prod);
     if (own && gl == 0) update(i, ax);
 }
 
 // ---------------------------------------------------------------------------
 // A dx for the adaptive step size (pdlp.cpp:208-210) over block tiles of A:
 // the tile's partial of |A dx|^2_{1/rownorm}.
-template <int NB>
+template <int NB, bool NT>
 __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
     __shared__ double prod[NB];
     __shared__ double red[kBT / 64];
@@ -304,7 +332,7 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
     if (tl.shape & 8) {  // one long row (r1 = r0 + 1)
         const int i = tl.r0;
         const double a = f32 ? long_row_dot<float>(p0, p1, d.ci, d.av32, d.dx32, red)
-                             : long_row_dot<double>(p0, p1, d.ci, d.av64, d.dx, red);
+                             : long_row_dot<double, NT>(p0, p1, d.ci, d.av64, d.dx, red);
         if (tid == 0 && run) d.part_adx[tile] = (a * a) / d.rnorm[i];
         return;
     }
@@ -319,7 +347,7 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
         q1 = d.rp[i + 1] - p0;
         if (gl == 0) rn = d.rnorm[i];
     }
-    const double a = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.dx, d.av32, d.dx32, prod);
+    const double a = tile_row_sum<NB, NT>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.dx, d.av32, d.dx32, prod);
     const double part = block_sum<kBT / 64>((own && gl == 0) ? (a * a) / rn : 0.0, red);
     if (tid == 0 && run) d.part_adx[tile] = part;
 }
@@ -333,7 +361,7 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
 // [(c * fields + k) * tiles + tile], with tiles = nbA / nbT.
 
 // Products of the tile's nonzeros with NC gathered vectors into prod[c * NB + q].
-template <int K, int NC>
+template <int K, int NC, bool NT = false>
 __device__ __forceinline__ void stage_products_multi(int p0, int len, const int32_t* __restrict__ idx,
                                                      const double* __restrict__ val, const double* const* vecs,
                                                      double* prod) {
@@ -343,9 +371,9 @@ __device__ __forceinline__ void stage_products_multi(int p0, int len, const int3
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int q = (int)threadIdx.x + k * kBT;
-        const bool in = q < len;
-        ii[k] = in ? idx[p0 + q] : 0;
-        aa[k] = in ? val[p0 + q] : 0.0;
+        ii[k] = 0;
+        aa[k] = 0.0;
+        if (q < len) stream_pair<NT>(idx, val, p0 + q, ii[k], aa[k]);
     }
     double vv[NC][K];
 #pragma unroll
@@ -365,7 +393,7 @@ __device__ __forceinline__ void stage_products_multi(int p0, int len, const int3
 }
 
 // Rows: A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
-template <int NB, int NC>
+template <int NB, int NC, bool NT>
 __global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
     __shared__ double prod[NC * NB];
     __shared__ double red[(kBT / 64) * 3];
@@ -410,7 +438,8 @@ __global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
         const int i = tl.r0;
         double ax[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) ax[c] = long_row_dot<double>(d.rp[i], d.rp[i + 1], d.ci, d.av64, xs[c], red);
+        for (int c = 0; c < NC; ++c)
+            ax[c] = long_row_dot<double, NT>(d.rp[i], d.rp[i + 1], d.ci, d.av64, xs[c], red);
         if (tid == 0) {
             load(i);
             for (int c = 0; c < NC; ++c) {
@@ -431,7 +460,7 @@ __global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
         q1 = d.rp[i + 1] - p0;
         if (gl == 0) load(i);
     }
-    stage_products_multi<NB / kBT, NC>(p0, len, d.ci, d.av64, xs, prod);
+    stage_products_multi<NB / kBT, NC, NT>(p0, len, d.ci, d.av64, xs, prod);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -673,45 +702,69 @@ void launch_primal_blk(const Dev& d, int t, bool adapt, hipStream_t s, hipEvent_
         primal_blk<false>(d, t, s, e0, e1);
 }
 
-void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (d.nbA == 0) return;
+template <bool NT>
+static void dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const dim3 g(block_grid(d.nbA)), b(kBT);
     if (d.budA == 256)
-        MCP_LAUNCH_TIMED((k_dual_blk<256>), g, b, s, e0, e1, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<256, NT>), g, b, s, e0, e1, d, t);
     else if (d.budA == 512)
-        MCP_LAUNCH_TIMED((k_dual_blk<512>), g, b, s, e0, e1, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<512, NT>), g, b, s, e0, e1, d, t);
     else if (d.budA == 2048)
-        MCP_LAUNCH_TIMED((k_dual_blk<2048>), g, b, s, e0, e1, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<2048, NT>), g, b, s, e0, e1, d, t);
     else
-        MCP_LAUNCH_TIMED((k_dual_blk<1024>), g, b, s, e0, e1, d, t);
+        MCP_LAUNCH_TIMED((k_dual_blk<1024, NT>), g, b, s, e0, e1, d, t);
+}
+
+void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (d.nbA == 0) return;
+    if (d.nt_rows)
+        dual_blk<true>(d, t, s, e0, e1);
+    else
+        dual_blk<false>(d, t, s, e0, e1);
+}
+
+template <bool NT>
+static void adx_blk(const Dev& d, int t, hipStream_t s) {
+    const dim3 g(block_grid(d.nbA)), b(kBT);
+    if (d.budA == 256)
+        hipLaunchKernelGGL((k_adx_blk<256, NT>), g, b, 0, s, d, t);
+    else if (d.budA == 512)
+        hipLaunchKernelGGL((k_adx_blk<512, NT>), g, b, 0, s, d, t);
+    else if (d.budA == 2048)
+        hipLaunchKernelGGL((k_adx_blk<2048, NT>), g, b, 0, s, d, t);
+    else
+        hipLaunchKernelGGL((k_adx_blk<1024, NT>), g, b, 0, s, d, t);
 }
 
 void launch_adx_blk(const Dev& d, int t, hipStream_t s) {
     if (d.nbA == 0) return;
+    if (d.nt_rows)
+        adx_blk<true>(d, t, s);
+    else
+        adx_blk<false>(d, t, s);
+}
+
+template <int NC, bool NT>
+static void resid_rows_blk(const Dev& d, hipStream_t s) {
     const dim3 g(block_grid(d.nbA)), b(kBT);
     if (d.budA == 256)
-        hipLaunchKernelGGL((k_adx_blk<256>), g, b, 0, s, d, t);
+        hipLaunchKernelGGL((k_resid_rows_blk<256, NC, NT>), g, b, 0, s, d);
     else if (d.budA == 512)
-        hipLaunchKernelGGL((k_adx_blk<512>), g, b, 0, s, d, t);
+        hipLaunchKernelGGL((k_resid_rows_blk<512, NC, NT>), g, b, 0, s, d);
     else if (d.budA == 2048)
-        hipLaunchKernelGGL((k_adx_blk<2048>), g, b, 0, s, d, t);
+        hipLaunchKernelGGL((k_resid_rows_blk<2048, NC, NT>), g, b, 0, s, d);
     else
-        hipLaunchKernelGGL((k_adx_blk<1024>), g, b, 0, s, d, t);
+        hipLaunchKernelGGL((k_resid_rows_blk<1024, NC, NT>), g, b, 0, s, d);
 }
 
 template <int NC>
 static void resid_blk(const Dev& d, hipStream_t s) {
     const dim3 b(kBT);
     if (d.nbA > 0) {
-        const dim3 g(block_grid(d.nbA));
-        if (d.budA == 256)
-            hipLaunchKernelGGL((k_resid_rows_blk<256, NC>), g, b, 0, s, d);
-        else if (d.budA == 512)
-            hipLaunchKernelGGL((k_resid_rows_blk<512, NC>), g, b, 0, s, d);
-        else if (d.budA == 2048)
-            hipLaunchKernelGGL((k_resid_rows_blk<2048, NC>), g, b, 0, s, d);
+        if (d.nt_rows)
+            resid_rows_blk<NC, true>(d, s);
         else
-            hipLaunchKernelGGL((k_resid_rows_blk<1024, NC>), g, b, 0, s, d);
+            resid_rows_blk<NC, false>(d, s);
     }
     if (d.nbT > 0) {
         const dim3 g(block_grid(d.nbT));
