@@ -263,6 +263,75 @@ double or_step_cap(int64_t m, int64_t n, const int64_t* cs, const int64_t* ri, c
     return cap;
 }
 
+/* Infeasibility certificates (extension; the reference declares
+ * PdlpStatus::infeasible_or_unbounded, pdlp.hpp:43, but never returns it).
+ * (dx, dy) is a movement of the iterate in the ORIGINAL space (Applegate et
+ * al. 2021, sec. 5: a diverging PDHG iterate moves along a ray).
+ *  1: dy is a dual ray -- the LP is primal infeasible: the homogeneous dual
+ *     objective D = sum_i -support_[lo_i, hi_i](dy_i) + sum_j (g_j > 0 ?
+ *     g_j lo_j : g_j < 0 ? g_j hi_j : 0), g = A^T dy, exceeds 1e-9 times the
+ *     sum of its terms' magnitudes, and the parts that would need an
+ *     infinite bound sum to at most tol D;
+ *  2: dx is a primal ray -- the LP is unbounded: c^T dx (c with the
+ *     objective's sense) is below -1e-9 times the sum of |c_j dx_j|, and the
+ *     recession violations of A dx against the finite row bounds and of dx
+ *     against the finite variable bounds are at most tol |c^T dx|;
+ *  0: neither.  A feasible LP has D <= 0 along every dual ray and a bounded
+ *  one c^T dx >= 0 along every primal ray (weak duality). */
+int or_ray_certificate(const mc_lp* lp, const double* dx, const double* dy, double tol) {
+    const int64_t m = lp->rows, n = lp->cols;
+    const double sense = (lp->sense == MC_MAXIMIZE) ? -1.0 : 1.0;
+    double* adx = (double*)calloc((size_t)(m > 0 ? m : 1), sizeof(double));
+    double* g = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t p = lp->column_start[j]; p < lp->column_start[j + 1]; ++p) {
+            adx[lp->row_index[p]] += lp->value[p] * dx[j];
+            g[j] += lp->value[p] * dy[lp->row_index[p]];
+        }
+    double dobj = 0.0, dmag = 0.0, dinf = 0.0, ymax = 0.0;
+    double cdx = 0.0, cmag = 0.0, pviol = 0.0, xmax = 0.0;
+    for (int64_t i = 0; i < m; ++i) {
+        const int has_lo = lp->row_lower[i].kind == MC_BOUND_FINITE, has_hi = lp->row_upper[i].kind == MC_BOUND_FINITE;
+        const double lo = lp->row_lower[i].value, hi = lp->row_upper[i].value;
+        /* dual ray: -support of the row interval along dy_i */
+        double t = 0.0;
+        if (dy[i] > 0.0) {
+            if (has_hi) t = -dy[i] * hi; else dinf += dy[i];
+        } else if (dy[i] < 0.0) {
+            if (has_lo) t = -dy[i] * lo; else dinf += -dy[i];
+        }
+        dobj += t;
+        dmag += fabs(t);
+        ymax = or_max(ymax, fabs(dy[i]));
+        /* primal ray: A dx in the recession cone of the row interval */
+        if (has_lo && adx[i] < 0.0) pviol = or_max(pviol, -adx[i]);
+        if (has_hi && adx[i] > 0.0) pviol = or_max(pviol, adx[i]);
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        const int has_lo = lp->var_lower[j].kind == MC_BOUND_FINITE, has_hi = lp->var_upper[j].kind == MC_BOUND_FINITE;
+        const double lo = lp->var_lower[j].value, hi = lp->var_upper[j].value;
+        double t = 0.0;
+        if (g[j] > 0.0) {
+            if (has_lo) t = g[j] * lo; else dinf += g[j];
+        } else if (g[j] < 0.0) {
+            if (has_hi) t = g[j] * hi; else dinf += -g[j];
+        }
+        dobj += t;
+        dmag += fabs(t);
+        if (has_lo && dx[j] < 0.0) pviol = or_max(pviol, -dx[j]);
+        if (has_hi && dx[j] > 0.0) pviol = or_max(pviol, dx[j]);
+        const double ct = sense * lp->objective[j] * dx[j];
+        cdx += ct;
+        cmag += fabs(ct);
+        xmax = or_max(xmax, fabs(dx[j]));
+    }
+    free(adx);
+    free(g);
+    if (ymax > 0.0 && dobj > 1e-9 * dmag && dinf <= tol * dobj) return 1;
+    if (xmax > 0.0 && cdx < -1e-9 * cmag && pviol <= tol * -cdx) return 2;
+    return 0;
+}
+
 /* Primal weight of the Halpern scheme at a restart (extension): from the
  * movement since the last restart (dX = |x - x_last|, dY = |y - y_last|) the
  * movement rule omega_move = ratio^(127/128) omega^(1/128), ratio = dY / dX
@@ -289,6 +358,23 @@ double or_halpern_weight(double omega, double dX, double dY, double primal, doub
         if (dual > 100.0 * primal && w > omega) w = omega;
     }
     return or_clamp(w, 1e-6, 1e6);
+}
+
+/* The check's certificate test: the candidate's movement since the last
+ * restart (the start point under the reference scheme), scaled back to the
+ * original space, tested by or_ray_certificate. */
+static int or_movement_certifies(const mc_lp* lp, const double* xc, const double* yc, const double* x_last,
+                                 const double* y_last, const double* col_scale, const double* row_scale, int ruiz,
+                                 double tol) {
+    const int64_t m = lp->rows, n = lp->cols;
+    double* dx = (double*)malloc(sizeof(double) * (size_t)n);
+    double* dy = (double*)malloc(sizeof(double) * (size_t)m);
+    for (int64_t j = 0; j < n; ++j) dx[j] = (xc[j] - x_last[j]) * (ruiz ? col_scale[j] : 1.0);
+    for (int64_t i = 0; i < m; ++i) dy[i] = (yc[i] - y_last[i]) * (ruiz ? row_scale[i] : 1.0);
+    const int r = or_ray_certificate(lp, dx, dy, tol > 0.0 ? tol : 1e-8);
+    free(dx);
+    free(dy);
+    return r != 0;
 }
 
 /* solve_pdlp, reference pdlp.cpp:139-276.  With o->restart_scheme ==
@@ -415,7 +501,7 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
     uint64_t iter = 0, avg_count = 0, since = 0;
     double last_restart_score = 1e300, prev_cand = 1e300;
     or_resid cur, cur2;
-    int optimal = 0, use_cur = 0;
+    int optimal = 0, use_cur = 0, infeasible = 0;
     /* Residuals of the average (and, for the KKT scheme, of the current
      * iterate); `cur`/xo/yo end up holding the chosen candidate. */
 #define OR_EVALUATE()                                                                             \
@@ -499,6 +585,11 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
             if (cur.primal <= o->primal_tolerance && cur.dual <= o->dual_tolerance &&
                 cur.gap <= o->gap_tolerance) {
                 optimal = 1;
+            } else if (o->detect_infeasibility && or_movement_certifies(lp, use_cur ? x : x_avg, use_cur ? y : y_avg,
+                                                                        x_last, y_last, col_scale, row_scale,
+                                                                        o->ruiz_scaling, o->infeasibility_tolerance)) {
+                infeasible = 1;
+                break;
             } else if (!kkt) {
                 const double score = cur.score;
                 const int do_restart =
@@ -574,12 +665,13 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
             if (optimal) break;
         }
     }
-    if (!optimal) { /* :263-265 */
+    if (!optimal && !infeasible) { /* :263-265 */
         OR_EVALUATE();
     }
 #undef OR_EVALUATE
-    res->status = optimal ? MC_PDLP_OPTIMAL : MC_PDLP_ITERATION_LIMIT;
-    strcpy(res->message, optimal ? "PDLP converged" : "iteration limit reached");
+    res->status = optimal ? MC_PDLP_OPTIMAL : infeasible ? MC_PDLP_INFEASIBLE_OR_UNBOUNDED : MC_PDLP_ITERATION_LIMIT;
+    strcpy(res->message, optimal ? "PDLP converged"
+                         : infeasible ? "infeasibility certificate found" : "iteration limit reached");
     res->objective = cur.objective;
     res->primal_infeasibility = cur.primal;
     res->dual_infeasibility = cur.dual;

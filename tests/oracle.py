@@ -54,6 +54,7 @@ def oracle_lib():
         lib.or_kkt.argtypes = [C.POINTER(mp.LP), P_f64, P_f64, P_f64]
         lib.or_step_cap.argtypes = [C.c_int64, C.c_int64, P_i64, P_i64, P_f64, P_f64, P_f64, C.c_int64]
         lib.or_step_cap.restype = C.c_double
+        lib.or_ray_certificate.argtypes = [C.POINTER(mp.LP), P_f64, P_f64, C.c_double]
         lib.or_halpern_weight.argtypes = [C.c_double] * 5
         lib.or_halpern_weight.restype = C.c_double
         _or = lib
@@ -199,3 +200,12 @@ def step_cap(lp: mp.LPData, iters: int) -> float:
 def halpern_weight(omega, dx, dy, primal, dual) -> float:
     """The Halpern scheme's primal weight after a restart (oracle)."""
     return oracle_lib().or_halpern_weight(omega, dx, dy, primal, dual)
+
+
+def ray_certificate(lp: mp.LPData, dx, dy, tol=1e-8) -> int:
+    """Oracle verdict on a movement (dx, dy) of the iterate in the original
+    space: 1 dual ray (primal infeasible), 2 primal ray (unbounded), 0 none."""
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    s = lp.struct()
+    return oracle_lib().or_ray_certificate(C.byref(s), _p(dx, C.c_double), _p(dy, C.c_double), tol)

@@ -355,3 +355,62 @@ def test_halpern_weight_guard_and_clamp():
     assert orc.halpern_weight(0.5, 1e-3, 1.0, 1e-3, 1e-6) > 0.5
     assert orc.halpern_weight(1e-6, 1.0, 1e-9, 1e-6, 1e-6) == 1e-6
     assert orc.halpern_weight(1e6, 1e-9, 1.0, 1e-6, 1e-6) == 1e6
+
+
+def test_ray_certificate_hand_cases():
+    """or_ray_certificate on LPs whose certificates are worked by hand.
+    x1 + x2 = 1 and x1 + x2 = 2, x >= 0: dy = (1, -1) gives A^T dy = 0 and
+    D = -1*1 - (-1)*2 = 1 > 0 (primal infeasible); dy = (-1, 1) gives D = -1.
+    min -x1 s.t. x1 - x2 = 0, x >= 0: dx = (1, 1) keeps A dx = 0 and x >= 0
+    with c^T dx = -1 (unbounded); dx = (1, 0) violates the row by 1; dx =
+    (-1, -1) leaves the bounds.  A dual direction that would need a missing
+    bound is rejected, and the maximize sense flips the cost."""
+    inf = np.inf
+    two = mp.LPData.from_dense([[1, 1], [1, 1]], [1, 1], [1, 2], [1, 2], [0, 0], [inf, inf])
+    assert orc.ray_certificate(two, [0, 0], [1, -1]) == 1
+    assert orc.ray_certificate(two, [0, 0], [-1, 1]) == 0
+    assert orc.ray_certificate(two, [0, 0], [0, 0]) == 0
+    unb = mp.LPData.from_dense([[1, -1]], [-1, 0], [0], [0], [0, 0], [inf, inf])
+    assert orc.ray_certificate(unb, [1, 1], [0]) == 2
+    assert orc.ray_certificate(unb, [1, 0], [0]) == 0
+    assert orc.ray_certificate(unb, [-1, -1], [0]) == 0
+    # maximize sense flips the cost: max -x1 along (1, 1) is bounded
+    unb_max = mp.LPData.from_dense([[1, -1]], [-1, 0], [0], [0], [0, 0], [inf, inf], sense=mp.MAXIMIZE)
+    assert orc.ray_certificate(unb_max, [1, 1], [0]) == 0
+    # rows x1 + x2 <= 1 and x1 + x2 >= 2, x free: dy = (1, -1) has A^T dy = 0 and pays the
+    # finite bounds only, D = -1*1 - (-1)*2 = 1; dy = (-1, 1) would need the missing lower
+    # bound of row 0 and upper bound of row 1: rejected
+    rows = mp.LPData.from_dense([[1, 1], [1, 1]], [0, 0], [-inf, 2], [1, inf], [-inf, -inf], [inf, inf])
+    assert orc.ray_certificate(rows, [0, 0], [1, -1]) == 1
+    assert orc.ray_certificate(rows, [0, 0], [-1, 1]) == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_ray_certificate_never_fires_on_feasible_bounded_lps(seed):
+    """By weak duality a feasible LP has no dual ray and a bounded one no primal
+    ray: random movements on planted LPs are never certified."""
+    lp = mp.generate(mp.FAMILY_ZIPF_MIXED, rows=80, cols=200, nnz_max=20, seed=seed)
+    rng = np.random.default_rng(seed)
+    for _ in range(200):
+        dx = rng.standard_normal(lp.cols) * rng.choice([0.0, 1.0, 1e3])
+        dy = rng.standard_normal(lp.rows) * rng.choice([0.0, 1.0, 1e3])
+        assert orc.ray_certificate(lp, dx, dy) == 0
+
+
+def test_oracle_detects_infeasibility():
+    """With detect_infeasibility the oracle stops the constructed infeasible and
+    unbounded LPs with infeasible_or_unbounded; on the feasible LP the test only
+    reads the iterate, so the solve equals the one without it; the host path
+    (product CPU backend) gives the same statuses."""
+    import test_host
+    cases, base = test_host._infeasibility_cases()
+    for scheme in (mp.SCHEME_KKT, mp.SCHEME_HALPERN):
+        kw = dict(tolerance=1e-6, restart_scheme=scheme, max_iterations=100000)
+        for name, lp in cases.items():
+            r = orc.solve(lp, detect_infeasibility=1, **kw)
+            assert r["status"] == "infeasible_or_unbounded", (name, scheme, r["status"])
+            assert r["message"] == "infeasibility certificate found"
+            assert mp.solve(lp, detect_infeasibility=1, **kw)["status"] == r["status"]
+        a = orc.solve(base, detect_infeasibility=1, **kw)
+        b = orc.solve(base, **kw)
+        assert a["status"] == "optimal" and a["iterations"] == b["iterations"] and a["objective"] == b["objective"]
