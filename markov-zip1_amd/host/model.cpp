@@ -5,7 +5,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <string_view>
+#include <thread>
 #include <unordered_set>
+#include <vector>
 
 namespace markov_cero::model {
 namespace {
@@ -27,16 +30,50 @@ void audit_bound_pair(const Bound& lo, const Bound& hi, const char* what, std::s
 }
 
 // Names are optional metadata, but when present there is one per entity,
-// each non-empty and unique.
+// each non-empty and unique; the first offending index (in entity order) is
+// reported.  Long lists are checked by several threads: every name is hashed
+// once, and thread t keeps the set of the names whose hash is t modulo the
+// thread count, so equal names always meet in the same set; each thread's
+// first repeat is its smallest index, and the smallest over the threads is the
+// serial scan's answer.
 void audit_names(const std::vector<std::string>& names, std::size_t count, const char* what) {
     if (names.size() != count)
         reject(std::string(what) + " names: " + std::to_string(names.size()) + " given for " + std::to_string(count));
-    std::unordered_set<std::string> seen;
-    seen.reserve(names.size());
-    for (std::size_t k = 0; k < names.size(); ++k) {
-        if (names[k].empty()) reject(at(what, k) + " has an empty name");
-        if (!seen.insert(names[k]).second) reject(std::string(what) + " name repeated: " + names[k]);
+    const std::size_t none = names.size();
+    const std::size_t parts =
+        names.size() < ((std::size_t)1 << 20) ? 1 : std::min<std::size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::size_t> hash(names.size());
+    std::size_t first_empty = none;
+    for (std::size_t k = 0; k < names.size() && first_empty == none; ++k)
+        if (names[k].empty()) first_empty = k;
+    std::vector<std::size_t> first_repeat(parts, none);
+    auto scan = [&](std::size_t t) {
+        std::unordered_set<std::string_view> seen;  // views into `names`: no copies
+        seen.reserve(2 * names.size() / parts + 16);
+        for (std::size_t k = 0; k < names.size(); ++k)
+            if (hash[k] % parts == t && !seen.insert(names[k]).second) {
+                first_repeat[t] = k;
+                return;
+            }
+    };
+    auto hash_range = [&](std::size_t a, std::size_t b) {
+        for (std::size_t k = a; k < b; ++k) hash[k] = std::hash<std::string_view>{}(names[k]);
+    };
+    if (parts == 1) {
+        hash_range(0, names.size());
+        scan(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (std::size_t t = 0; t < parts; ++t)
+            pool.emplace_back(hash_range, names.size() * t / parts, names.size() * (t + 1) / parts);
+        for (auto& th : pool) th.join();
+        pool.clear();
+        for (std::size_t t = 0; t < parts; ++t) pool.emplace_back(scan, t);
+        for (auto& th : pool) th.join();
     }
+    const std::size_t dup = *std::min_element(first_repeat.begin(), first_repeat.end());
+    if (first_empty < dup) reject(at(what, first_empty) + " has an empty name");
+    if (dup != none) reject(std::string(what) + " name repeated: " + names[dup]);
 }
 
 }  // namespace

@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <thread>
+#include <vector>
 
 namespace markov_cero::verify {
 namespace {
@@ -51,6 +53,40 @@ struct Audit {
     }
 };
 
+// Row activities A x of a validated matrix, each row's terms accumulated in
+// long double in increasing column order -- the order of
+// SparseMatrixCSC::multiply, so the values are identical.  Large matrices are
+// split by row ranges over threads: every thread walks all columns and adds
+// only the entries of its rows (found by binary search in each sorted
+// column), so no row's order of additions changes.
+std::vector<double> row_activity(const model::SparseMatrixCSC& A, const std::vector<double>& x) {
+    const std::size_t m = A.row_count, n = A.column_count, nnz = A.value.size();
+    std::vector<double> out(m);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const std::size_t parts = nnz < (std::size_t)1 << 20 ? 1 : std::min<std::size_t>({hw, 16, m});
+    auto run = [&](std::size_t r0, std::size_t r1) {
+        std::vector<long double> acc(r1 - r0, 0.0L);
+        for (std::size_t j = 0; j < n; ++j) {
+            const auto b = A.row_index.begin() + (std::ptrdiff_t)A.column_start[j];
+            const auto e = A.row_index.begin() + (std::ptrdiff_t)A.column_start[j + 1];
+            const long double xj = x[j];
+            for (auto it = parts == 1 ? b : std::lower_bound(b, e, r0); it != e && *it < r1; ++it)
+                acc[*it - r0] += static_cast<long double>(A.value[(std::size_t)(it - A.row_index.begin())]) * xj;
+        }
+        for (std::size_t i = r0; i < r1; ++i) out[i] = static_cast<double>(acc[i - r0]);
+    };
+    if (parts <= 1) {
+        run(0, m);
+    } else {
+        std::vector<std::thread> pool;
+        for (std::size_t k = 0; k < parts; ++k) pool.emplace_back(run, m * k / parts, m * (k + 1) / parts);
+        for (auto& t : pool) t.join();
+    }
+    for (std::size_t i = 0; i < m; ++i)
+        if (!std::isfinite(out[i])) throw std::overflow_error("row activity overflows in row " + std::to_string(i));
+    return out;
+}
+
 }  // namespace
 
 // Allowances above 1e-4 are refused (reference primal_verifier.cpp:39-43).
@@ -79,7 +115,7 @@ PrimalVerificationReport verify_primal(const model::Model& md, const Candidate& 
                    rep.maximum_variable_violation);
         if (md.variable_type[j] != model::VariableType::continuous) audit.integral(cand.primal[j], j, int_tol);
     }
-    const std::vector<double> activity = md.matrix.multiply(cand.primal);
+    const std::vector<double> activity = row_activity(md.matrix, cand.primal);  // md validated above
     for (std::size_t i = 0; i < activity.size(); ++i)
         audit.pair(activity[i], md.row_lower[i], md.row_upper[i], "row_lower", "row_upper", i,
                    rep.maximum_row_violation);
