@@ -98,6 +98,9 @@ class WarmStart:
         self.c_mean, self.c_std = Cf.mean(axis=0), Cf.std(axis=0) + 1e-12
         Cs = ((Cf - self.c_mean) / self.c_std).astype(np.float32)
         explained = 1.0 - np.sum((Z - self.z_mean - Cf @ self.z_basis.T) ** 2) / np.sum((Z - self.z_mean) ** 2)
+        # mcp_mlp_train skips the tail that does not fill a mini-batch: with fewer
+        # samples than a batch, one batch of all of them
+        batch = max(1, min(batch, len(lps)))
         self.net = mp.MLP(self.n_features, self.hidden, self.layers, self.n_coeffs,
                           capacity=max(32, batch), seed=self.seed, device=self.device)
         loss, ms = self.net.train(Fs, Cs, batch=batch, epochs=epochs, lr=lr, seed=self.seed + 1)
