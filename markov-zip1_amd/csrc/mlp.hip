@@ -4,12 +4,13 @@
 //
 //   k_gemm_bf16   C(M x N, fp32) = op(A)(M x K) op(B)(K x N), BF16 operands,
 //                 FP32 accumulation on v_mfma_f32_32x32x8bf16_1k; one wave per
-//                 32 x 32 output tile; strided operand access gives both
-//                 transposes the backward pass needs
+//                 32 x 32 output tile, operand strips staged in LDS 64 k at a
+//                 time; strided operand access gives both transposes the
+//                 backward pass needs
 //   k_bias_act    + bias, ReLU, BF16 copy of the activation (forward)
 //   k_mse_grad    dY = 2 (Y - T) / (batch * out), loss partials
 //   k_relu_back   dZ = dA * (Z > 0), BF16 copy
-//   k_colsum      bias gradient (column sums of dZ)
+//   k_colsum      bias gradient (column sums of dZ, 16 row groups per column)
 //   k_adam        Adam step on the FP32 master weights + BF16 refresh
 //
 // Shapes are multiples of 32 (rows, columns) and 8 (reduction); the host
@@ -44,37 +45,79 @@ __host__ __device__ inline uint16_t f2bf(float f) {
 }
 
 // One wave per 32 x 32 tile of C.  Operand element (i, k) of A at
-// A[i*sai + k*sak], (k, j) of B at B[k*sbk + j*sbj].  MFMA 32x32x8 layout:
-// lane l supplies A(l%32, 4*(l/32)+u) and B(4*(l/32)+u, l%32), u = 0..3, and
-// holds C(8*(r/4) + 4*(l/32) + r%4, l%32) in accumulator register r.
+// A[i*sai + k*sak], (k, j) of B at B[k*sbk + j*sbj].  The wave stages its
+// 32 x KC strip of A and KC x 32 strip of B in LDS (both k-contiguous, B
+// transposed; rows padded against bank conflicts), every load of a chunk in
+// flight at once -- one memory latency per KC = 64 instead of one per MFMA
+// step -- with lanes running along whichever index of an operand has unit
+// stride, so the global loads coalesce for every transpose.  MFMA 32x32x8
+// layout: lane l supplies A(l%32, 4*(l/32)+u) and B(4*(l/32)+u, l%32),
+// u = 0..3, and holds C(8*(r/4) + 4*(l/32) + r%4, l%32) in accumulator
+// register r; the k steps run in increasing order.  Waves past the last row
+// tile stage a valid tile and store nothing, so every wave reaches the
+// workgroup barriers.
+constexpr int kGemmKC = 64;               // k per staged chunk
+constexpr int kGemmRow = kGemmKC + 4;     // padded LDS row, BF16 elements
+
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A,
                                                    int64_t sai, int64_t sak, const uint16_t* __restrict__ B,
                                                    int64_t sbk, int64_t sbj, float* __restrict__ C, int ldc,
                                                    int accumulate) {
-    const int lane = threadIdx.x & 63;
+    __shared__ uint16_t As[4][32 * kGemmRow];
+    __shared__ uint16_t Bs[4][32 * kGemmRow];  // B strip transposed: [column][k]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int tn = N >> 5;
     const int ti = wave / tn, tj = wave % tn;
-    if (ti >= (M >> 5)) return;
-    const int i = ti * 32 + (lane & 31);
-    const int j = tj * 32 + (lane & 31);
-    const int kq = 4 * (lane >> 5);
+    const bool live = ti < (M >> 5);
+    const int i0 = (live ? ti : 0) * 32, j0 = tj * 32;
+    uint16_t* as = As[wv];
+    uint16_t* bs = Bs[wv];
+    const bool a_rows_unit = sai == 1 && sak != 1;  // A transposed: unit stride along rows
+    const bool b_cols_unit = sbj == 1;              // B row-major: unit stride along columns
+    const int kq = 4 * (lane >> 5), rl = lane & 31;
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        bf16x4 a, b;
+    for (int k0 = 0; k0 < K; k0 += kGemmKC) {
+        const int kc = min(kGemmKC, K - k0);  // a multiple of 8
+        // element e of a 32 x 64 strip: (e % 32, e / 32) along the unit stride
+        // of rows/columns, else (e / 64, e % 64); k >= kc is not loaded.  All 32
+        // loads of each operand are issued before the first LDS store.
+        uint16_t ra[2048 / 64], rb[2048 / 64];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = k0 + kq + u;
-            a[u] = (short)A[i * sai + k * sak];
-            b[u] = (short)B[k * sbk + j * sbj];
+        for (int q = 0; q < 2048 / 64; ++q) {
+            const int e = lane + 64 * q;
+            const int ar = a_rows_unit ? (e & 31) : (e >> 6), ak = a_rows_unit ? (e >> 5) : (e & 63);
+            const int bc = b_cols_unit ? (e & 31) : (e >> 6), bk = b_cols_unit ? (e >> 5) : (e & 63);
+            ra[q] = ak < kc ? A[(int64_t)(i0 + ar) * sai + (int64_t)(k0 + ak) * sak] : (uint16_t)0;
+            rb[q] = bk < kc ? B[(int64_t)(k0 + bk) * sbk + (int64_t)(j0 + bc) * sbj] : (uint16_t)0;
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 2048 / 64; ++q) {
+            const int e = lane + 64 * q;
+            const int ar = a_rows_unit ? (e & 31) : (e >> 6), ak = a_rows_unit ? (e >> 5) : (e & 63);
+            const int bc = b_cols_unit ? (e & 31) : (e >> 6), bk = b_cols_unit ? (e >> 5) : (e & 63);
+            as[ar * kGemmRow + ak] = ra[q];
+            bs[bc * kGemmRow + bk] = rb[q];
+        }
+        __syncthreads();
+        for (int ks = 0; ks < kc; ks += 8) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = (short)as[rl * kGemmRow + ks + kq + u];
+                b[u] = (short)bs[rl * kGemmRow + ks + kq + u];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
     }
+    if (!live) return;
+    const int j = j0 + rl;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int row = ti * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int row = i0 + 8 * (r >> 2) + kq + (r & 3);
         float* c = C + (int64_t)row * ldc + j;
         *c = accumulate ? *c + acc[r] : acc[r];
     }
@@ -122,13 +165,25 @@ __global__ void k_relu_back(int64_t n, float* dA, const float* Z, uint16_t* dZbf
     dZbf[k] = f2bf(g);
 }
 
-// db[j] = sum_i dZ[i, j] (serial over rows per column: fixed order).
-__global__ void k_colsum(int rows, int cols, const float* dZ, float* db) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
+// db[j] = sum_i dZ[i, j]: a workgroup takes 64 columns with 16 row groups;
+// a thread sums rows g, g + 16, ... of its column in order, and the 16
+// partials are added in group order (a fixed order: deterministic).
+constexpr int kColsumGroups = 16;
+__global__ __launch_bounds__(64 * kColsumGroups) void k_colsum(int rows, int cols, const float* __restrict__ dZ,
+                                                               float* __restrict__ db) {
+    __shared__ float part[kColsumGroups][64];
+    const int tx = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + tx;
     float s = 0.0f;
-    for (int i = 0; i < rows; ++i) s += dZ[(int64_t)i * cols + j];
-    db[j] = s;
+    if (j < cols)
+        for (int i = g; i < rows; i += kColsumGroups) s += dZ[(int64_t)i * cols + j];
+    part[g][tx] = s;
+    __syncthreads();
+    if (g == 0 && j < cols) {
+        float t = 0.0f;
+        for (int q = 0; q < kColsumGroups; ++q) t += part[q][tx];
+        db[j] = t;
+    }
 }
 
 // Adam (Kingma & Ba 2015) on fp32 master params; refreshes the BF16 weights.
@@ -302,7 +357,8 @@ struct mcp_mlp {
             const int K = dims[l], N = dims[l + 1];
             // dW_l = A_l^T (K x rows) * dZ (rows x N)
             gemm(K, N, rows, Abf[l], 1, K, dbf, N, 1, g + woff[l], N, false);
-            hipLaunchKernelGGL(k_colsum, dim3(blocks(N)), dim3(256), 0, s, rows, N, dA, g + boff[l]);
+            hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(64 * kColsumGroups), 0, s, rows, N, dA,
+                               g + boff[l]);
             if (l == 0) break;
             // dA_{l} = dZ (rows x N) * W_l^T (N x K)
             gemm(rows, K, N, dbf, N, 1, wbf + woff[l], 1, N, dB, K, false);
