@@ -67,6 +67,7 @@ WORKLOADS = {
 
 
 SCHEMES = {"reference": 0, "kkt": 1, "halpern": 2}  # mcpdlp.SCHEME_*
+PRECISIONS = {"fp64": 0, "mixed": 1, "packed": 2}  # mcpdlp.PRECISION_*
 
 
 def log(*a):
@@ -88,7 +89,7 @@ def main():
     ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
     ap.add_argument("--tolerance", type=float, default=None)
     ap.add_argument("--max-iterations", type=int, default=2_000_000)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "packed", "fp64"])
     ap.add_argument("--plain", action="store_true",
                     help="also time the plain FP64 GPU solve of the same LP to the same tolerance")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
@@ -139,7 +140,7 @@ def main():
     lp = mp.generate(wl["family"], **wl["spec"])
     log(f"[bench] generated {wl['name']}: m={lp.rows} n={lp.cols} nnz={lp.nnz} in {time.time() - t:.1f}s")
 
-    prec = mp.PRECISION_MIXED if args.precision == "mixed" else mp.PRECISION_FP64
+    prec = PRECISIONS[args.precision]
     common = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
                   graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
     opts = mp.options(precision=prec, max_iterations=args.max_iterations, **common)
@@ -171,7 +172,7 @@ def main():
     while len(walls) < max(1, args.steps):
         if walls:
             per = max(walls) / 1e3
-            want_fp64 = fp64 is None and prec == mp.PRECISION_MIXED and not args.no_fp64_compare
+            want_fp64 = fp64 is None and prec != mp.PRECISION_FP64 and not args.no_fp64_compare
             slack = left() - reserve - (1.3 * per if want_fp64 else 0.0)
             if slack < 1.15 * per:
                 break
@@ -182,7 +183,7 @@ def main():
             f"setup={res['setup_ms']:.1f} h2d={res['h2d_ms']:.1f} d2h={res['d2h_ms']:.1f} ({left():.0f}s left)")
     ms = float(np.mean(walls))
 
-    if prec == mp.PRECISION_MIXED and not args.no_fp64_compare and left() - reserve > 1.3 * max(walls) / 1e3:
+    if prec != mp.PRECISION_FP64 and not args.no_fp64_compare and left() - reserve > 1.3 * max(walls) / 1e3:
         # capped so that it ends inside the budget even if its trajectory is longer
         per_it_s = 1.2 * max(walls) / 1e3 / max(1, res["iterations"])
         cap = int(min(args.max_iterations, max(1000, (left() - reserve) / per_it_s)))
@@ -237,7 +238,7 @@ def main():
         "metric": METRIC,
         "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": len(walls), "warmup": warm_done,
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "fp32+fp64" if prec == mp.PRECISION_MIXED else "fp64", "data": "synthetic (planted-feasible, seeded)",
+        "dtype": "fp64" if prec == mp.PRECISION_FP64 else "fp32+fp64", "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
                    "precision": args.precision, "restart_scheme": args.scheme, "parallelism": "single GPU"},
         "steps_requested": args.steps, "warmup_requested": args.warmup,
@@ -440,7 +441,7 @@ def bench_batch(args):
     kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1,
               max_iterations=args.max_iterations, graph_chunk=args.graph_chunk)
     t0 = time.perf_counter()
-    prec = mp.PRECISION_MIXED if args.precision == "mixed" else mp.PRECISION_FP64
+    prec = PRECISIONS[args.precision]
     res = mp.solve_batch(lps, precision=prec, restart_scheme=SCHEMES[args.scheme], **kw)
     wall = (time.perf_counter() - t0) * 1e3
     its = np.array([r["iterations"] for r in res])

@@ -79,6 +79,11 @@ typedef struct mc_lp {
 #define MC_PRECISION_MIXED 1 /* FP32 SpMVs (values, gathered vectors, products and sums); the
                               * iterate, its updates and every check stay FP64; all-FP64
                               * iterations once the FP32-SpMV phase stalls              */
+#define MC_PRECISION_PACKED 2 /* SpMV values packed in 8 bytes per nonzero with the index:
+                               * the FP32 value plus an 8-bit correction in the index word's
+                               * top byte (about 31 significant bits); FP64 gathered vectors,
+                               * products and sums, FP64 iterate and checks; all-FP64
+                               * iterations if this phase stalls.  Needs m, n < 2^24.    */
 /* Restart scheme (extension; both backends).
  * REFERENCE: the reference's rule (pdlp.cpp:246-260): restart to the average when
  *   its score fell to 0.368 x the last restart's, primal weight from sqrt(primal/dual).
@@ -175,10 +180,12 @@ typedef struct mc_pdlp_result {
     char message[128];
     char backend_actually_used[32]; /* "cpu", "hip" or "cpu_fallback" */
     /* --- extensions --- */
-    uint64_t fp32_iterations;       /* iterations with FP32 SpMVs before the all-FP64 phase (mixed) */
+    uint64_t fp32_iterations;       /* iterations with FP32 SpMVs (mixed)                */
     double setup_ms;                /* Ruiz + transpose + norms on the device          */
     double primal_weight;           /* final primal weight omega (scaled space)        */
     double step_size;               /* final step-size factor eta                      */
+    uint64_t packed_iterations;     /* iterations on the packed stream (MC_PRECISION_PACKED, or
+                                     * mixed after its FP32 phase)                     */
 } mc_pdlp_result;
 
 /* Fill `opt` with the reference defaults (pdlp.hpp:19-34) and the extension defaults. */

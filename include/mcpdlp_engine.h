@@ -132,12 +132,13 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats);
 /* Per-instance outcome. */
 typedef struct mcp_inst_result {
     int32_t status;             /* MC_PDLP_* */
-    int32_t mode;               /* SpMV precision at the end: 0 FP64, 1 FP32 */
+    int32_t mode;               /* SpMV precision at the end: 0 FP64, 1 FP32, 2 packed */
     uint64_t iterations;
-    uint64_t fp32_iterations;
+    uint64_t fp32_iterations;   /* iterations of the FP32-SpMV phase */
     double objective;
     double primal_infeasibility, dual_infeasibility, duality_gap;
     double omega, eta;
+    uint64_t packed_iterations; /* iterations of the packed-stream phase */
 } mcp_inst_result;
 
 /* Copy the unscaled averaged iterate (x[n], y[m]; either may be NULL) and the

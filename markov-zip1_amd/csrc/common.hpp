@@ -35,9 +35,12 @@ constexpr int32_t kOptimal = 1;
 constexpr int32_t kIterLimit = 2;
 constexpr int32_t kNumFail = 3;
 constexpr int32_t kInfeasible = 4;  // a primal or dual ray certificate was found
-// Arithmetic mode of an instance's iterations.
+// Arithmetic mode of an instance's SpMVs (the iterate, its updates and the
+// checks are FP64 in every mode).
 constexpr int32_t kModeFp64 = 0;
-constexpr int32_t kModeFp32 = 1;
+constexpr int32_t kModeFp32 = 1;  // FP32 values, FP32 gathered vectors, FP32 products and sums
+constexpr int32_t kModePacked = 2;  // packed stream (FP32 value + 8-bit correction in the index
+                                    // word): FP64 gathered vectors, products and sums
 
 // Per-instance scalar PDHG state (device resident, one per LP).
 struct InstState {
@@ -48,12 +51,12 @@ struct InstState {
     double best_score;       // mixed precision: best checked score while in FP32
     int64_t restart_iter;    // global iteration count at the last restart
     int64_t done_iter;       // iteration count when the instance stopped
-    int64_t fp32_iters;      // iterations spent in FP32 before switching to FP64
+    int64_t fp32_iters;      // iteration count when the FP32-SpMV phase ended
     int32_t status;
-    int32_t mode;            // kModeFp64 / kModeFp32 (FP32 SpMV values and gathers; the
-                             // iterate and every update stay FP64 in both modes)
+    int32_t mode;            // kModeFp64 / kModeFp32 / kModePacked (the iterate and every
+                             // update stay FP64 in all modes)
     int32_t restart_flag;    // set by a check, consumed by k_post_check
-    int32_t switch_flag;     // set by the check that moved the instance from FP32 to FP64
+    int32_t switch_flag;     // set by the check that moved the instance to another SpMV mode
     int32_t mode_at_check;   // mode of the iterations before the last check
     int32_t stall;           // FP32: checks since the last new best score
     int32_t cand_cur;        // KKT scheme: this check's candidate is the current iterate
@@ -64,6 +67,7 @@ struct InstState {
     double b_norm_inf, c_norm_inf;  // setup: norms of the scaled b / c (floored at 1)
     double eta_cap;          // largest step factor of the adaptive rule: 0.99, or 0.95 / L
                              // (Halpern scheme, L the power-method norm of the scaled matrix)
+    int64_t packed_end;      // iteration count when the packed-stream phase ended
 };
 
 // Options every iteration kernel reads (passed by value).
@@ -85,6 +89,8 @@ struct IterOpts {
     int32_t detect;              // test the iterate's movement for infeasibility certificates
     int32_t adaptive_step;       // A dx + eta update every 10 iterations (part of the captured
                                  // kernel sequence, so a change must drop the cached graphs)
+    int32_t packed_next;         // the FP32-SpMV phase hands over to the packed stream (else FP64)
+    int32_t packed_floor;        // the check also scores the average with the packed values
     int32_t pad_;
 };
 
@@ -175,14 +181,58 @@ __device__ __forceinline__ T block_max(T v, T* scratch) {
     return r;
 }
 
+// Packed matrix stream (MC_PRECISION_PACKED): each nonzero is one uint32
+// word holding the column (row) index in its low 24 bits and a signed 8-bit
+// correction k in its top byte, beside the FP32 value a32 = fl32(a).  The
+// value used is a32 + k ulp(a32)/256 = a32 + k 2^(e-31) (e the exponent of
+// a32), exact in FP64 and within ulp(a32)/256 of the FP64 value a: about 31
+// significant bits for the 8 bytes per nonzero of an FP32 stream.
+constexpr uint32_t kPackIndexMask = 0xFFFFFFu;
+constexpr int32_t kPackMaxIndex = 1 << 24;  // indices must stay below this
+// The packed stream hands over to FP64 once the check's score is within
+// kFloorMargin of the packed floor -- the largest change of a KKT component
+// when the check's candidate is scored with the packed values instead of the
+// FP64 ones -- or below kPackedFloor.
+constexpr double kFloorMargin = 8.0;
+constexpr double kPackedFloor = 1e-8;
+__device__ __forceinline__ double packed_value(float a32, uint32_t w) {
+    const int k = (int)(int8_t)(uint8_t)(w >> 24);
+    const float p2 = __int_as_float(__float_as_int(a32) & 0x7F800000);  // 2^e; 0 for zero/subnormal
+    return (double)a32 + (double)k * ((double)p2 * 0x1p-31);
+}
+// The correction k of an FP64 value a with a32 = fl32(a):
+// round((a - a32) / 2^(e-31)) clamped to [-127, 127].
+__device__ __forceinline__ int pack_correction(double a, float a32) {
+    const float p2 = __int_as_float(__float_as_int(a32) & 0x7F800000);
+    if (p2 == 0.0f) return 0;
+    const double r = (a - (double)a32) / ((double)p2 * 0x1p-31);
+    return (int)rint(r < -127.0 ? -127.0 : (r > 127.0 ? 127.0 : r));
+}
+// The value the packed stream holds for the FP64 value a (what the SpMVs of
+// the packed mode multiply by), recomputed from a alone.
+__device__ __forceinline__ double packed_of(double a) {
+    const float a32 = (float)a;
+    const float p2 = __int_as_float(__float_as_int(a32) & 0x7F800000);
+    return (double)a32 + (double)pack_correction(a, a32) * ((double)p2 * 0x1p-31);
+}
+// Index and value of one stream element: plain (int32 index, value as
+// stored) or packed (uint32 word, FP32 value).
+__device__ __forceinline__ int ix_of(int32_t i) { return i; }
+__device__ __forceinline__ int ix_of(uint32_t w) { return (int)(w & kPackIndexMask); }
+template <typename TS, typename TA>
+__device__ __forceinline__ TS val_of(TA a, int32_t) { return (TS)a; }
+template <typename TS>
+__device__ __forceinline__ TS val_of(float a, uint32_t w) { return (TS)packed_value(a, w); }
+
 // Dot product of one CSR row [p0, p1) with a gathered vector, by the G lanes
 // of a row group (gl = lane within the group).  Each lane keeps four
 // (index, value) loads and four gathers in flight; the group then reduces by
 // xor-shuffle.  With G = 1 the lane adds the products in increasing index
 // order, one rounding per product and per add -- the reference's serial
 // order.  Every lane of the wave must call (the shuffles need their partners).
-template <typename TS, typename TA, typename TV>
-__device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const int32_t* __restrict__ idx,
+// RP: multiply by packed_of(value) (the packed stream's value of an FP64 entry).
+template <typename TS, typename TA, typename TV, typename IX = int32_t, bool RP = false>
+__device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const IX* __restrict__ idx,
                                       const TA* __restrict__ val, const TV* __restrict__ vec) {
     // Every round issues four index/value loads and four gathers per lane
     // before the first product is used, the tail round included: a lane
@@ -194,12 +244,12 @@ __device__ __forceinline__ TS row_dot(int p0, int p1, int gl, int G, const int32
     for (int q = p0 + gl; q < p1; q += 4 * G) {
         const bool m1 = q + G < p1, m2 = q + 2 * G < p1, m3 = q + 3 * G < p1;
         const int q1 = m1 ? q + G : q, q2 = m2 ? q + 2 * G : q, q3 = m3 ? q + 3 * G : q;
-        const int i0 = idx[q], i1 = idx[q1], i2 = idx[q2], i3 = idx[q3];
+        const IX w0 = idx[q], w1 = idx[q1], w2 = idx[q2], w3 = idx[q3];
         const TA a0 = val[q], a1 = val[q1], a2 = val[q2], a3 = val[q3];
-        const TS t0 = (TS)a0 * (TS)vec[i0];
-        const TS t1 = (TS)a1 * (TS)vec[i1];
-        const TS t2 = (TS)a2 * (TS)vec[i2];
-        const TS t3 = (TS)a3 * (TS)vec[i3];
+        const TS t0 = (RP ? (TS)packed_of((double)a0) : val_of<TS>(a0, w0)) * (TS)vec[ix_of(w0)];
+        const TS t1 = (RP ? (TS)packed_of((double)a1) : val_of<TS>(a1, w1)) * (TS)vec[ix_of(w1)];
+        const TS t2 = (RP ? (TS)packed_of((double)a2) : val_of<TS>(a2, w2)) * (TS)vec[ix_of(w2)];
+        const TS t3 = (RP ? (TS)packed_of((double)a3) : val_of<TS>(a3, w3)) * (TS)vec[ix_of(w3)];
         s += t0;
         if (m1) s += t1;
         if (m2) s += t2;

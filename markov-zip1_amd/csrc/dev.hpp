@@ -32,6 +32,10 @@ struct Dev {
     double* tv64;       // CSR(A^T) values
     float* av32;
     float* tv32;
+    // packed streams of A and A^T (kModePacked): index | correction << 24 per
+    // nonzero, beside av32 / tv32; null when an index does not fit 24 bits
+    uint32_t* cix;
+    uint32_t* rix;
     const int32_t* inst_row;   // [B+1] packed row offsets
     const int32_t* inst_col;   // [B+1]
     const int32_t* row_inst;   // [m] instance of each row
@@ -124,8 +128,9 @@ void launch_adx(const Dev& d, int t, bool exact, hipStream_t s);
 void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s);
 void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s);
 // The check's residual terms: the average and, with `both`, the current
-// iterate (block tiles in one pass when the engine uses them).
-void launch_check_resid(const Dev& d, bool exact, bool both, hipStream_t s);
+// iterate (block tiles in one pass when the engine uses them); with
+// `packed_floor` also the average scored with the packed stream's values.
+void launch_check_resid(const Dev& d, bool exact, bool both, bool packed_floor, hipStream_t s);
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
                            hipStream_t s);
 void launch_post_check(const Dev& d, bool kkt, hipStream_t s);
@@ -142,7 +147,8 @@ void launch_step_cap(const Dev& d, int64_t iters, bool exact, hipStream_t s);
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);
 void launch_setup_data(const Dev& d, hipStream_t s);
 void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm_y, hipStream_t s);
-void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, bool mixed,
+// start_mode: the SpMV mode every instance starts in (kModeFp64 / kModeFp32 / kModePacked).
+void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, int start_mode,
                         hipStream_t s);
 void launch_fill(double* p, double v, int64_t n, hipStream_t s);
 // Plain streaming copy of n floats (n a multiple of 4, 16-byte aligned
@@ -161,7 +167,13 @@ void launch_dual_blk(const Dev& d, int t, hipStream_t s, hipEvent_t e0 = nullptr
 void launch_adx_blk(const Dev& d, int t, hipStream_t s);
 // Residual terms of the check's candidates over block tiles in one pass:
 // the average only, or (both) the average and the current iterate.
-void launch_resid_blk(const Dev& d, bool both, hipStream_t s);
+void launch_resid_blk(const Dev& d, bool both, bool packed_floor, hipStream_t s);
+// Packed-stream SpMV (index words with the value correction, FP32 values,
+// FP64 vector, products and sums), block and wave forms.
+void launch_spmv_blk_packed(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const uint32_t* idxp,
+                            const float* val32, const double* vec, double* out, hipStream_t s);
+void launch_spmv_plain_packed(const Tile* tiles, int ntiles, const int32_t* ptr, const uint32_t* idxp,
+                              const float* val32, const double* vec, double* out, hipStream_t s);
 void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
                      const void* val, const void* vec, void* out, bool fp32, hipStream_t s);
 // Gather probe (FP64 block SpMV, stream.hip): mode 0 gathers vec[idx], mode 1

@@ -301,8 +301,10 @@ struct mcp_engine {
         // partial-sum slots follow the tile counts
         d.part_dxn = buf<double>("part_dxn", std::max(tT.size(), bT.size()));
         d.part_adx = buf<double>("part_adx", std::max(tA.size(), bA.size()));
-        d.part_rows = buf<double>("part_rows", 6 * std::max(tA.size(), bA.size()));
-        d.part_cols = buf<double>("part_cols", 10 * std::max(tT.size(), bT.size()));
+        // three candidates per check: the average, the current iterate, the
+        // average on the packed stream's values
+        d.part_rows = buf<double>("part_rows", 9 * std::max(tA.size(), bA.size()));
+        d.part_cols = buf<double>("part_cols", 15 * std::max(tT.size(), bT.size()));
         d.part_ray_rows = buf<double>("part_ray_rows", 5 * tA.size());
         d.part_ray_cols = buf<double>("part_ray_cols", 7 * tT.size());
         d.tA = dA;
@@ -456,7 +458,9 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.restart_strategy = o->restart_strategy;
     io.adaptive_weight = o->adaptive_primal_weight;
     io.ruiz = 1;
-    io.mixed = o->precision == MC_PRECISION_MIXED;
+    io.mixed = o->precision != MC_PRECISION_FP64;
+    io.packed_next = 0;  // set by mcp_solve when the packed streams exist
+    io.packed_floor = 0;
     io.kkt = (o->restart_scheme == MC_SCHEME_KKT || o->restart_scheme == MC_SCHEME_HALPERN) ? 1 : 0;
     io.halpern = o->restart_scheme == MC_SCHEME_HALPERN ? 1 : 0;
     io.stall_checks = o->fp32_stall_checks > 0 ? o->fp32_stall_checks : 50;
@@ -485,7 +489,7 @@ void enqueue_chunk(mcp_engine* e, const mcp::IterOpts& io, bool adaptive, bool e
         }
         mcp::launch_dual(d, t, exact, s);
         if ((it + 1) % io.restart_every == 0) {
-            mcp::launch_check_resid(d, exact, io.kkt != 0, s);
+            mcp::launch_check_resid(d, exact, io.kkt != 0, io.packed_floor != 0, s);
             mcp::launch_check_finalize(d, io, t, exact, false, s);
             if (io.detect) mcp::launch_ray_check(d, io, t, s);
             mcp::launch_post_check(d, io.kkt != 0, s);
@@ -605,6 +609,10 @@ int mcp_load(mcp_engine* e, const mcp_problem* p) {
         HIPCHK(hipEventRecord(e->ev[1], s));
         d.av32 = e->buf<float>("av32", nnz);
         d.tv32 = e->buf<float>("tv32", nnz);
+        // packed streams (index | value correction << 24) when every index fits 24 bits
+        const bool packable = m < mcp::kPackMaxIndex && n < mcp::kPackMaxIndex;
+        d.cix = packable ? e->buf<uint32_t>("cix", nnz) : nullptr;
+        d.rix = packable ? e->buf<uint32_t>("rix", nnz) : nullptr;
         d.row_scale = e->buf<double>("row_scale", m);
         d.col_scale = e->buf<double>("col_scale", n);
         d.rtmp = e->buf<double>("rtmp", m);
@@ -712,7 +720,9 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         require(e->setup_done, "mcp_solve before mcp_setup");
         require(o->restart_every >= 1, "restart_every must be >= 1");
         const bool exact = o->reduction_order == MC_ORDER_REFERENCE;
-        const bool mixed = o->precision == MC_PRECISION_MIXED;
+        require(o->precision >= MC_PRECISION_FP64 && o->precision <= MC_PRECISION_PACKED, "unknown precision");
+        require(o->precision != MC_PRECISION_PACKED || e->d.cix, "packed precision needs fewer than 2^24 rows and columns");
+        const bool mixed = o->precision != MC_PRECISION_FP64;
         require(!(exact && mixed), "reference reduction order requires FP64 precision");
         require(!exact || e->exact_tiles, "reference order needs mcp_setup(exact=1)");
         require(!(exact && o->detect_infeasibility), "infeasibility detection runs in the fast reduction order");
@@ -723,6 +733,8 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                 "unknown restart scheme");
         mcp::IterOpts io{};
         make_opts(o, io);
+        io.packed_next = (o->precision == MC_PRECISION_MIXED && d.cix) ? 1 : 0;
+        io.packed_floor = (!exact && (io.packed_next || o->precision == MC_PRECISION_PACKED)) ? 1 : 0;
         d.halpern = io.halpern;
         // start point
         const double* wx = nullptr;
@@ -738,11 +750,14 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
             wy = t;
         }
         mcp::launch_init_iterates(d, wx, wy, s);
-        // The FP32-SpMV phase serves the start far from the optimum; its
-        // rounding floor (1e-2 to 1e-4 relative KKT on planted LPs) lies above
-        // a warm start's accuracy, so a warm-started solve begins in FP64.
+        // The reduced-precision phase serves the start far from the optimum;
+        // the FP32-SpMV floor (1e-2 to 1e-4 relative KKT on planted LPs) lies
+        // above a warm start's accuracy, so a warm-started solve begins in FP64.
         const bool fp32_start = mixed && !wx && !wy;
-        mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, fp32_start, s);
+        const int start_mode = !fp32_start ? mcp::kModeFp64
+                               : o->precision == MC_PRECISION_MIXED ? mcp::kModeFp32
+                                                                    : mcp::kModePacked;
+        mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, start_mode, s);
         if (io.halpern && io.adaptive_step && o->power_iterations > 0)
             mcp::launch_step_cap(d, (int64_t)o->power_iterations, exact, s);
         int32_t* pin = e->pinned;
@@ -852,7 +867,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                                      "(p=%.2e d=%.2e g=%.2e) omega=%.3g eta=%.3g mode=%s\n",
                                      std::chrono::duration<double>(now - prog_start).count(), (long long)iter,
                                      pin[2 * prev], pin[2 * prev + 1], s0.score, s0.primal, s0.dual, s0.gap,
-                                     s0.omega, s0.eta, s0.mode == mcp::kModeFp32 ? "fp32" : "fp64");
+                                     s0.omega, s0.eta, s0.mode == mcp::kModeFp32 ? "fp32" : s0.mode == mcp::kModePacked ? "packed" : "fp64");
                         std::fflush(stderr);
                     }
                 }
@@ -863,7 +878,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         const int32_t still_running = chunks > 0 ? pin[2 * last] : e->B;
         if (!finished && still_running > 0) {
             // iteration limit: residuals of the final average (pdlp.cpp:263-265)
-            mcp::launch_check_resid(d, exact, io.kkt != 0, s);
+            mcp::launch_check_resid(d, exact, io.kkt != 0, io.packed_floor != 0, s);
             mcp::launch_check_finalize(d, io, -1, exact, true, s);
             HIPCHK(hipGetLastError());
         }
@@ -917,6 +932,7 @@ int mcp_fetch(mcp_engine* e, double* x, double* y, mcp_inst_result* res, double*
                 r.mode = q.mode;
                 r.iterations = (uint64_t)q.done_iter;
                 r.fp32_iterations = (uint64_t)q.fp32_iters;
+                r.packed_iterations = q.packed_end > q.fp32_iters ? (uint64_t)(q.packed_end - q.fp32_iters) : 0;
                 r.objective = q.objective;
                 r.primal_infeasibility = q.primal;
                 r.dual_infeasibility = q.dual;
@@ -936,8 +952,10 @@ int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
         hipStream_t s = e->stream;
         const bool t = transpose != 0;
         const bool f32 = precision == MC_PRECISION_MIXED;
+        const bool pk = precision == MC_PRECISION_PACKED;
         const bool exact = order == MC_ORDER_REFERENCE;
-        require(!(exact && f32), "reference order is FP64 only");
+        require(!(exact && (f32 || pk)), "reference order is FP64 only");
+        require(!pk || d.cix, "the packed streams need indices below 2^24");
         const int nin = t ? e->m : e->n, nout = t ? e->n : e->m;
         std::vector<mcp::Tile> tl;
         std::vector<int32_t> tb;
@@ -968,8 +986,16 @@ int mcp_spmv(mcp_engine* e, int32_t transpose, int32_t precision, int32_t order,
         const int32_t* ptr = t ? d.cp : d.rp;
         const int32_t* idx = t ? d.ri : d.ci;
         const void* val = f32 ? (const void*)(t ? d.tv32 : d.av32) : (const void*)(t ? d.tv64 : d.av64);
+        const uint32_t* idxp = t ? d.rix : d.cix;
+        const float* val32 = t ? d.tv32 : d.av32;
         auto launch = [&] {
-            if (blk)
+            if (pk && blk)
+                mcp::launch_spmv_blk_packed(dt, (int)tl.size(), bud, ptr, idxp, val32, (const double*)din,
+                                            (double*)dout, s);
+            else if (pk)
+                mcp::launch_spmv_plain_packed(dt, (int)tl.size(), ptr, idxp, val32, (const double*)din, (double*)dout,
+                                              s);
+            else if (blk)
                 mcp::launch_spmv_blk(dt, (int)tl.size(), bud, ptr, idx, val, din, dout, f32, s);
             else
                 mcp::launch_spmv_plain(dt, (int)tl.size(), ptr, idx, val, din, dout, f32, s);
@@ -1017,8 +1043,12 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
 int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes) {
     return guarded([&] {
         require(e && e->setup_done, "mcp_time_kernel before mcp_setup");
-        require(kind >= 0 && kind <= 9, "unknown kernel kind");
-        const bool probe = kind >= 7;  // gather probes over the block tiles of A (FP64)
+        require(kind >= 0 && kind <= 11, "unknown kernel kind");
+        const bool probe = kind >= 7 && kind <= 9;  // gather probes over the block tiles of A (FP64)
+        // kinds 10, 11: the fused primal / dual step on the packed streams
+        // (FP32 value + 8-bit correction) with FP64 gathers, products and sums
+        const bool packed = kind == 10 || kind == 11;
+        require(!packed || e->d.cix, "the packed streams need indices below 2^24");
         require(!probe || (e->d.blocks && e->d.nbA > 0 && e->n > 0), "gather probes need block tiles of A");
         HIPCHK(hipSetDevice(e->device));
         auto& d = e->d;
@@ -1037,6 +1067,10 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
             b = (4.0 + w) * nnz + 4.0 * (n + 1) + w * m + (72.0 + (f32 ? 4.0 : 0.0)) * n;
         else if (kind == 1 || kind == 3)
             b = (4.0 + w) * nnz + 4.0 * (m + 1) + w * n + (56.0 + (f32 ? 4.0 : 0.0)) * m;
+        else if (kind == 10)
+            b = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * m + 72.0 * n;
+        else if (kind == 11)
+            b = 8.0 * nnz + 4.0 * (m + 1) + 8.0 * n + 56.0 * m;
         else if (kind == 4)
             b = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 4.0 * m;
         else if (kind == 5)
@@ -1058,7 +1092,7 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         HIPCHK(hipMemcpy(st.data(), d.st, sizeof(mcp::InstState) * e->B, hipMemcpyDeviceToHost));
         for (auto& q : st) {
             q.status = mcp::kRunning;
-            q.mode = f32 ? mcp::kModeFp32 : mcp::kModeFp64;
+            q.mode = f32 ? mcp::kModeFp32 : packed ? mcp::kModePacked : mcp::kModeFp64;
             q.restart_iter = 0;
             if (!(q.eta > 0.0)) q.eta = 0.9;
             if (!(q.omega > 0.0)) q.omega = 1.0;
@@ -1069,9 +1103,11 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         auto launch = [&] {
             switch (kind) {
                 case 0:
-                case 2: mcp::launch_primal(d, 1, false, false, s); break;
+                case 2:
+                case 10: mcp::launch_primal(d, 1, false, false, s); break;
                 case 1:
-                case 3: mcp::launch_dual(d, 1, false, s); break;
+                case 3:
+                case 11: mcp::launch_dual(d, 1, false, s); break;
                 case 4: mcp::launch_spmv_plain(d.tA, d.ntA, d.rp, d.ci, d.av32, d.xbar32, d.y32, true, s); break;
                 case 5: mcp::launch_spmv_plain(d.tT, d.ntT, d.cp, d.ri, d.tv32, d.y32, d.dx32, true, s); break;
                 case 6: mcp::launch_copy(d.av32, copy_dst, ncopy, s); break;
@@ -1082,12 +1118,12 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
         };
         launch();
         const int nrep = std::max(1, reps);
-        if (kind <= 3 || probe) {
+        if (kind <= 3 || probe || packed) {
             // The fused steps are timed as the solve runs them: each launch
             // follows the other step, which streams the other matrix through
             // the caches in between.
             auto partner = [&] {
-                if (kind == 0 || kind == 2)
+                if (kind == 0 || kind == 2 || kind == 10)
                     mcp::launch_dual(d, 1, false, s);
                 else
                     mcp::launch_primal(d, 1, false, false, s);
@@ -1104,7 +1140,7 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
                 for (auto& ev : evs) HIPCHK(hipEventCreate(&ev));
                 for (int r = 0; r < nrep; ++r) {
                     partner();
-                    if (kind == 0 || kind == 2)
+                    if (kind == 0 || kind == 2 || kind == 10)
                         mcp::launch_primal(d, 1, false, false, s, evs[2 * r], evs[2 * r + 1]);
                     else if (probe)
                         mcp::launch_gather_probe(d.bA, d.nbA, d.budA, d.rp, d.ci, d.av64, d.xbar, d.ax, kind - 7,

@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
     const int64_t iter = *d.iter_base + t;
     const double eta = s.eta, omega = s.omega;
     const double kavg = (double)(iter - s.restart_iter + 1);
-    const bool f32 = !EXACT && s.mode == kModeFp32;
+    const int mode = EXACT ? kModeFp64 : s.mode;
+    const bool f32 = mode == kModeFp32;
     // Halpern scheme: x_avg <- T(z)_x = x+, x <- w x_bar + (1 - w) x0 with
     // x_bar = 2 x+ - x (the reflection), w = k/(k+1), k = kavg.
     const bool halp = d.halpern != 0;
@@ -87,7 +88,8 @@ __global__ __launch_bounds__(kBlock) void k_primal(Dev d, int t) {
             xa = xsrc[j];  // running average, or the Halpern anchor
         }
         const double aty = f32 ? (double)row_dot<float>(p0, p1, L.gl, L.G, d.ri, d.tv32, d.y32)
-                               : row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.y);
+                           : mode == kModePacked ? row_dot<double>(p0, p1, L.gl, L.G, d.rix, d.tv32, d.y)
+                                               : row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, d.y);
         if (own && L.gl == 0) {
             const double g = c + aty;
             const double tau = (eta / omega) / cn;
@@ -131,7 +133,8 @@ __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
     const int64_t iter = *d.iter_base + t;
     const double eta = s.eta, omega = s.omega;
     const double kavg = (double)(iter - s.restart_iter + 1);
-    const bool f32 = !EXACT && s.mode == kModeFp32;
+    const int mode = EXACT ? kModeFp64 : s.mode;
+    const bool f32 = mode == kModeFp32;
     // Halpern scheme: y_avg <- T(z)_y = y+, y <- w (2 y+ - y) + (1 - w) y0.
     const bool halp = d.halpern != 0;
     const double w = kavg / (kavg + 1.0);
@@ -151,7 +154,8 @@ __global__ __launch_bounds__(kBlock) void k_dual(Dev d, int t) {
             ya = ysrc[i];
         }
         const double ax = f32 ? (double)row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.xbar32)
-                              : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.xbar);
+                          : mode == kModePacked ? row_dot<double>(p0, p1, L.gl, L.G, d.cix, d.av32, d.xbar)
+                                              : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.xbar);
         if (own && L.gl == 0) {
             const double sigma = (eta * omega) / rn;
             const double v = yo + sigma * ax;
@@ -180,7 +184,8 @@ __global__ __launch_bounds__(kBlock) void k_adx(Dev d, int) {
     const InstState& s = d.st[tl.inst];
     if (s.status != kRunning) return;
     const Lanes L = lanes_of(tl);
-    const bool f32 = !EXACT && s.mode == kModeFp32;
+    const int mode = EXACT ? kModeFp64 : s.mode;
+    const bool f32 = mode == kModeFp32;
     double part = 0.0;
     for (int base = tl.r0; base < tl.r1; base += L.step) {
         const int i = base + (L.lane >> L.lg);
@@ -193,7 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_adx(Dev d, int) {
             rn = d.rnorm[i];
         }
         const double a = f32 ? (double)row_dot<float>(p0, p1, L.gl, L.G, d.ci, d.av32, d.dx32)
-                             : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.dx);
+                         : mode == kModePacked ? row_dot<double>(p0, p1, L.gl, L.G, d.cix, d.av32, d.dx)
+                                             : row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, d.dx);
         if (own && L.gl == 0) {
             if (EXACT)
                 d.ax[i] = a;
@@ -272,12 +278,14 @@ __global__ __launch_bounds__(kFinal) void k_adapt_finalize(Dev d) {
 
 // ---------------------------------------------------------------------------
 // FP64 residual terms of a candidate iterate on the unscaled model
-// (pdlp.cpp:76-131).  CUR = false: the running average (the reference's only
-// candidate); CUR = true: the current iterate (KKT restart scheme).  The
-// iterate and the matrix are read in FP64 in both modes.
+// (pdlp.cpp:76-131).  CAND 0: the running average (the reference's only
+// candidate); 1: the current iterate (KKT restart scheme); 2: the average
+// scored with the packed stream's values (the packed floor).  The iterate and
+// the matrix are read in FP64 in all of them.
 // Rows: A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
-template <bool EXACT, bool CUR>
+template <bool EXACT, int CAND>
 __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
+    constexpr bool CUR = CAND == 1;
     const int tile = wave_tile(d.ntA);
     if (tile >= d.ntA) return;
     const Tile tl = d.tA[tile];
@@ -301,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
             yc = y64[i];
             yl = d.ylast[i];
         }
-        const double ax = row_dot<double>(p0, p1, L.gl, L.G, d.ci, d.av64, x64);
+        const double ax = row_dot<double, double, double, int32_t, CAND == 2>(p0, p1, L.gl, L.G, d.ci, d.av64, x64);
         if (own && L.gl == 0) {
             if (EXACT) {
                 (CUR ? d.ax2 : d.ax)[i] = ax;
@@ -326,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
         dob = wave_sum(dob);
         mv = wave_sum(mv);
         if (L.lane == 0) {  // field-major: [field][tile], one block per candidate
-            double* out = d.part_rows + (CUR ? 3 * d.ntA : 0) + tile;
+            double* out = d.part_rows + CAND * 3 * d.ntA + tile;
             out[0] = viol;
             out[d.ntA] = dob;
             out[2 * d.ntA] = mv;
@@ -336,8 +344,9 @@ __global__ __launch_bounds__(kBlock) void k_resid_rows(Dev d) {
 
 // Columns: A^T y_c, bound violation, dual residual, primal/dual objective
 // parts, |x_c - x_last|^2.
-template <bool EXACT, bool CUR>
+template <bool EXACT, int CAND>
 __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
+    constexpr bool CUR = CAND == 1;
     const int tile = wave_tile(d.ntT);
     if (tile >= d.ntT) return;
     const Tile tl = d.tT[tile];
@@ -363,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
             xc = x64[j];
             xl = d.xlast[j];
         }
-        const double aty = row_dot<double>(p0, p1, L.gl, L.G, d.ri, d.tv64, y64);
+        const double aty = row_dot<double, double, double, int32_t, CAND == 2>(p0, p1, L.gl, L.G, d.ri, d.tv64, y64);
         if (own && L.gl == 0) {
             if (EXACT) {
                 (CUR ? d.aty2 : d.aty)[j] = aty;
@@ -394,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_cols(Dev d) {
         dob = wave_sum(dob);
         mv = wave_sum(mv);
         if (L.lane == 0) {  // field-major: [field][tile], one block per candidate
-            double* out = d.part_cols + (CUR ? 5 * d.ntT : 0) + tile;
+            double* out = d.part_cols + CAND * 5 * d.ntT + tile;
             out[0] = viol;
             out[d.ntT] = dres;
             out[2 * d.ntT] = pob;
@@ -486,8 +495,8 @@ __device__ void partial_residuals(const Dev& d, int b, int cand, double c_scale,
     const int ntA = d.blocks ? d.nbA : d.ntA, ntT = d.blocks ? d.nbT : d.ntT;
     const int32_t* ba = d.blocks ? d.bA_begin : d.tA_begin;
     const int32_t* bt = d.blocks ? d.bT_begin : d.tT_begin;
-    const double* pr = d.part_rows + (cand ? 3 * ntA : 0);
-    const double* pc = d.part_cols + (cand ? 5 * ntT : 0);
+    const double* pr = d.part_rows + cand * 3 * ntA;
+    const double* pc = d.part_cols + cand * 5 * ntT;
     const int a0 = ba[b], a1 = ba[b + 1], t0 = bt[b], t1 = bt[b + 1];
     constexpr int NW = kFinal / 64;
     double mp = 0.0, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dr, ym, dsq, po, dc, xm
@@ -522,8 +531,11 @@ __device__ __forceinline__ void stop_instance(const Dev& d, InstState& s, int32_
     s.status = status;
     s.done_iter = it;
     atomicSub(&d.counters[0], 1);
-    if (s.mode == kModeFp32) {  // finished without leaving the FP32-SpMV phase
-        s.fp32_iters = it;
+    if (s.mode != kModeFp64) {  // finished without leaving the reduced-precision phase
+        if (s.mode == kModeFp32)
+            s.fp32_iters = it;
+        else
+            s.packed_end = it;
         atomicSub(&d.counters[1], 1);
     }
 }
@@ -545,16 +557,21 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
     }
     if (!running) return;
     const int64_t iter_after = *d.iter_base + t + 1;
-    CandResid r, rc;
+    CandResid r, rc, rpk;
     if (!EXACT) {
         partial_residuals(d, b, 0, s.c_scale, r, red);
         if (o.kkt) partial_residuals(d, b, 1, s.c_scale, rc, red);
+        if (o.packed_floor) partial_residuals(d, b, 2, s.c_scale, rpk, red);
     }
     if (threadIdx.x != 0) return;
     if (EXACT) {
         serial_residuals(d, b, d.xavg, d.yavg, d.ax, d.aty, s.c_scale, r);
         if (o.kkt) serial_residuals(d, b, d.x, d.y, d.ax2, d.aty2, s.c_scale, rc);
     }
+    // the packed floor: how far the packed values move the average's scores
+    const double pk_floor = (!EXACT && o.packed_floor)
+                                ? maxv(fabs(r.primal - rpk.primal), maxv(fabs(r.dual - rpk.dual), fabs(r.gap - rpk.gap)))
+                                : 0.0;
     int use_cur = 0;
     if (o.kkt) {
         if (rc.score < r.score) {
@@ -641,21 +658,40 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
         s.restart_iter = iter_after;
         s.last_restart_score = score;
     }
-    if (s.mode == kModeFp32) {
-        // FP32 iterations stop paying once the score is near tolerance or has
-        // not set a new best for stall_checks checks (the FP32 rounding
-        // floor); continue from this iterate in FP64.
+    if (s.mode != kModeFp64) {
+        // FP32 products stop paying once the score is near tolerance or has
+        // not set a new best for stall_checks checks (their rounding floor,
+        // 1e-2 to 1e-4 on planted LPs); the instance moves to the packed
+        // stream (about 31 significant bits of A, FP64 sums) or to FP64.  The
+        // packed stream's floor is measured at every check (pk_floor); it is
+        // usually far below the scores where restarted PDHG has plateaus of
+        // tens of thousands of iterations without a new best, so the packed
+        // stream leaves at the polish threshold, within kFloorMargin of its
+        // floor, below kPackedFloor, or after 100x as many checks without a
+        // new best.
         if (score < s.best_score) {
             s.best_score = score;
             s.stall = 0;
         } else {
             s.stall += 1;
         }
-        if (score <= o.polish_threshold || s.stall >= o.stall_checks) {
+        const bool fp32 = s.mode == kModeFp32;
+        const bool leave = score <= o.polish_threshold ||
+                           (fp32 ? s.stall >= o.stall_checks
+                                 : (s.stall >= 100 * o.stall_checks || score <= kPackedFloor ||
+                                    score <= kFloorMargin * pk_floor));
+        if (leave) {
             s.switch_flag = 1;
-            s.mode = kModeFp64;
-            s.fp32_iters = iter_after;
-            atomicSub(&d.counters[1], 1);
+            if (fp32) s.fp32_iters = iter_after;
+            if (fp32 && o.packed_next && score > o.polish_threshold) {
+                s.mode = kModePacked;
+                s.stall = 0;
+                s.best_score = score;
+            } else {
+                if (!fp32) s.packed_end = iter_after;
+                s.mode = kModeFp64;
+                atomicSub(&d.counters[1], 1);
+            }
         }
     }
 }
@@ -959,6 +995,13 @@ __global__ __launch_bounds__(kBlock) void k_ruiz_apply(Dev d) {
     }
 }
 
+// Packed stream word of one nonzero (see packed_value): index | k << 24 with
+// k = round((a - fl32(a)) / (ulp(fl32(a)) / 256)) clamped to [-127, 127].
+__device__ __forceinline__ uint32_t pack_word(int32_t index, double a) {
+    const int k = pack_correction(a, (float)a);
+    return (uint32_t)index | ((uint32_t)(k & 0xFF) << 24);
+}
+
 // Scaled bounds/objective (ruiz_scaling.cpp:221-244), sentinels and norms
 // (pdlp.cpp:153-176) and FP32 copies; once per loaded problem.  The L1 norms
 // are summed serially by the group's first lane in the reference's order.
@@ -975,7 +1018,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_rows(Dev d) {
         const int i = base + (L.lane >> L.lg);
         if (i >= tl.r1) continue;
         const int p0 = d.rp[i], p1 = d.rp[i + 1];
-        for (int p = p0 + L.gl; p < p1; p += L.G) d.av32[p] = (float)d.av64[p];
+        for (int p = p0 + L.gl; p < p1; p += L.G) {
+            d.av32[p] = (float)d.av64[p];
+            if (d.cix) d.cix[p] = pack_word(d.ci[p], d.av64[p]);
+        }
         if (L.gl != 0) continue;
         const double rs = d.row_scale[i];
         const double lo = d.orow_lo[i], hi = d.orow_hi[i];
@@ -1006,7 +1052,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_cols(Dev d) {
         const int j = base + (L.lane >> L.lg);
         if (j >= tl.r1) continue;
         const int p0 = d.cp[j], p1 = d.cp[j + 1];
-        for (int p = p0 + L.gl; p < p1; p += L.G) d.tv32[p] = (float)d.tv64[p];
+        for (int p = p0 + L.gl; p < p1; p += L.G) {
+            d.tv32[p] = (float)d.tv64[p];
+            if (d.rix) d.rix[p] = pack_word(d.ri[p], d.tv64[p]);
+        }
         if (L.gl != 0) continue;
         const double cs = d.col_scale[j];
         const double inv = 1.0 / cs;
@@ -1054,7 +1103,7 @@ __global__ void k_init_iterates(Dev d, const double* warm_x, const double* warm_
     }
 }
 
-__global__ void k_setup_state(Dev d, double initial_weight, double step_reduction, int mixed) {
+__global__ void k_setup_state(Dev d, double initial_weight, double step_reduction, int start_mode) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.B) return;
     InstState& s = d.st[b];
@@ -1067,8 +1116,9 @@ __global__ void k_setup_state(Dev d, double initial_weight, double step_reductio
     s.restart_iter = 0;
     s.done_iter = 0;
     s.fp32_iters = 0;
+    s.packed_end = 0;
     s.status = kRunning;
-    s.mode = mixed ? kModeFp32 : kModeFp64;
+    s.mode = start_mode;
     s.mode_at_check = s.mode;
     s.restart_flag = 0;
     s.switch_flag = 0;
@@ -1099,9 +1149,9 @@ __global__ void k_fill(double* p, double v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-template <typename T>
+template <typename T, typename TA = T, typename IX = int32_t>
 __global__ __launch_bounds__(kBlock) void k_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr,
-                                                       const int32_t* idx, const T* val, const T* vec, T* out) {
+                                                       const IX* idx, const TA* val, const T* vec, T* out) {
     const int tile = wave_tile(ntiles);
     if (tile >= ntiles) return;
     const Tile tl = tiles[tile];
@@ -1241,39 +1291,40 @@ void launch_adapt_finalize(const Dev& d, bool exact, hipStream_t s) {
         hipLaunchKernelGGL((k_adapt_finalize<false>), g, b, 0, s, d);
 }
 
-template <bool CUR>
+template <int CAND>
 static void resid_pair(const Dev& d, bool exact, hipStream_t s) {
     const dim3 b(kBlock);
     if (d.ntA > 0) {
         const dim3 g(tile_grid(d.ntA));
         if (exact)
-            hipLaunchKernelGGL((k_resid_rows<true, CUR>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_rows<true, CAND>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_rows<false, CUR>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_rows<false, CAND>), g, b, 0, s, d);
     }
     if (d.ntT > 0) {
         const dim3 g(tile_grid(d.ntT));
         if (exact)
-            hipLaunchKernelGGL((k_resid_cols<true, CUR>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols<true, CAND>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_cols<false, CUR>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols<false, CAND>), g, b, 0, s, d);
     }
 }
 
 void launch_resid(const Dev& d, bool exact, bool current, hipStream_t s) {
     if (current)
-        resid_pair<true>(d, exact, s);
+        resid_pair<1>(d, exact, s);
     else
-        resid_pair<false>(d, exact, s);
+        resid_pair<0>(d, exact, s);
 }
 
-void launch_check_resid(const Dev& d, bool exact, bool both, hipStream_t s) {
+void launch_check_resid(const Dev& d, bool exact, bool both, bool packed_floor, hipStream_t s) {
     if (!exact && d.blocks) {
-        launch_resid_blk(d, both, s);
+        launch_resid_blk(d, both, packed_floor, s);
         return;
     }
     launch_resid(d, exact, false, s);
     if (both) launch_resid(d, exact, true, s);
+    if (packed_floor && !exact) resid_pair<2>(d, false, s);
 }
 
 void launch_check_finalize(const Dev& d, const IterOpts& o, int t, bool exact, bool final_pass,
@@ -1333,10 +1384,10 @@ void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm
     hipLaunchKernelGGL(k_init_iterates, dim3(elem_grid(tot)), dim3(kBlock), 0, s, d, warm_x, warm_y);
 }
 
-void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, bool mixed,
+void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, int start_mode,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_setup_state, dim3(elem_grid(d.B)), dim3(kBlock), 0, s, d, initial_weight,
-                       step_reduction, mixed ? 1 : 0);
+                       step_reduction, start_mode);
 }
 
 void launch_step_cap(const Dev& d, int64_t iters, bool exact, hipStream_t s) {
@@ -1396,6 +1447,13 @@ void launch_spmv_plain(const Tile* tiles, int ntiles, const int32_t* ptr, const 
     else
         hipLaunchKernelGGL((k_spmv_plain<double>), g, b, 0, s, tiles, ntiles, ptr, idx, (const double*)val,
                            (const double*)vec, (double*)out);
+}
+
+void launch_spmv_plain_packed(const Tile* tiles, int ntiles, const int32_t* ptr, const uint32_t* idxp,
+                              const float* val32, const double* vec, double* out, hipStream_t s) {
+    if (ntiles == 0) return;
+    hipLaunchKernelGGL((k_spmv_plain<double, float, uint32_t>), dim3(tile_grid(ntiles)), dim3(kBlock), 0, s, tiles,
+                       ntiles, ptr, idxp, val32, vec, out);
 }
 
 }  // namespace mcp

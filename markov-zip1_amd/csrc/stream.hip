@@ -41,9 +41,9 @@ __device__ __forceinline__ int block_tile_id(int ntiles) {
 // matrix (see Dev::nt_rows).
 // NT is a template parameter: the compiler merges a run-time choice between
 // the two load forms into one plain load.
-template <bool NT, typename TA>
-__device__ __forceinline__ void stream_pair(const int32_t* __restrict__ idx, const TA* __restrict__ val, int p,
-                                            int& i, TA& a) {
+template <bool NT, typename TA, typename IX>
+__device__ __forceinline__ void stream_pair(const IX* __restrict__ idx, const TA* __restrict__ val, int p, IX& i,
+                                            TA& a) {
     if (NT) {
         i = __builtin_nontemporal_load(idx + p);
         a = __builtin_nontemporal_load(val + p);
@@ -55,11 +55,12 @@ __device__ __forceinline__ void stream_pair(const int32_t* __restrict__ idx, con
 
 // Products val[p] * vec[idx[p]] of the nonzeros [p0, p0 + len) into prod[0, len),
 // len <= K * kBT.  All K index/value loads are issued, then all K gathers.
-template <int K, typename TS, typename TA, typename TV, bool NT = false>
-__device__ __forceinline__ void stage_products(int p0, int len, const int32_t* __restrict__ idx,
+// IX = uint32_t: the packed stream (index word with the value correction).
+template <int K, typename TS, typename TA, typename TV, bool NT = false, typename IX>
+__device__ __forceinline__ void stage_products(int p0, int len, const IX* __restrict__ idx,
                                                const TA* __restrict__ val, const TV* __restrict__ vec,
                                                TS* prod) {
-    int ii[K];
+    IX ii[K];
     TA aa[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -72,12 +73,12 @@ __device__ __forceinline__ void stage_products(int p0, int len, const int32_t* _
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int q = (int)threadIdx.x + k * kBT;
-        vv[k] = q < len ? vec[ii[k]] : (TV)0;
+        vv[k] = q < len ? vec[ix_of(ii[k])] : (TV)0;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int q = (int)threadIdx.x + k * kBT;
-        if (q < len) prod[q] = (TS)aa[k] * (TS)vv[k];
+        if (q < len) prod[q] = val_of<TS>(aa[k], ii[k]) * (TS)vv[k];
     }
 }
 
@@ -95,51 +96,71 @@ __device__ __forceinline__ TS group_sum(const TS* prod, int q0, int q1, int gl, 
 
 // Dot product of one long row [p0, p1) by the whole workgroup (four
 // accumulators per thread, fixed-order block reduction); valid in thread 0.
-template <typename TS, bool NT = false, typename TA, typename TV>
-__device__ __forceinline__ double long_row_dot(int p0, int p1, const int32_t* __restrict__ idx,
+// RP: multiply by packed_of(value) (the packed stream's value of an FP64 entry).
+template <typename TS, bool NT = false, bool RP = false, typename TA, typename TV, typename IX>
+__device__ __forceinline__ double long_row_dot(int p0, int p1, const IX* __restrict__ idx,
                                                const TA* __restrict__ val, const TV* __restrict__ vec,
                                                double* red) {
     TS s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int p = p0 + (int)threadIdx.x;
     for (; p + 3 * kBT < p1; p += 4 * kBT) {
-        int i0, i1, i2, i3;
+        IX i0, i1, i2, i3;
         TA a0, a1, a2, a3;
         stream_pair<NT>(idx, val, p, i0, a0);
         stream_pair<NT>(idx, val, p + kBT, i1, a1);
         stream_pair<NT>(idx, val, p + 2 * kBT, i2, a2);
         stream_pair<NT>(idx, val, p + 3 * kBT, i3, a3);
-        s0 += (TS)a0 * (TS)vec[i0];
-        s1 += (TS)a1 * (TS)vec[i1];
-        s2 += (TS)a2 * (TS)vec[i2];
-        s3 += (TS)a3 * (TS)vec[i3];
+        s0 += (RP ? (TS)packed_of((double)a0) : val_of<TS>(a0, i0)) * (TS)vec[ix_of(i0)];
+        s1 += (RP ? (TS)packed_of((double)a1) : val_of<TS>(a1, i1)) * (TS)vec[ix_of(i1)];
+        s2 += (RP ? (TS)packed_of((double)a2) : val_of<TS>(a2, i2)) * (TS)vec[ix_of(i2)];
+        s3 += (RP ? (TS)packed_of((double)a3) : val_of<TS>(a3, i3)) * (TS)vec[ix_of(i3)];
     }
     for (; p < p1; p += kBT) {
-        int i;
+        IX i;
         TA a;
         stream_pair<NT>(idx, val, p, i, a);
-        s0 += (TS)a * (TS)vec[i];
+        s0 += (RP ? (TS)packed_of((double)a) : val_of<TS>(a, i)) * (TS)vec[ix_of(i)];
     }
     return block_sum<kBT / 64>((double)((s0 + s1) + (s2 + s3)), red);
 }
 
-// Row sum of a block tile in the instance's precision: FP32 values and
-// gathers with FP32 products and sums, or FP64.  Every thread must call.
-// NT64: the FP64 matrix stream uses non-temporal loads (FP32 streams never do:
-// the pair of FP32 matrices is small enough to share the cache).
+// Row sum of a block tile in the instance's mode: FP32 values and gathers with
+// FP32 products and sums (kModeFp32), the packed stream (index words with
+// the value correction, FP32 values) with FP64 gathers, products and sums
+// (kModePacked), or FP64.  Every thread must call.
+// NT64: the FP64 matrix stream uses non-temporal loads (FP32 and packed
+// streams never do: the pair of 8-byte-per-nonzero matrices is small enough
+// to share the cache).
 template <int NB, bool NT64 = false, typename TA32, typename TV32>
-__device__ __forceinline__ double tile_row_sum(bool f32, int p0, int len, int q0, int q1, int gl, int G,
-                                               const int32_t* __restrict__ idx, const double* __restrict__ val64,
-                                               const double* __restrict__ vec64, const TA32* __restrict__ val32,
-                                               const TV32* __restrict__ vec32, double* prod) {
-    if (f32) {
+__device__ __forceinline__ double tile_row_sum(int mode, int p0, int len, int q0, int q1, int gl, int G,
+                                               const int32_t* __restrict__ idx, const uint32_t* __restrict__ idxp,
+                                               const double* __restrict__ val64, const double* __restrict__ vec64,
+                                               const TA32* __restrict__ val32, const TV32* __restrict__ vec32,
+                                               double* prod) {
+    if (mode == kModeFp32) {
         float* pr = reinterpret_cast<float*>(prod);
         stage_products<NB / kBT, float>(p0, len, idx, val32, vec32, pr);
         __syncthreads();
         return (double)group_sum<float>(pr, q0, q1, gl, G);
     }
-    stage_products<NB / kBT, double, double, double, NT64>(p0, len, idx, val64, vec64, prod);
+    if (mode == kModePacked)
+        stage_products<NB / kBT, double>(p0, len, idxp, val32, vec64, prod);
+    else
+        stage_products<NB / kBT, double, double, double, NT64>(p0, len, idx, val64, vec64, prod);
     __syncthreads();
     return group_sum<double>(prod, q0, q1, gl, G);
+}
+
+// Dot product of one long row in the instance's mode (see tile_row_sum);
+// valid in thread 0.
+template <bool NT64 = false, typename TA32, typename TV32>
+__device__ __forceinline__ double long_row_mode(int mode, int p0, int p1, const int32_t* __restrict__ idx,
+                                                const uint32_t* __restrict__ idxp, const double* __restrict__ val64,
+                                                const double* __restrict__ vec64, const TA32* __restrict__ val32,
+                                                const TV32* __restrict__ vec32, double* red) {
+    if (mode == kModeFp32) return long_row_dot<float>(p0, p1, idx, val32, vec32, red);
+    if (mode == kModePacked) return long_row_dot<double>(p0, p1, idxp, val32, vec64, red);
+    return long_row_dot<double, NT64>(p0, p1, idx, val64, vec64, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -207,8 +228,7 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
     };
     if (tl.shape & 8) {  // one long column (r1 = r0 + 1), strided by the whole workgroup
         const int j = tl.r0;
-        const double aty = f32 ? long_row_dot<float>(p0, p1, d.ri, d.tv32, d.y32, red)
-                               : long_row_dot<double>(p0, p1, d.ri, d.tv64, d.y, red);
+        const double aty = long_row_mode(mode, p0, p1, d.ri, d.rix, d.tv64, d.y, d.tv32, d.y32, red);
         if (tid == 0 && run) {
             load(j);
             update(j, aty);
@@ -226,7 +246,7 @@ __global__ __launch_bounds__(kBT) void k_primal_blk(Dev d, int t) {
         q1 = d.cp[j + 1] - p0;
         if (gl == 0) load(j);
     }
-    const double aty = tile_row_sum<NB>(f32, p0, len, q0, q1, gl, G, d.ri, d.tv64, d.y, d.tv32, d.y32, prod);
+    const double aty = tile_row_sum<NB>(mode, p0, len, q0, q1, gl, G, d.ri, d.rix, d.tv64, d.y, d.tv32, d.y32, prod);
     if (own && gl == 0) update(j, aty);
     if (ADAPT) {
         part = block_sum<kBT / 64>(part, red);
@@ -287,8 +307,7 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
     };
     if (tl.shape & 8) {  // one long row (r1 = r0 + 1)
         const int i = tl.r0;
-        const double ax = f32 ? long_row_dot<float>(p0, p1, d.ci, d.av32, d.xbar32, red)
-                              : long_row_dot<double, NT>(p0, p1, d.ci, d.av64, d.xbar, red);
+        const double ax = long_row_mode<NT>(mode, p0, p1, d.ci, d.cix, d.av64, d.xbar, d.av32, d.xbar32, red);
         if (tid == 0 && run) {
             load(i);
             update(i, ax);
@@ -305,7 +324,7 @@ __global__ __launch_bounds__(kBT) void k_dual_blk(Dev d, int t) {
         q1 = d.rp[i + 1] - p0;
         if (gl == 0) load(i);
     }
-    const double ax = tile_row_sum<NB, NT>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.xbar, d.av32, d.xbar32, prod);
+    const double ax = tile_row_sum<NB, NT>(mode, p0, len, q0, q1, gl, G, d.ci, d.cix, d.av64, d.xbar, d.av32, d.xbar32, prod);
     if (own && gl == 0) update(i, ax);
 }
 
@@ -326,13 +345,11 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
     const int32_t status = s.status, mode = s.mode;
     const int p0 = ptr[tl.r0], pe = ptr[tl.r1];  // both loaded whatever the status
     const bool run = status == kRunning;
-    const bool f32 = mode == kModeFp32;
     const int p1 = run ? pe : p0;
     const int tid = threadIdx.x;
     if (tl.shape & 8) {  // one long row (r1 = r0 + 1)
         const int i = tl.r0;
-        const double a = f32 ? long_row_dot<float>(p0, p1, d.ci, d.av32, d.dx32, red)
-                             : long_row_dot<double, NT>(p0, p1, d.ci, d.av64, d.dx, red);
+        const double a = long_row_mode<NT>(mode, p0, p1, d.ci, d.cix, d.av64, d.dx, d.av32, d.dx32, red);
         if (tid == 0 && run) d.part_adx[tile] = (a * a) / d.rnorm[i];
         return;
     }
@@ -347,7 +364,7 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
         q1 = d.rp[i + 1] - p0;
         if (gl == 0) rn = d.rnorm[i];
     }
-    const double a = tile_row_sum<NB, NT>(f32, p0, len, q0, q1, gl, G, d.ci, d.av64, d.dx, d.av32, d.dx32, prod);
+    const double a = tile_row_sum<NB, NT>(mode, p0, len, q0, q1, gl, G, d.ci, d.cix, d.av64, d.dx, d.av32, d.dx32, prod);
     const double part = block_sum<kBT / 64>((own && gl == 0) ? (a * a) / rn : 0.0, red);
     if (tid == 0 && run) d.part_adx[tile] = part;
 }
@@ -360,8 +377,10 @@ __global__ __launch_bounds__(kBT) void k_adx_blk(Dev d, int) {
 // to the same field-major slots as the wave-task check kernels of kernels.hip,
 // [(c * fields + k) * tiles + tile], with tiles = nbA / nbT.
 
-// Products of the tile's nonzeros with NC gathered vectors into prod[c * NB + q].
-template <int K, int NC, bool NT = false>
+// Products of the tile's nonzeros with NC gathered vectors into prod[c * NB + q];
+// with PK also the packed stream's values times the first vector into
+// prod[NC * NB + q] (the packed floor of the check).
+template <int K, int NC, bool NT = false, bool PK = false>
 __device__ __forceinline__ void stage_products_multi(int p0, int len, const int32_t* __restrict__ idx,
                                                      const double* __restrict__ val, const double* const* vecs,
                                                      double* prod) {
@@ -390,12 +409,22 @@ __device__ __forceinline__ void stage_products_multi(int p0, int len, const int3
             const int q = (int)threadIdx.x + k * kBT;
             if (q < len) prod[c * NB + q] = aa[k] * vv[c][k];
         }
+    if (PK) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int q = (int)threadIdx.x + k * kBT;
+            if (q < len) prod[NC * NB + q] = packed_of(aa[k]) * vv[0][k];
+        }
+    }
 }
 
 // Rows: A x_c, row violation, dual-objective row part, |y_c - y_last|^2.
-template <int NB, int NC, bool NT>
+// PK: a further candidate, the average with the packed stream's values, in
+// slot 2 (the packed floor).
+template <int NB, int NC, bool NT, bool PK>
 __global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
-    __shared__ double prod[NC * NB];
+    constexpr int NP = NC + (PK ? 1 : 0);  // products staged per nonzero
+    __shared__ double prod[NP * NB];
     __shared__ double red[(kBT / 64) * 3];
     const int tile = block_tile_id(d.nbA);
     if (tile >= d.nbA) return;
@@ -436,16 +465,17 @@ __global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
     };
     if (tl.shape & 8) {
         const int i = tl.r0;
-        double ax[NC];
+        double ax[NP];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
             ax[c] = long_row_dot<double, NT>(d.rp[i], d.rp[i + 1], d.ci, d.av64, xs[c], red);
+        if (PK) ax[NP - 1] = long_row_dot<double, NT, true>(d.rp[i], d.rp[i + 1], d.ci, d.av64, xs[0], red);
         if (tid == 0) {
             load(i);
-            for (int c = 0; c < NC; ++c) {
+            for (int c = 0; c < NP; ++c) {
                 double v, ob, mv;
-                terms(c, ax[c], v, ob, mv);
-                store(c, v, ob, mv);
+                terms(c < NC ? c : 0, ax[c], v, ob, mv);
+                store(c < NC ? c : 2, v, ob, mv);
             }
         }
         return;
@@ -460,23 +490,24 @@ __global__ __launch_bounds__(kBT) void k_resid_rows_blk(Dev d) {
         q1 = d.rp[i + 1] - p0;
         if (gl == 0) load(i);
     }
-    stage_products_multi<NB / kBT, NC, NT>(p0, len, d.ci, d.av64, xs, prod);
+    stage_products_multi<NB / kBT, NC, NT, PK>(p0, len, d.ci, d.av64, xs, prod);
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
+    for (int c = 0; c < NP; ++c) {
         const double ax = group_sum<double>(prod + c * NB, q0, q1, gl, G);
         double v = 0.0, ob[2] = {0.0, 0.0};
-        if (own && gl == 0) terms(c, ax, v, ob[0], ob[1]);
+        if (own && gl == 0) terms(c < NC ? c : 0, ax, v, ob[0], ob[1]);
         block_max_sums<kBT / 64, 2>(v, ob, red);
-        if (tid == 0) store(c, v, ob[0], ob[1]);
+        if (tid == 0) store(c < NC ? c : 2, v, ob[0], ob[1]);
     }
 }
 
 // Columns: A^T y_c, bound violation, dual residual, primal/dual objective
 // parts, |x_c - x_last|^2.
-template <int NB, int NC>
+template <int NB, int NC, bool PK>
 __global__ __launch_bounds__(kBT) void k_resid_cols_blk(Dev d) {
-    __shared__ double prod[NC * NB];
+    constexpr int NP = NC + (PK ? 1 : 0);  // products staged per nonzero (slot 2: packed floor)
+    __shared__ double prod[NP * NB];
     __shared__ double red[(kBT / 64) * 5];
     const int tile = block_tile_id(d.nbT);
     if (tile >= d.nbT) return;
@@ -525,15 +556,16 @@ __global__ __launch_bounds__(kBT) void k_resid_cols_blk(Dev d) {
     };
     if (tl.shape & 8) {
         const int j = tl.r0;
-        double aty[NC];
+        double aty[NP];
 #pragma unroll
         for (int c = 0; c < NC; ++c) aty[c] = long_row_dot<double>(d.cp[j], d.cp[j + 1], d.ri, d.tv64, ys[c], red);
+        if (PK) aty[NP - 1] = long_row_dot<double, false, true>(d.cp[j], d.cp[j + 1], d.ri, d.tv64, ys[0], red);
         if (tid == 0) {
             load(j);
-            for (int c = 0; c < NC; ++c) {
+            for (int c = 0; c < NP; ++c) {
                 double v, t[4];
-                terms(c, aty[c], v, t);
-                store(c, v, t);
+                terms(c < NC ? c : 0, aty[c], v, t);
+                store(c < NC ? c : 2, v, t);
             }
         }
         return;
@@ -548,23 +580,25 @@ __global__ __launch_bounds__(kBT) void k_resid_cols_blk(Dev d) {
         q1 = d.cp[j + 1] - p0;
         if (gl == 0) load(j);
     }
-    stage_products_multi<NB / kBT, NC>(p0, len, d.ri, d.tv64, ys, prod);
+    stage_products_multi<NB / kBT, NC, false, PK>(p0, len, d.ri, d.tv64, ys, prod);
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
+    for (int c = 0; c < NP; ++c) {
         const double aty = group_sum<double>(prod + c * NB, q0, q1, gl, G);
         double v = 0.0, t[4] = {0.0, 0.0, 0.0, 0.0};
-        if (own && gl == 0) terms(c, aty, v, t);
+        if (own && gl == 0) terms(c < NC ? c : 0, aty, v, t);
         block_max_sums<kBT / 64, 4>(v, t, red);
-        if (tid == 0) store(c, v, t);
+        if (tid == 0) store(c < NC ? c : 2, v, t);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Plain SpMV over block tiles: out[r] = sum val * vec (tests, autotuning).
-template <typename T, int NB>
+// T: the type of the vectors, products and sums; TA/IX: the stored values and
+// index words (T/int32, or float/uint32 for the packed stream with T double).
+template <typename T, int NB, typename TA = T, typename IX = int32_t>
 __global__ __launch_bounds__(kBT) void k_spmv_blk(const Tile* tiles, int ntiles, const int32_t* __restrict__ ptr,
-                                                  const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                                  const IX* __restrict__ idx, const TA* __restrict__ val,
                                                   const T* __restrict__ vec, T* __restrict__ out) {
     __shared__ T prod[NB];
     __shared__ double red[kBT / 64];
@@ -744,60 +778,64 @@ void launch_adx_blk(const Dev& d, int t, hipStream_t s) {
         adx_blk<false>(d, t, s);
 }
 
-template <int NC, bool NT>
+template <int NC, bool NT, bool PK>
 static void resid_rows_blk(const Dev& d, hipStream_t s) {
     const dim3 g(block_grid(d.nbA)), b(kBT);
     if (d.budA == 256)
-        hipLaunchKernelGGL((k_resid_rows_blk<256, NC, NT>), g, b, 0, s, d);
+        hipLaunchKernelGGL((k_resid_rows_blk<256, NC, NT, PK>), g, b, 0, s, d);
     else if (d.budA == 512)
-        hipLaunchKernelGGL((k_resid_rows_blk<512, NC, NT>), g, b, 0, s, d);
+        hipLaunchKernelGGL((k_resid_rows_blk<512, NC, NT, PK>), g, b, 0, s, d);
     else if (d.budA == 2048)
-        hipLaunchKernelGGL((k_resid_rows_blk<2048, NC, NT>), g, b, 0, s, d);
+        hipLaunchKernelGGL((k_resid_rows_blk<2048, NC, NT, PK>), g, b, 0, s, d);
     else
-        hipLaunchKernelGGL((k_resid_rows_blk<1024, NC, NT>), g, b, 0, s, d);
+        hipLaunchKernelGGL((k_resid_rows_blk<1024, NC, NT, PK>), g, b, 0, s, d);
 }
 
-template <int NC>
+template <int NC, bool PK>
 static void resid_blk(const Dev& d, hipStream_t s) {
     const dim3 b(kBT);
     if (d.nbA > 0) {
         if (d.nt_rows)
-            resid_rows_blk<NC, true>(d, s);
+            resid_rows_blk<NC, true, PK>(d, s);
         else
-            resid_rows_blk<NC, false>(d, s);
+            resid_rows_blk<NC, false, PK>(d, s);
     }
     if (d.nbT > 0) {
         const dim3 g(block_grid(d.nbT));
         if (d.budT == 256)
-            hipLaunchKernelGGL((k_resid_cols_blk<256, NC>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols_blk<256, NC, PK>), g, b, 0, s, d);
         else if (d.budT == 512)
-            hipLaunchKernelGGL((k_resid_cols_blk<512, NC>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols_blk<512, NC, PK>), g, b, 0, s, d);
         else if (d.budT == 2048)
-            hipLaunchKernelGGL((k_resid_cols_blk<2048, NC>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols_blk<2048, NC, PK>), g, b, 0, s, d);
         else
-            hipLaunchKernelGGL((k_resid_cols_blk<1024, NC>), g, b, 0, s, d);
+            hipLaunchKernelGGL((k_resid_cols_blk<1024, NC, PK>), g, b, 0, s, d);
     }
 }
 
-void launch_resid_blk(const Dev& d, bool both, hipStream_t s) {
-    if (both)
-        resid_blk<2>(d, s);
+void launch_resid_blk(const Dev& d, bool both, bool packed_floor, hipStream_t s) {
+    if (both && packed_floor)
+        resid_blk<2, true>(d, s);
+    else if (both)
+        resid_blk<2, false>(d, s);
+    else if (packed_floor)
+        resid_blk<1, true>(d, s);
     else
-        resid_blk<1>(d, s);
+        resid_blk<1, false>(d, s);
 }
 
-template <typename T>
-static void spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
-                     const T* val, const T* vec, T* out, hipStream_t s) {
+template <typename T, typename TA = T, typename IX = int32_t>
+static void spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const IX* idx,
+                     const TA* val, const T* vec, T* out, hipStream_t s) {
     const dim3 g(block_grid(ntiles)), b(kBT);
     if (budget == 256)
-        hipLaunchKernelGGL((k_spmv_blk<T, 256>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+        hipLaunchKernelGGL((k_spmv_blk<T, 256, TA, IX>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
     else if (budget == 512)
-        hipLaunchKernelGGL((k_spmv_blk<T, 512>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+        hipLaunchKernelGGL((k_spmv_blk<T, 512, TA, IX>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
     else if (budget == 2048)
-        hipLaunchKernelGGL((k_spmv_blk<T, 2048>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+        hipLaunchKernelGGL((k_spmv_blk<T, 2048, TA, IX>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
     else
-        hipLaunchKernelGGL((k_spmv_blk<T, 1024>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
+        hipLaunchKernelGGL((k_spmv_blk<T, 1024, TA, IX>), g, b, 0, s, tiles, ntiles, ptr, idx, val, vec, out);
 }
 
 void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const int32_t* idx,
@@ -807,6 +845,12 @@ void launch_spmv_blk(const Tile* tiles, int ntiles, int budget, const int32_t* p
         spmv_blk<float>(tiles, ntiles, budget, ptr, idx, (const float*)val, (const float*)vec, (float*)out, s);
     else
         spmv_blk<double>(tiles, ntiles, budget, ptr, idx, (const double*)val, (const double*)vec, (double*)out, s);
+}
+
+void launch_spmv_blk_packed(const Tile* tiles, int ntiles, int budget, const int32_t* ptr, const uint32_t* idxp,
+                            const float* val32, const double* vec, double* out, hipStream_t s) {
+    if (ntiles == 0) return;
+    spmv_blk<double, float, uint32_t>(tiles, ntiles, budget, ptr, idxp, val32, vec, out, s);
 }
 
 }  // namespace mcp

@@ -2,7 +2,7 @@
 //
 //   markov-cero-solve MODEL.(mps|mclp) [--engine pdlp] [--backend cpu|gpu]
 //                     [--tolerance T] [--iteration-limit N] [--output f.json]
-//                     [--precision fp64|mixed] [--order fast|reference] [--autotune]
+//                     [--precision fp64|mixed|packed] [--order fast|reference] [--autotune]
 //
 // Mirrors the reference's pdlp branch (apps/markov_cero_solve.cpp:136-189):
 // --backend/--tolerance/--iteration-limit map into PdlpOptions (an iteration
@@ -62,7 +62,7 @@ std::string esc(const std::string& s) {
 int usage() {
     std::cerr << "usage: markov-cero-solve MODEL.mps|MODEL.mclp [--engine pdlp] [--backend cpu|gpu]\n"
                  "       [--tolerance T] [--iteration-limit N] [--output FILE]\n"
-                 "       [--precision fp64|mixed] [--order fast|reference] [--autotune]\n"
+                 "       [--precision fp64|mixed|packed] [--order fast|reference] [--autotune]\n"
                  "       [--restart-scheme reference|kkt|halpern]\n";
     return 4;
 }
@@ -130,7 +130,9 @@ int main(int argc, char** argv) {
         o.backend = backend == "gpu" ? fo::Backend::gpu : fo::Backend::cpu;
         o.max_iterations = (iteration_limit != 10000 && iteration_limit > 0) ? iteration_limit : 100000;
         o.set_tolerance(tol);
-        o.precision = precision == "mixed" ? fo::Precision::mixed : fo::Precision::fp64;
+        o.precision = precision == "mixed"        ? fo::Precision::mixed
+                      : precision == "packed" ? fo::Precision::packed
+                                                  : fo::Precision::fp64;
         o.reduction_order = order == "reference" ? fo::ReductionOrder::reference : fo::ReductionOrder::fast;
         o.autotune = autotune;
         o.restart_scheme = scheme == "kkt"       ? fo::RestartScheme::kkt
@@ -183,7 +185,8 @@ int main(int argc, char** argv) {
        << "\",\"backend_actually_used\":\"" << esc(res.backend_actually_used) << "\",\"h2d_ms\":" << num(res.h2d_ms)
        << ",\"kernel_ms\":" << num(res.kernel_ms) << ",\"d2h_ms\":" << num(res.d2h_ms)
        << ",\"total_ms\":" << num(res.total_ms) << ",\"precision\":\"" << precision
-       << "\",\"fp32_iterations\":" << res.fp32_iterations << ",\"setup_ms\":" << num(res.setup_ms)
+       << "\",\"fp32_iterations\":" << res.fp32_iterations << ",\"packed_iterations\":" << res.packed_iterations
+       << ",\"setup_ms\":" << num(res.setup_ms)
        << ",\"restart_scheme\":\"" << scheme << "\",\"dual\":[";
     // extension keys: the dual vector (reference sign convention, g = c + A^T y) and setup time
     for (std::size_t i = 0; i < res.dual.size(); ++i) js << (i ? "," : "") << num(res.dual[i]);

@@ -55,7 +55,9 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
     p.ruiz_scaling = o.ruiz_scaling != 0;
     p.ruiz_iterations = o.ruiz_iterations;
     p.autotune = o.autotune != 0;
-    p.precision = o.precision == MC_PRECISION_MIXED ? fo::Precision::mixed : fo::Precision::fp64;
+    p.precision = o.precision == MC_PRECISION_MIXED        ? fo::Precision::mixed
+                  : o.precision == MC_PRECISION_PACKED ? fo::Precision::packed
+                                                           : fo::Precision::fp64;
     p.reduction_order = o.reduction_order == MC_ORDER_REFERENCE ? fo::ReductionOrder::reference
                                                                 : fo::ReductionOrder::fast;
     p.restart_scheme = o.restart_scheme == MC_SCHEME_KKT       ? fo::RestartScheme::kkt
@@ -94,6 +96,7 @@ void to_result(const fo::PdlpResult& r, mc_pdlp_result* out) {
     if (out->primal && !r.primal.empty()) std::memcpy(out->primal, r.primal.data(), sizeof(double) * r.primal.size());
     if (out->dual && !r.dual.empty()) std::memcpy(out->dual, r.dual.data(), sizeof(double) * r.dual.size());
     out->fp32_iterations = r.fp32_iterations;
+    out->packed_iterations = r.packed_iterations;
     out->setup_ms = r.setup_ms;
     out->primal_weight = r.primal_weight;
     out->step_size = r.step_size;

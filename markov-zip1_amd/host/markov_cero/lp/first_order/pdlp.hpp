@@ -16,7 +16,7 @@ namespace markov_cero::lp::first_order {
 
 enum class Backend { cpu, gpu };
 enum class RestartStrategy { none, adaptive, fixed };
-enum class Precision { fp64, mixed };               // extension
+enum class Precision { fp64, mixed, packed };   // extension
 enum class ReductionOrder { fast, reference };      // extension
 // extension: `reference` keeps the reference's restart rule (pdlp.cpp:246-260);
 // `kkt` uses the adaptive restarts of Applegate et al. (2021): candidate = the
@@ -90,10 +90,11 @@ struct PdlpResult {
     // no usable device).  Never claims a GPU that did not run.
     std::string backend_actually_used{"cpu"};
     // --- extensions ---
-    std::size_t fp32_iterations{0};
+    std::size_t fp32_iterations{0};    // iterations with FP32 SpMVs (mixed)
     double setup_ms{0.0};
     double primal_weight{0.0};  // final omega (scaled space): seeds initial_primal_weight of a warm start
     double step_size{0.0};      // final eta
+    std::size_t packed_iterations{0};  // iterations on the packed stream (packed, or mixed after FP32)
 };
 
 // Solve an LP with matrix-free PDHG (only A*x and A^T*y, no factorization).

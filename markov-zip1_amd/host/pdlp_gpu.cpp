@@ -358,7 +358,9 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.adaptive_step_size = opt.adaptive_step_size ? 1 : 0;
     o.restart_reduction_factor = opt.restart_reduction_factor;
     o.adaptive_primal_weight = opt.adaptive_primal_weight ? 1 : 0;
-    o.precision = opt.precision == Precision::mixed ? MC_PRECISION_MIXED : MC_PRECISION_FP64;
+    o.precision = opt.precision == Precision::mixed        ? MC_PRECISION_MIXED
+                  : opt.precision == Precision::packed ? MC_PRECISION_PACKED
+                                                           : MC_PRECISION_FP64;
     o.initial_primal_weight = opt.initial_primal_weight;
     o.primal_weight_smoothing = opt.primal_weight_smoothing;
     o.polish_threshold = opt.polish_threshold;
@@ -414,6 +416,7 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
         r.duality_gap = q.duality_gap;
         r.iterations = (std::size_t)q.iterations;
         r.fp32_iterations = (std::size_t)q.fp32_iterations;
+        r.packed_iterations = (std::size_t)q.packed_iterations;
         r.primal_weight = q.omega;
         r.step_size = q.eta;
         r.h2d_ms = st.h2d_ms;

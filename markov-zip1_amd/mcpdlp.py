@@ -25,7 +25,7 @@ NEG_INF, FINITE, POS_INF = 0, 1, 2
 MINIMIZE, MAXIMIZE = 0, 1
 BACKEND_CPU, BACKEND_GPU = 0, 1
 RESTART_NONE, RESTART_ADAPTIVE, RESTART_FIXED = 0, 1, 2
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_PACKED = 0, 1, 2
 ORDER_FAST, ORDER_REFERENCE = 0, 1
 SCHEME_REFERENCE, SCHEME_KKT, SCHEME_HALPERN = 0, 1, 2
 STATUS = {0: "optimal", 1: "iteration_limit", 2: "infeasible_or_unbounded", 3: "numerical_failure"}
@@ -82,6 +82,7 @@ class Result(C.Structure):
         ("message", C.c_char * 128), ("backend_actually_used", C.c_char * 32),
         ("fp32_iterations", C.c_uint64), ("setup_ms", C.c_double),
         ("primal_weight", C.c_double), ("step_size", C.c_double),
+        ("packed_iterations", C.c_uint64),
     ]
 
 
@@ -150,6 +151,7 @@ class EngInstResult(C.Structure):
         ("status", C.c_int32), ("mode", C.c_int32), ("iterations", C.c_uint64), ("fp32_iterations", C.c_uint64),
         ("objective", C.c_double), ("primal_infeasibility", C.c_double), ("dual_infeasibility", C.c_double),
         ("duality_gap", C.c_double), ("omega", C.c_double), ("eta", C.c_double),
+        ("packed_iterations", C.c_uint64),
     ]
 
 
@@ -426,7 +428,8 @@ def _result_dict(r: Result, prim, dual) -> dict:
         "duality_gap": r.duality_gap, "tolerance": r.tolerance, "iterations": int(r.iterations),
         "h2d_ms": r.h2d_ms, "kernel_ms": r.kernel_ms, "d2h_ms": r.d2h_ms, "total_ms": r.total_ms,
         "message": r.message.decode(), "backend_actually_used": r.backend_actually_used.decode(),
-        "fp32_iterations": int(r.fp32_iterations), "setup_ms": r.setup_ms,
+        "fp32_iterations": int(r.fp32_iterations), "packed_iterations": int(r.packed_iterations),
+        "setup_ms": r.setup_ms,
         "primal_weight": r.primal_weight, "step_size": r.step_size,
     }
 
@@ -609,11 +612,14 @@ class Engine:
         s = EngSetup(int(ruiz), int(ruiz_iterations), int(exact), int(autotune))
         _check_eng(self.lib.mcp_setup(self.h, C.byref(s)))
 
-    def spmv(self, v, transpose=False, fp32=False, exact=False, reps=1):
+    def spmv(self, v, transpose=False, fp32=False, exact=False, reps=1, packed=False):
+        """A v (or A^T v) of the scaled matrix: FP64, FP32 (fp32) or on the packed
+        stream (packed: FP32 value + 8-bit correction, FP64 vector and sums)."""
         v = np.ascontiguousarray(v, dtype=np.float64)
         out = np.zeros(self.n if transpose else self.m)
         us = C.c_double(0.0)
-        _check_eng(self.lib.mcp_spmv(self.h, int(transpose), PRECISION_MIXED if fp32 else PRECISION_FP64,
+        prec = PRECISION_PACKED if packed else (PRECISION_MIXED if fp32 else PRECISION_FP64)
+        _check_eng(self.lib.mcp_spmv(self.h, int(transpose), prec,
                                      ORDER_REFERENCE if exact else ORDER_FAST, _ptr(v, C.c_double),
                                      _ptr(out, C.c_double), int(reps), C.byref(us)))
         return out, us.value
@@ -651,7 +657,8 @@ class Engine:
     # (cache hits), and with no gathers -- what the random gathers cost on this matrix
     KERNELS = {"primal_fp32": 0, "dual_fp32": 1, "primal_fp64": 2, "dual_fp64": 3,
                "spmv_fp32": 4, "spmv_t_fp32": 5, "copy": 6,
-               "gather_fp64": 7, "gather_local_fp64": 8, "stream_fp64": 9}
+               "gather_fp64": 7, "gather_local_fp64": 8, "stream_fp64": 9,
+               "primal_packed": 10, "dual_packed": 11}
 
     def time_kernel(self, kind: str, reps: int = 20):
         """(mean device us per launch, algorithmic bytes per launch) of one fused kernel."""
@@ -688,7 +695,8 @@ class Engine:
         for b in range(self.B):
             r = res[b]
             out.append({"status": STATUS[r.status], "mode": r.mode, "iterations": int(r.iterations),
-                        "fp32_iterations": int(r.fp32_iterations), "objective": r.objective,
+                        "fp32_iterations": int(r.fp32_iterations),
+                        "packed_iterations": int(r.packed_iterations), "objective": r.objective,
                         "primal_infeasibility": r.primal_infeasibility,
                         "dual_infeasibility": r.dual_infeasibility, "duality_gap": r.duality_gap,
                         "omega": r.omega, "eta": r.eta,
