@@ -68,6 +68,9 @@ WORKLOADS = {
 
 SCHEMES = {"reference": 0, "kkt": 1, "halpern": 2}  # mcpdlp.SCHEME_*
 PRECISIONS = {"fp64": 0, "mixed": 1, "packed": 2}  # mcpdlp.PRECISION_*
+# the arithmetic of each precision's SpMVs (the iterate and the checks are FP64 in all)
+DTYPES = {"fp64": "fp64", "mixed": "fp32 then packed fp32+8-bit values, fp64 sums",
+          "packed": "packed fp32+8-bit values, fp64 sums"}
 
 
 def log(*a):
@@ -179,6 +182,7 @@ def main():
         res, wall = run(opts)
         walls.append(wall)
         log(f"[bench] step {len(walls) - 1}: {res['status']} it={res['iterations']} fp32_it={res['fp32_iterations']} "
+            f"packed_it={res['packed_iterations']} "
             f"wall={wall:.1f}ms total={res['total_ms']:.1f} kernel={res['kernel_ms']:.1f} "
             f"setup={res['setup_ms']:.1f} h2d={res['h2d_ms']:.1f} d2h={res['d2h_ms']:.1f} ({left():.0f}s left)")
     ms = float(np.mean(walls))
@@ -218,12 +222,11 @@ def main():
     # the precision most of the solve's iterations ran in
     kern, roof, ta, tt = {}, None, None, None
     if not args.no_roofline:
-        kern, (ta, tt) = time_kernels(mp, lp, ("primal_fp32", "dual_fp32", "primal_fp64", "dual_fp64",
-                                               "spmv_fp32", "spmv_t_fp32", "gather_fp64", "gather_local_fp64",
-                                               "stream_fp64", "copy"),
+        kern, (ta, tt) = time_kernels(mp, lp, ("primal_fp32", "dual_fp32", "primal_packed", "dual_packed",
+                                               "primal_fp64", "dual_fp64", "spmv_fp32", "spmv_t_fp32",
+                                               "gather_fp64", "gather_local_fp64", "stream_fp64", "copy"),
                                       not args.no_autotune)
-        fp32_major = res["fp32_iterations"] * 2 > res["iterations"]
-        kinds = ("primal_fp32", "dual_fp32") if fp32_major else ("primal_fp64", "dual_fp64")
+        kinds = dominant_kinds(res)
         dom = max(kinds, key=lambda k: kern[k]["us"])
         ach = kern[dom]["GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -238,13 +241,14 @@ def main():
         "metric": METRIC,
         "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": len(walls), "warmup": warm_done,
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "fp64" if prec == mp.PRECISION_FP64 else "fp32+fp64", "data": "synthetic (planted-feasible, seeded)",
+        "dtype": DTYPES[args.precision], "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
                    "precision": args.precision, "restart_scheme": args.scheme, "parallelism": "single GPU"},
         "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup_iterations": args.warmup_iterations, "budget_s": args.budget_s,
         "walls_ms": [round(w, 1) for w in walls],
         "status": res["status"], "iterations": res["iterations"], "fp32_iterations": res["fp32_iterations"],
+        "packed_iterations": res["packed_iterations"],
         "kernel_ms": round(res["kernel_ms"], 3), "setup_ms": round(res["setup_ms"], 3),
         "h2d_ms": round(res["h2d_ms"], 3), "d2h_ms": round(res["d2h_ms"], 3),
         "kkt_fp64_rescored": {k: float(f"{kkt[k]:.4e}") for k in ("primal", "dual", "gap", "score")},
@@ -257,6 +261,17 @@ def main():
         "roofline": roof, "gather_ceiling": gather, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def dominant_kinds(res):
+    """The fused primal/dual kinds of the precision most of the solve's
+    iterations ran in: FP32 SpMVs, the packed stream or FP64."""
+    its, f32, pk = res["iterations"], res["fp32_iterations"], res.get("packed_iterations", 0)
+    if 2 * f32 > its:
+        return ("primal_fp32", "dual_fp32")
+    if 2 * pk > its:
+        return ("primal_packed", "dual_packed")
+    return ("primal_fp64", "dual_fp64")
 
 
 def gather_ceiling(kern, nnz):
@@ -320,7 +335,8 @@ def cpu_baseline(orc, lp, tol, gpu_iterations, seconds):
 
 
 KERNEL_SYMBOL = {"primal_fp32": "k_primal_blk", "dual_fp32": "k_dual_blk", "primal_fp64": "k_primal_blk",
-                 "dual_fp64": "k_dual_blk", "spmv_fp32": "k_spmv_plain", "spmv_t_fp32": "k_spmv_plain",
+                 "dual_fp64": "k_dual_blk", "primal_packed": "k_primal_blk", "dual_packed": "k_dual_blk",
+                 "spmv_fp32": "k_spmv_plain", "spmv_t_fp32": "k_spmv_plain",
                  "copy": "k_copy", "gather_fp64": "k_gather_probe<.,0>", "gather_local_fp64": "k_gather_probe<.,1>",
                  "stream_fp64": "k_gather_probe<.,2>"}
 
@@ -426,7 +442,8 @@ def _line(value_ms, workload, cfg, extra, steps=1, warmup=0):
 
 
 def bench_batch(args):
-    """Batched persistent-launch PDLP vs the plain FP64 GPU PDLP run instance by instance."""
+    """Batched PDLP (all instances in one packed problem, graph-replayed launches)
+    vs the plain FP64 GPU PDLP run instance by instance."""
     import numpy as np
     import mcpdlp as mp
 
@@ -456,8 +473,9 @@ def bench_batch(args):
         worst_obj = max(worst_obj, abs(res[k]["objective"] - lps[k].planted_objective) /
                         (1.0 + abs(lps[k].planted_objective)))
     # roofline of the dominant fused kernel on the packed batch
-    fp32_major = sum(r["fp32_iterations"] for r in res) * 2 > int(its.sum())
-    kinds = ("primal_fp32", "dual_fp32") if fp32_major else ("primal_fp64", "dual_fp64")
+    kinds = dominant_kinds({"iterations": int(its.sum()),
+                            "fp32_iterations": sum(r["fp32_iterations"] for r in res),
+                            "packed_iterations": sum(r["packed_iterations"] for r in res)})
     kern, _ = time_kernels(mp, lps, kinds, True)
     dom = max(kinds, key=lambda k: kern[k]["us"])
     roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -130,7 +130,8 @@ __device__ __forceinline__ double long_row_dot(int p0, int p1, const IX* __restr
 // (kModePacked), or FP64.  Every thread must call.
 // NT64: the FP64 matrix stream uses non-temporal loads (FP32 and packed
 // streams never do: the pair of 8-byte-per-nonzero matrices is small enough
-// to share the cache).
+// to share the cache, and on the 200k x 500k LP a non-temporal packed row
+// stream made the dual step 83 instead of 63 us).
 template <int NB, bool NT64 = false, typename TA32, typename TV32>
 __device__ __forceinline__ double tile_row_sum(int mode, int p0, int len, int q0, int q1, int gl, int G,
                                                const int32_t* __restrict__ idx, const uint32_t* __restrict__ idxp,
