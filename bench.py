@@ -478,6 +478,15 @@ def bench_batch(args):
                             "packed_iterations": sum(r["packed_iterations"] for r in res)})
     kern, _ = time_kernels(mp, lps, kinds, True)
     dom = max(kinds, key=lambda k: kern[k]["us"])
+    # what a persistent batch kernel would pay per phase boundary: a grid-wide
+    # barrier in a cooperative kernel vs a dependent launch in a captured graph
+    eng = mp.Engine(0)
+    sync = {}
+    for g in (8, 64, 256):
+        bu, lu = eng.time_sync(g, 2000)
+        sync[str(g)] = {"grid_barrier_us": round(bu, 3), "graph_launch_us": round(lu, 3)}
+        log(f"[bench] sync probe, {g} workgroups: grid barrier {bu:.2f} us, graph launch {lu:.2f} us")
+    eng.close()
     roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(kern[dom]["GBps"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic("batch", dom)}
     cpu = None
@@ -531,7 +540,10 @@ def bench_batch(args):
                                        "iterations_mean": float(np.mean(plain_it)),
                                        "note": "evenly spaced by size; total = mean x instances (extrapolated)"},
            "speedup_vs_plain_fp64_per_instance": round(est / wall, 2),
-           "kernels": kern, "roofline": roof, "cpu_baseline": cpu})
+           "kernels": kern, "roofline": roof, "cpu_baseline": cpu,
+           "sync_probe": {"workgroups": sync,
+                          "note": "per phase boundary: grid-wide barrier inside a cooperative persistent kernel "
+                                  "vs dependent launch in a captured HIP graph (mcp_time_sync)"}})
 
 
 def bench_warm(args):

@@ -159,22 +159,38 @@ int mcp_get_scaling(mcp_engine* e, double* col_val, double* row_scale, double* c
                     int32_t* passes, int32_t* converged);
 
 /* Micro-benchmark of one fused iteration kernel on the loaded, set-up problem,
- * launched `reps` times on the engine's stream.  The fused steps (kinds 0-3)
- * are timed as the solve runs them: each timed launch follows a launch of the
- * other step (untimed), which streams the other matrix through the caches;
- * all launches are queued before one wait, and each timed launch records a
- * HIP event pair as part of its own dispatch (hipExtLaunchKernelGGL), so the
- * mean is the kernel's own duration.  The other kinds are timed back to back
- * between two events.
+ * launched `reps` times on the engine's stream.  The fused steps (kinds 0-3,
+ * 10, 11) and the gather probes (7-9) are timed as the solve runs them: each
+ * timed launch follows a launch of the other step (untimed), which streams
+ * the other matrix through the caches; all launches are queued before one
+ * wait, and each timed launch records a HIP event pair as part of its own
+ * dispatch (hipExtLaunchKernelGGL), so the mean is the kernel's own duration.
+ * The other kinds are timed back to back between two events.
  * kind: 0 primal (A^T y + primal update) FP32, 1 dual (A x_bar + dual update)
  * FP32, 2 primal FP64, 3 dual FP64, 4 plain SpMV A x FP32, 5 plain A^T y FP32,
  * 6 plain streaming copy of the FP32 values of A (the calibration kernel for
- * HBM byte counters: it moves exactly 8 bytes per copied value).
+ * HBM byte counters: it moves exactly 8 bytes per copied value), 7-9 the FP64
+ * block SpMV A x_bar with random gathers, with every gather inside a window of
+ * at most 256 entries, and with no gathers, 10 primal and 11 dual on the
+ * packed streams (MC_PRECISION_PACKED).
  * *avg_us = mean device time per launch; *bytes = the kernel's algorithmic
  * HBM bytes per launch (matrix and pointer streams once, gathered vector once,
  * per-row/column operands once).  Leaves the iterate in an unspecified state;
  * the next mcp_solve re-initializes it. */
 int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, double* bytes);
+
+/* Synchronization probe (extension; no reference counterpart -- a
+ * measurement).  *barrier_us: mean cost of one grid-wide barrier inside a
+ * cooperative persistent kernel of `workgroups` 256-thread workgroups (all
+ * resident; more than can be is refused), each barrier an agent-scope release,
+ * one atomic arrival and an acquire-spin on a generation word, over `rounds`
+ * rounds of two barriers that also check that every workgroup sees its
+ * neighbour's writes.  *launch_us: mean cost of one dependent kernel launch of
+ * the same workgroups in a captured HIP graph.  A persistent batch kernel that
+ * replaced the iteration's launches by barriers would pay barrier_us where the
+ * graph pays launch_us.  Every wait is bounded: a barrier that runs out of
+ * spins is reported as an error, never left hanging. */
+int mcp_time_sync(mcp_engine* e, int32_t workgroups, int32_t rounds, double* barrier_us, double* launch_us);
 
 /* Pin the tile policies of the loaded problem (rows of A, rows of A^T) and
  * re-tile; the autotuner's choice is overwritten.  For tuning experiments and

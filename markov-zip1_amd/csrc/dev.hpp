@@ -151,6 +151,12 @@ void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm
 void launch_setup_state(const Dev& d, double initial_weight, double step_reduction, int start_mode,
                         hipStream_t s);
 void launch_fill(double* p, double v, int64_t n, hipStream_t s);
+// Synchronization probe (mcp_time_sync): the cooperative barrier kernel, the
+// number of its workgroups that can be resident at once, and one round of
+// the barrier-free form.
+int barrier_probe_capacity();
+hipError_t launch_barrier_probe(unsigned* bar, double* buf, int words, int rounds, int workgroups, hipStream_t s);
+void launch_barrier_step(double* buf, int words, int round, int workgroups, hipStream_t s);
 // Plain streaming copy of n floats (n a multiple of 4, 16-byte aligned
 // buffers): the reference kernel that calibrates the HBM byte counters.
 void launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);

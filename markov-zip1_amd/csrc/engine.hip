@@ -1170,6 +1170,58 @@ int mcp_time_kernel(mcp_engine* e, int32_t kind, int32_t reps, double* avg_us, d
     });
 }
 
+int mcp_time_sync(mcp_engine* e, int32_t workgroups, int32_t rounds, double* barrier_us, double* launch_us) {
+    return guarded([&] {
+        require(e && barrier_us && launch_us, "null argument");
+        require(workgroups >= 1 && rounds >= 1 && rounds <= 100000, "workgroups >= 1 and rounds in [1, 100000]");
+        HIPCHK(hipSetDevice(e->device));
+        const int cap = mcp::barrier_probe_capacity();
+        require(workgroups <= cap, "more workgroups than can be resident at once (" + std::to_string(cap) + ")");
+        hipStream_t s = e->stream;
+        const int words = 64;
+        unsigned* bar = e->buf<unsigned>("sync_bar", 4);
+        double* slots = e->buf<double>("sync_slots", (size_t)workgroups * words);
+        HIPCHK(hipMemsetAsync(bar, 0, 4 * sizeof(unsigned), s));
+        HIPCHK(mcp::launch_barrier_probe(bar, slots, words, 4, workgroups, s));  // warm
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        HIPCHK(mcp::launch_barrier_probe(bar, slots, words, rounds, workgroups, s));
+        HIPCHK(hipEventRecord(e->ev[1], s));
+        HIPCHK(hipEventSynchronize(e->ev[1]));
+        unsigned hb[4];
+        HIPCHK(hipMemcpy(hb, bar, sizeof(hb), hipMemcpyDeviceToHost));
+        require(hb[2] == 0, "grid barrier: a wait ran out of spins");
+        require(hb[3] == 0, "grid barrier: a read after the barrier saw a stale value");
+        *barrier_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / (2.0 * rounds);
+        // the same rounds as dependent launches in a captured graph (at most
+        // 256 per graph, replayed)
+        const int per = std::min(rounds, 256);
+        const int reps = (rounds + per - 1) / per;
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < per; ++r) mcp::launch_barrier_step(slots, words, r, workgroups, s);
+        HIPCHK(hipStreamEndCapture(s, &g));
+        hipGraphExec_t ex;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ie);
+        auto run = [&] {
+            HIPCHK(hipGraphLaunch(ex, s));  // warm
+            HIPCHK(hipEventRecord(e->ev[0], s));
+            for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ex, s));
+            HIPCHK(hipEventRecord(e->ev[1], s));
+            HIPCHK(hipEventSynchronize(e->ev[1]));
+        };
+        try {
+            run();
+        } catch (...) {
+            (void)hipGraphExecDestroy(ex);
+            throw;
+        }
+        (void)hipGraphExecDestroy(ex);
+        *launch_us = 1000.0 * ms_between(e->ev[0], e->ev[1]) / ((double)reps * per);
+    });
+}
+
 int mcp_set_tile_policy(mcp_engine* e, const mcp_tile_policy* rows_policy, const mcp_tile_policy* cols_policy) {
     return guarded([&] {
         require(e && e->loaded && rows_policy && cols_policy, "mcp_set_tile_policy needs a loaded problem");

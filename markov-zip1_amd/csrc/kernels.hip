@@ -1165,6 +1165,82 @@ __global__ __launch_bounds__(kBlock) void k_spmv_plain(const Tile* tiles, int nt
 }
 
 // ---------------------------------------------------------------------------
+// Synchronization probe (measurement only, mcp_time_sync): what a persistent
+// kernel would pay for a grid-wide barrier between the phases of an
+// iteration, against the boundary between two dependent launches in a
+// captured graph.  bar[0]: arrivals, bar[1]: generation, bar[2]: set when a
+// wait ran out of spins (the barrier is then abandoned, never hung), bar[3]:
+// reads after a barrier that saw a stale value.
+constexpr unsigned kBarrierSpinLimit = 1u << 20;
+
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned a = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (a == nblocks - 1) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + 1, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g &&
+                   __hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kBarrierSpinLimit) {
+                    __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// `rounds` rounds of: write this workgroup's slot of `words` doubles,
+// barrier, check the next workgroup's slot, barrier.  Cooperative launch
+// only (every workgroup resident).
+__global__ __launch_bounds__(kBlock) void k_barrier_probe(unsigned* bar, double* buf, int words, int rounds) {
+    const unsigned nb = gridDim.x;
+    double* mine = buf + (size_t)blockIdx.x * words;
+    const double* next = buf + (size_t)((blockIdx.x + 1) % nb) * words;
+    for (int r = 0; r < rounds; ++r) {
+        for (int k = threadIdx.x; k < words; k += blockDim.x) mine[k] = (double)(r + 1);
+        grid_barrier(bar, nb);
+        int stale = 0;
+        for (int k = threadIdx.x; k < words; k += blockDim.x) stale += next[k] != (double)(r + 1);
+        if (stale) atomicAdd(bar + 3, 1u);
+        grid_barrier(bar, nb);
+    }
+}
+
+// One round's write of the same slots as a kernel of its own (a graph of
+// `rounds` dependent launches is the barrier-free alternative).
+__global__ __launch_bounds__(kBlock) void k_barrier_step(double* buf, int words, int round) {
+    double* mine = buf + (size_t)blockIdx.x * words;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) mine[k] = (double)(round + 1);
+}
+
+int barrier_probe_capacity() {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_barrier_probe, kBlock, 0) != hipSuccess)
+        return 0;
+    return cus * per;
+}
+
+hipError_t launch_barrier_probe(unsigned* bar, double* buf, int words, int rounds, int workgroups, hipStream_t s) {
+    void* args[] = {&bar, &buf, &words, &rounds};
+    return hipLaunchCooperativeKernel((const void*)k_barrier_probe, dim3(workgroups), dim3(kBlock), args, 0, s);
+}
+
+void launch_barrier_step(double* buf, int words, int round, int workgroups, hipStream_t s) {
+    hipLaunchKernelGGL(k_barrier_step, dim3(workgroups), dim3(kBlock), 0, s, buf, words, round);
+}
+
+// ---------------------------------------------------------------------------
 // Step cap of the Halpern scheme (the oracle's or_step_cap, per instance):
 // v_j = 1 + (j mod 7)/1000 (j local to the instance), S = |v|^2; per iteration
 // t = (v / sqrt(S)) / sqrt(cnorm), u = (A t) / rnorm, v = (A^T u) / sqrt(cnorm),

@@ -210,6 +210,7 @@ def engine_lib():
         lib.mcp_get_scaling.argtypes = [C.c_void_p, P_f64, P_f64, P_f64, P_i32, P_i32]
         lib.mcp_get_tiling.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy), P_i32, P_i32]
         lib.mcp_time_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P_f64, P_f64]
+        lib.mcp_time_sync.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P_f64, P_f64]
         lib.mcp_set_tile_policy.argtypes = [C.c_void_p, C.POINTER(TilePolicy), C.POINTER(TilePolicy)]
         lib.mcp_set_block_budget.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
         lib.mcp_get_block_tiling.argtypes = [C.c_void_p, P_i32, P_i32, P_i32, P_i32]
@@ -665,6 +666,13 @@ class Engine:
         us, b = C.c_double(), C.c_double()
         _check_eng(self.lib.mcp_time_kernel(self.h, self.KERNELS[kind], int(reps), C.byref(us), C.byref(b)))
         return us.value, b.value
+
+    def time_sync(self, workgroups, rounds=2000):
+        """(us per grid-wide barrier in a cooperative persistent kernel of
+        `workgroups` workgroups, us per dependent launch in a captured graph)."""
+        bu, lu = C.c_double(), C.c_double()
+        _check_eng(self.lib.mcp_time_sync(self.h, int(workgroups), int(rounds), C.byref(bu), C.byref(lu)))
+        return bu.value, lu.value
 
     def solve(self, tolerance=1e-4, max_iterations=100000, precision=PRECISION_FP64, order=ORDER_FAST,
               graph_chunk=0, **kw):
