@@ -750,13 +750,17 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
             wy = t;
         }
         mcp::launch_init_iterates(d, wx, wy, s);
-        // The reduced-precision phase serves the start far from the optimum;
-        // the FP32-SpMV floor (1e-2 to 1e-4 relative KKT on planted LPs) lies
-        // above a warm start's accuracy, so a warm-started solve begins in FP64.
-        const bool fp32_start = mixed && !wx && !wy;
-        const int start_mode = !fp32_start ? mcp::kModeFp64
-                               : o->precision == MC_PRECISION_MIXED ? mcp::kModeFp32
-                                                                    : mcp::kModePacked;
+        // The FP32-SpMV phase serves the start far from the optimum; its floor
+        // (1e-2 to 1e-4 relative KKT on planted LPs) lies above a warm start's
+        // accuracy, so a warm-started mixed solve begins on the packed stream
+        // (whose floor is measured at every check), or in FP64 without one.
+        const bool warm = wx || wy;
+        int start_mode = mcp::kModeFp64;
+        if (o->precision == MC_PRECISION_PACKED)
+            start_mode = mcp::kModePacked;
+        else if (o->precision == MC_PRECISION_MIXED)
+            start_mode = !warm ? mcp::kModeFp32 : (d.cix ? mcp::kModePacked : mcp::kModeFp64);
+        const bool fp32_start = start_mode != mcp::kModeFp64;  // a reduced-precision start
         mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, start_mode, s);
         if (io.halpern && io.adaptive_step && o->power_iterations > 0)
             mcp::launch_step_cap(d, (int64_t)o->power_iterations, exact, s);
