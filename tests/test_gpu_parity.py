@@ -379,3 +379,8 @@ def test_cli_gpu_backend(gpu):
     code_c, js_c, _ = run("--backend", "cpu")
     assert js["objective"] == js_c["objective"] and js["primal"] == js_c["primal"]
     assert round(js["objective"], 7) == -9.0000013
+    # the packed stream from the CLI: same optimum, every iteration counted on it or in FP64
+    code_p, js_p, err_p = run("--backend", "gpu", "--precision", "packed", "--restart-scheme", "halpern")
+    assert code_p == 0 and js_p["status"] == "Optimal" and js_p["verified"] and js_p["precision"] == "packed"
+    assert js_p["backend_actually_used"] == "hip" and 0 < js_p["packed_iterations"] <= js_p["lp_iterations"]
+    assert abs(js_p["objective"] - (-9.0)) <= 1e-5
