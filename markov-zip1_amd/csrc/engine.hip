@@ -754,12 +754,18 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         // (1e-2 to 1e-4 relative KKT on planted LPs) lies above a warm start's
         // accuracy, so a warm-started mixed solve begins on the packed stream
         // (whose floor is measured at every check), or in FP64 without one.
+        // On problems of up to kLatencyNnz nonzeros an iteration is a chain of
+        // launch and memory latencies whatever the arithmetic (the 2000 x 5000
+        // LP: 5.4-5.8 us per step in FP32, packed and FP64 alike), so the FP32
+        // phase saves nothing there and only moves the trajectory (that LP:
+        // 137,640 iterations after an FP32 start, 50,920 from a packed one).
         const bool warm = wx || wy;
+        const bool fp32_pays = e->nnz > kLatencyNnz || !d.cix;
         int start_mode = mcp::kModeFp64;
         if (o->precision == MC_PRECISION_PACKED)
             start_mode = mcp::kModePacked;
         else if (o->precision == MC_PRECISION_MIXED)
-            start_mode = !warm ? mcp::kModeFp32 : (d.cix ? mcp::kModePacked : mcp::kModeFp64);
+            start_mode = (!warm && fp32_pays) ? mcp::kModeFp32 : (d.cix ? mcp::kModePacked : mcp::kModeFp64);
         const bool fp32_start = start_mode != mcp::kModeFp64;  // a reduced-precision start
         mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, start_mode, s);
         if (io.halpern && io.adaptive_step && o->power_iterations > 0)

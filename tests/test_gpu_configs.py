@@ -6,8 +6,8 @@ src/scale/ruiz_scaling.cpp:152-247).
   (11.1M nonzeros): device Ruiz bit-identical to the oracle; SpMV in both
   orientations on the tiles and block budgets the solve uses (autotuned, rows
   and columns relabelled as the solve relabels them): reference order
-  bit-identical, fast order within 1e-13, FP32 within 2e-6 of the row's
-  absolute sum; 120 iterations in reference order bit-identical to the oracle
+  bit-identical, fast order within 1e-13, the packed stream within 2^-30 and
+  FP32 within 2e-6 of the row's absolute sum; 120 iterations in reference order bit-identical to the oracle
   (reference and Halpern schemes); and the benchmark's own solve (mixed
   precision, Halpern scheme, autotune on) to 1e-6, re-scored by the oracle's
   FP64 residuals and held to 1e-6 x (1 + |p*|) of the planted optimum;
@@ -75,6 +75,9 @@ def check_ruiz_and_spmv(lp):
             assert np.all(np.abs(f - want) <= 1e-13 * (np.abs(want) + bound)), ("fast order", t)
             f32, _ = e.spmv(v, transpose=t, fp32=True)
             assert np.all(np.abs(f32 - want) <= 2e-6 * bound + 1e-6 * np.abs(want) + 1e-30), ("fp32", t)
+            # the packed stream the benchmark's solve runs on: within 2^-30 of the absolute sum
+            pk, _ = e.spmv(v, transpose=t, packed=True)
+            assert np.all(np.abs(pk - want) <= 2.0 ** -30 * bound + 1e-13 * (np.abs(want) + bound)), ("packed", t)
     finally:
         e.close()
 
@@ -127,6 +130,9 @@ def test_large_benchmark_solve_to_1e6(large):
     r = mp.solve(large, tolerance=1e-6, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_MIXED,
                  restart_scheme=mp.SCHEME_HALPERN, autotune=1, max_iterations=2_000_000)
     assert r["backend_actually_used"] == "hip"
+    # 11.1M nonzeros: an FP32 phase, then the packed stream
+    assert r["fp32_iterations"] > 0 and r["packed_iterations"] > 0
+    assert r["fp32_iterations"] + r["packed_iterations"] <= r["iterations"]
     score, err = check_solution(large, r)
     print("iterations", r["iterations"], "fp32", r["fp32_iterations"], "score", score, "objective error", err,
           "wall", r["total_ms"], flush=True)

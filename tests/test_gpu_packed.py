@@ -13,8 +13,10 @@ Bars:
 * solves in packed and mixed precision reach the tolerance; the returned pair
   re-scored by the oracle's FP64 residuals is within 1.5x of it; the objective
   is within 1e-6 x (1 + |p*|) of the planted optimum (or the oracle's FP64
-  objective at 1e-9); packed solves report every iteration on the packed
-  stream, mixed ones an FP32 phase followed by the packed one;
+  objective at 1e-9); packed solves, and mixed ones on latency-bound LPs
+  (which skip the FP32 phase), report their iterations on the packed stream
+  or, near its floor, in FP64 (the FP32 -> packed hand-over runs at the
+  benchmark sizes, tests/test_gpu_configs.py);
 * once the score nears the packed floor (the largest change of a KKT
   component when the check's average is scored with the packed values; about
   5e-7 in the duality gap of the uniform planted LPs, whose objective cancels
@@ -88,7 +90,8 @@ def test_packed_solves_reach_the_optimum(maker, tol, precision):
         # the last iterations may run in FP64, once the score nears the packed floor
         assert f32 == 0 and 0 < pk <= its
     else:
-        assert f32 > 0 and f32 + pk <= its
+        # these LPs are latency-bound (at most 2^21 nonzeros): mixed skips the FP32 phase
+        assert f32 == 0 and 0 < pk <= its
 
 
 def test_packed_hands_over_to_fp64_below_its_floor():

@@ -214,7 +214,8 @@ def test_kkt_scheme_converges_to_1e6(gpu, precision, scheme):
     k = orc.kkt(lp, got["primal"], got["dual"])
     assert k["score"] <= 1.5e-6, k
     if precision == mp.PRECISION_MIXED:
-        assert got["fp32_iterations"] > 0
+        # latency-bound (25k nonzeros): mixed starts on the packed stream, no FP32 phase
+        assert got["fp32_iterations"] == 0 and got["packed_iterations"] > 0
     tight = gpu_solve(lp, tolerance=1e-8, restart_scheme=scheme, precision=precision, max_iterations=200000)
     assert tight["status"] == "optimal", tight["message"]
     assert orc.kkt(lp, tight["primal"], tight["dual"])["score"] <= 1.5e-8
@@ -261,7 +262,7 @@ def test_netlib_fast_order_reaches_published_optimum(gpu, name, precision, schem
 
 def test_repeated_solves_follow_their_own_options(gpu):
     """Solving one LP again on the same engine with other options (tolerance,
-    FP32 stall window) must use those options, not the ones the cached HIP
+    polish threshold) must use those options, not the ones the cached HIP
     graphs were captured with; a repeat of the first solve is bit-identical."""
     lp = small_uniform(seed=21)
     kw = dict(restart_scheme=mp.SCHEME_HALPERN, max_iterations=200000)
@@ -272,9 +273,11 @@ def test_repeated_solves_follow_their_own_options(gpu):
     assert b["iterations"] > a["iterations"]
     assert orc.kkt(lp, b["primal"], b["dual"])["score"] <= 1.5e-6
     assert c["iterations"] == a["iterations"] and np.array_equal(c["primal"], a["primal"])
-    m1 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1, **kw)
-    m2 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_MIXED, fp32_stall_checks=1000000, **kw)
-    assert m1["fp32_iterations"] < m2["fp32_iterations"]
+    m1 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_PACKED, polish_threshold=1e-3, **kw)
+    m2 = gpu_solve(lp, tolerance=1e-6, precision=mp.PRECISION_PACKED, polish_threshold=0.0, **kw)
+    assert m1["status"] == m2["status"] == "optimal"
+    # m1 leaves the packed stream at a score of 1e-3, m2 only near its floor
+    assert m1["packed_iterations"] < m2["packed_iterations"]
 
 
 def test_repeated_solves_follow_adaptive_step_option(gpu):
