@@ -288,7 +288,10 @@ def gather_ceiling(kern, nnz):
             "random_gathers_per_s": float(f"{nnz / g * 1e6:.4e}"),
             "dual_fp64_vs_plain_spmv": round(g / kern["dual_fp64"]["us"], 4),
             "primal_fp64_vs_plain_spmv": round(g / kern["primal_fp64"]["us"], 4),
-            "note": "fraction = plain FP64 SpMV time with the same random gathers / fused step time"}
+            "dual_packed_vs_plain_spmv": round(g / kern["dual_packed"]["us"], 4) if "dual_packed" in kern else None,
+            "primal_packed_vs_plain_spmv": round(g / kern["primal_packed"]["us"], 4) if "primal_packed" in kern else None,
+            "note": "fraction = plain FP64 SpMV time with the same random gathers / fused step time; "
+                    "above 1 when the fused step streams fewer bytes per nonzero (packed)"}
 
 
 def cpu_model():
