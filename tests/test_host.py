@@ -97,6 +97,21 @@ def test_backend_labels_and_trivial():
     assert r["status"] == "optimal" and r["message"] == "trivial" and r["primal"].size == 0  # pdlp.cpp:144-149
 
 
+def test_precision_option_on_the_cpu_backend():
+    """The precision extension (fp64, mixed, packed) selects the GPU engine's
+    SpMV arithmetic; the host restatement is FP64 whatever it says, so every
+    value gives the same bits there and reports no reduced-precision
+    iterations."""
+    lp = tiny()
+    runs = [mp.solve(lp, tolerance=1e-6, precision=p) for p in (mp.PRECISION_FP64, mp.PRECISION_MIXED,
+                                                               mp.PRECISION_PACKED)]
+    for r in runs:
+        assert r["backend_actually_used"] == "cpu" and r["status"] == "optimal"
+        assert r["fp32_iterations"] == 0 and r["packed_iterations"] == 0
+        assert r["iterations"] == runs[0]["iterations"] and r["objective"] == runs[0]["objective"]
+        assert np.array_equal(r["primal"], runs[0]["primal"])
+
+
 def test_iteration_limit_zero_reports_start_point():
     lp = mp.generate(mp.FAMILY_UNIFORM_EQ, rows=50, cols=120, seed=2)
     a, b = mp.solve(lp, max_iterations=0), orc.solve(lp, max_iterations=0)
