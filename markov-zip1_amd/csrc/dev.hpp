@@ -96,7 +96,7 @@ struct Dev {
     double* pw;         // [2B] step-cap power method: S_b = |v_b|^2, and 1 once S_b stopped being > 0
     InstState* st;      // [B]
     int64_t* iter_base; // global iteration counter at the start of the current chunk
-    int32_t* counters;  // [0] running instances, [1] running instances in FP32
+    int32_t* counters;  // [0] running instances, [1] running instances in FP32 or packed mode
     // block tiles of the fused fast-order primal/dual kernels (stream.hip):
     // one 256-thread workgroup per tile
     const Tile* bA;

@@ -756,7 +756,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         // (whose floor is measured at every check), or in FP64 without one.
         // On problems of up to kLatencyNnz nonzeros an iteration is a chain of
         // launch and memory latencies whatever the arithmetic (the 2000 x 5000
-        // LP: 5.4-5.8 us per step in FP32, packed and FP64 alike), so the FP32
+        // LP: 5.2-5.8 us per step in FP32, packed and FP64 alike), so the FP32
         // phase saves nothing there and only moves the trajectory (that LP:
         // 137,640 iterations after an FP32 start, 50,920 from a packed one).
         const bool warm = wx || wy;
@@ -766,13 +766,13 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
             start_mode = mcp::kModePacked;
         else if (o->precision == MC_PRECISION_MIXED)
             start_mode = (!warm && fp32_pays) ? mcp::kModeFp32 : (d.cix ? mcp::kModePacked : mcp::kModeFp64);
-        const bool fp32_start = start_mode != mcp::kModeFp64;  // a reduced-precision start
+        const bool reduced_start = start_mode != mcp::kModeFp64;
         mcp::launch_setup_state(d, o->initial_primal_weight, o->step_size_reduction, start_mode, s);
         if (io.halpern && io.adaptive_step && o->power_iterations > 0)
             mcp::launch_step_cap(d, (int64_t)o->power_iterations, exact, s);
         int32_t* pin = e->pinned;
         pin[4] = e->B;
-        pin[5] = fp32_start ? e->B : 0;
+        pin[5] = reduced_start ? e->B : 0;
         HIPCHK(hipMemcpyAsync(d.counters, pin + 4, 2 * sizeof(int32_t), hipMemcpyHostToDevice, s));
         int64_t zero = 0;
         HIPCHK(hipMemcpyAsync(d.iter_base, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
@@ -873,7 +873,7 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
                         mcp::InstState s0;
                         HIPCHK(hipMemcpy(&s0, d.st, sizeof(s0), hipMemcpyDeviceToHost));
                         std::fprintf(stderr,
-                                     "[mcpdlp] %.1fs iterations=%lld running=%d fp32=%d | inst0 score=%.3e "
+                                     "[mcpdlp] %.1fs iterations=%lld running=%d reduced=%d | inst0 score=%.3e "
                                      "(p=%.2e d=%.2e g=%.2e) omega=%.3g eta=%.3g mode=%s\n",
                                      std::chrono::duration<double>(now - prog_start).count(), (long long)iter,
                                      pin[2 * prev], pin[2 * prev + 1], s0.score, s0.primal, s0.dual, s0.gap,
