@@ -588,8 +588,15 @@ def bench_warm(args):
     t2 = time.perf_counter()
     warm, fallbacks = ws.solve_batch(test, **kw)
     warm_ms = (time.perf_counter() - t2) * 1e3
+    # ablation: every test instance started at the training solutions' mean
+    t3 = time.perf_counter()
+    mean_start, mean_fallbacks = ws.solve_batch(test, use_network=False, **kw)
+    mean_ms = (time.perf_counter() - t3) * 1e3
     ci = np.array([r["iterations"] for r in cold])
     wi = np.array([r["iterations"] for r in warm])
+    mi = np.array([r["iterations"] for r in mean_start])
+    log(f"[bench] mean start (no network): it mean={mi.mean():.0f} max={mi.max()} {mean_ms:.0f}ms "
+        f"({mean_fallbacks} cold fallbacks)")
     log(f"[bench] cold: it mean={ci.mean():.0f} max={ci.max()} {cold_ms:.0f}ms | warm: it mean={wi.mean():.0f} "
         f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference; {fallbacks} cold fallbacks)")
     _line(warm_ms, WORKLOADS["warm"]["name"].format(rows=base.rows, cols=base.cols),
@@ -601,6 +608,10 @@ def bench_warm(args):
            "optimal_cold": int(sum(r["status"] == "optimal" for r in cold)),
            "optimal_warm": int(sum(r["status"] == "optimal" for r in warm)),
            "warm_fallbacks": int(fallbacks), "start_primal_weight": tune,
+           "mean_start": {"iterations_mean": float(mi.mean()), "iterations_max": int(mi.max()),
+                          "ms": round(mean_ms, 1), "fallbacks": int(mean_fallbacks),
+                          "note": "ablation: every instance started at the training solutions' mean in the "
+                                  "reduced basis, same start weight and fallback"},
            "speedup_warm_vs_cold": round(cold_ms / warm_ms, 3), "fit": fit,
            "training_solutions_ms": round(gen_ms, 1)})
 

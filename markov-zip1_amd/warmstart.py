@@ -108,9 +108,14 @@ class WarmStart:
                 "samples": len(lps), "params": self.net.nparam, "primal_weight": self.omega,
                 "iteration_cap": self.cap}
 
-    def predict(self, lps):
-        """[(x0, y0)] per instance; pass self.omega as initial_primal_weight."""
-        C = self.net.forward(self.features(lps)).astype(np.float64) * self.c_std + self.c_mean
+    def predict(self, lps, use_network: bool = True):
+        """[(x0, y0)] per instance; pass self.omega as initial_primal_weight.
+        use_network=False: the training solutions' mean in the reduced basis for
+        every instance (the ablation that shows what the network adds)."""
+        if use_network:
+            C = self.net.forward(self.features(lps)).astype(np.float64) * self.c_std + self.c_mean
+        else:
+            C = np.tile(self.c_mean, (len(lps), 1))
         Z = self.z_mean + C @ self.z_basis.T
         n = self.base.cols
         out = []
@@ -121,12 +126,12 @@ class WarmStart:
             out.append((np.clip(x, lo, hi), z[n:] * self.sy))
         return out
 
-    def solve_batch(self, lps, omega=None, **kw):
+    def solve_batch(self, lps, omega=None, use_network: bool = True, **kw):
         """Warm-started batched solve (mcpdlp.solve_batch keywords) with the
         cold start as the fallback.  Returns (results, fallbacks): an instance
         re-solved cold carries 'warm_fallback': True and the warm attempt's
         iterations added to its own."""
-        pred = self.predict(lps)
+        pred = self.predict(lps, use_network)
         w = self.omega if omega is None else omega
         kw_w = dict(kw)
         if self.cap > 0:
