@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--scheme", default="halpern", choices=sorted(SCHEMES),
                     help="restart scheme of the measured solves")
+    ap.add_argument("--row-weighting", type=int, default=3, choices=range(9), metavar="0..8",
+                    help="rows scaled by (1 + |bound|)^(-k/8) after Ruiz in the measured solves (0: the "
+                         "reference's scaling; the plain FP64 comparison always uses 0)")
     ap.add_argument("--plain-scheme", default="kkt", choices=sorted(SCHEMES),
                     help="restart scheme of the plain FP64 comparison (classic PDLP restarts to the average)")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -145,7 +148,8 @@ def main():
 
     prec = PRECISIONS[args.precision]
     common = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
-                  graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1)
+                  graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1,
+                  row_weighting=args.row_weighting)
     opts = mp.options(precision=prec, max_iterations=args.max_iterations, **common)
 
     def run(o):
@@ -243,7 +247,8 @@ def main():
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": DTYPES[args.precision], "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
-                   "precision": args.precision, "restart_scheme": args.scheme, "parallelism": "single GPU"},
+                   "precision": args.precision, "restart_scheme": args.scheme,
+                   "row_weighting": args.row_weighting, "parallelism": "single GPU"},
         "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup_iterations": args.warmup_iterations, "budget_s": args.budget_s,
         "walls_ms": [round(w, 1) for w in walls],
@@ -462,7 +467,8 @@ def bench_batch(args):
               max_iterations=args.max_iterations, graph_chunk=args.graph_chunk)
     t0 = time.perf_counter()
     prec = PRECISIONS[args.precision]
-    res = mp.solve_batch(lps, precision=prec, restart_scheme=SCHEMES[args.scheme], **kw)
+    res = mp.solve_batch(lps, precision=prec, restart_scheme=SCHEMES[args.scheme], row_weighting=args.row_weighting,
+                         **kw)
     wall = (time.perf_counter() - t0) * 1e3
     its = np.array([r["iterations"] for r in res])
     ok = sum(r["status"] == "optimal" for r in res)
@@ -534,6 +540,7 @@ def bench_batch(args):
     _line(wall, WORKLOADS["batch"]["name"],
           {"instances": len(lps), "sum_rows": int(sum(l.rows for l in lps)), "sum_nnz": int(sum(l.nnz for l in lps)),
            "tolerance": tol, "precision": args.precision, "restart_scheme": args.scheme,
+           "row_weighting": args.row_weighting,
            "plain_restart_scheme": args.plain_scheme},
           {"optimal": int(ok), "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
            "kkt_fp64_rescored_worst_of_sample": float(f"{worst:.4e}"),
@@ -564,7 +571,8 @@ def bench_warm(args):
     log(f"[bench] family: {ntr} train + {nte} test instances of {base.rows}x{base.cols} "
         f"({base.nnz} nnz) in {time.time() - t:.1f}s")
     kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
-              max_iterations=args.max_iterations, precision=mp.PRECISION_MIXED, graph_chunk=args.graph_chunk)
+              max_iterations=args.max_iterations, precision=mp.PRECISION_MIXED, graph_chunk=args.graph_chunk,
+              row_weighting=args.row_weighting)
     t0 = time.perf_counter()
     sols = []
     for k in range(0, ntr, 128):
@@ -601,7 +609,8 @@ def bench_warm(args):
         f"max={wi.max()} {warm_ms:.0f}ms (incl. features+inference; {fallbacks} cold fallbacks)")
     _line(warm_ms, WORKLOADS["warm"]["name"].format(rows=base.rows, cols=base.cols),
           {"train": ntr, "test": nte, "m": base.rows, "n": base.cols, "nnz": base.nnz, "tolerance": tol,
-           "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": args.scheme},
+           "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": args.scheme,
+           "row_weighting": args.row_weighting},
           {"cold_ms": round(cold_ms, 1), "warm_ms": round(warm_ms, 1),
            "iterations_cold_mean": float(ci.mean()), "iterations_warm_mean": float(wi.mean()),
            "iterations_cold_max": int(ci.max()), "iterations_warm_max": int(wi.max()),

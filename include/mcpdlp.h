@@ -148,6 +148,11 @@ typedef struct mc_pdlp_options {
                                          since the last restart is a primal or dual ray certificate (0: never,
                                          as the reference) */
     double infeasibility_tolerance;   /* certificate tolerance (0 => 1e-8) */
+    int32_t row_weighting;            /* k in 0..8: after the Ruiz passes, row i of A and its bounds are scaled by
+                                         (1 + beta_i)^(-k/8), beta_i the largest magnitude of its finite original
+                                         bounds; the reported residuals and vectors stay those of the original
+                                         model (0: the reference's scaling) */
+    int32_t pad3_;
 } mc_pdlp_options;
 
 /* PdlpStatus (reference: pdlp.hpp:43). */

@@ -82,6 +82,8 @@ typedef struct mcp_setup_opts {
     int32_t ruiz_iterations;   /* PdlpOptions::ruiz_iterations */
     int32_t exact;             /* MC_ORDER_REFERENCE tiling (1 lane per row) */
     int32_t autotune;          /* time candidate tile policies on the device */
+    int32_t row_weighting;     /* mc_pdlp_options::row_weighting (0..8) */
+    int32_t pad_;
 } mcp_setup_opts;
 int mcp_setup(mcp_engine* e, const mcp_setup_opts* o);
 

@@ -215,6 +215,20 @@ __device__ __forceinline__ double packed_of(double a) {
     const float p2 = __int_as_float(__float_as_int(a32) & 0x7F800000);
     return (double)a32 + (double)pack_correction(a, a32) * ((double)p2 * 0x1p-31);
 }
+// Row weight (1 + beta)^(-k/8), k in 0..8, from the square roots of 1 + beta
+// that the bits of k select, multiplied in the order 8, 4, 2, 1: every
+// operation is correctly rounded, so host and device agree bit for bit.
+constexpr int kMaxRowWeighting = 8;
+__device__ __forceinline__ double row_weight(double beta, int k) {
+    const double s0 = 1.0 + beta;
+    const double s1 = sqrt(s0), s2 = sqrt(s1), s3 = sqrt(s2);
+    double p = 1.0;
+    if (k & 8) p *= s0;
+    if (k & 4) p *= s1;
+    if (k & 2) p *= s2;
+    if (k & 1) p *= s3;
+    return 1.0 / p;
+}
 // Index and value of one stream element: plain (int32 index, value as
 // stored) or packed (uint32 word, FP32 value).
 __device__ __forceinline__ int ix_of(int32_t i) { return i; }

@@ -145,6 +145,8 @@ void launch_advance(const Dev& d, int len, hipStream_t s);
 void launch_step_cap(const Dev& d, int64_t iters, bool exact, hipStream_t s);
 // Setup kernels (run on the current tiles).
 void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s);
+// rows scaled by (1 + beta_i)^(-k8/8) after the Ruiz passes (mcp_setup_opts::row_weighting)
+void launch_row_weights(const Dev& d, int k8, hipStream_t s);
 void launch_setup_data(const Dev& d, hipStream_t s);
 void launch_init_iterates(const Dev& d, const double* warm_x, const double* warm_y, hipStream_t s);
 // start_mode: the SpMV mode every instance starts in (kModeFp64 / kModeFp32 / kModePacked).

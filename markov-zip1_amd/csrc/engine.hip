@@ -222,7 +222,7 @@ struct mcp_engine {
     int budA = kBlockBudget, budT = kBlockBudget;
     bool loaded = false, setup_done = false, exact_tiles = false;
     bool compacted = false;  // tiles cover only the instances still running (inside mcp_solve)
-    int ruiz_passes_req = -1, ruiz_on_req = -1;
+    int ruiz_passes_req = -1, ruiz_on_req = -1, row_weighting_req = -1;
     double h2d_ms = 0.0, setup_ms = 0.0;
     std::map<std::tuple<int64_t, int, int, int, int>, hipGraphExec_t> graphs;
     mcp::Dev graph_dev{};      // kernel arguments the cached graphs were captured with:
@@ -659,10 +659,12 @@ int mcp_setup(mcp_engine* e, const mcp_setup_opts* o) {
     return guarded([&] {
         require(e && o, "null argument");
         require(e->loaded, "mcp_setup before mcp_load");
+        require(o->row_weighting >= 0 && o->row_weighting <= mcp::kMaxRowWeighting, "row_weighting must be in 0..8");
         HIPCHK(hipSetDevice(e->device));
         if (e->setup_done) {
-            require(e->ruiz_on_req == o->ruiz && e->ruiz_passes_req == o->ruiz_iterations,
-                    "problem already equilibrated with other Ruiz options; reload it");
+            require(e->ruiz_on_req == o->ruiz && e->ruiz_passes_req == o->ruiz_iterations &&
+                        e->row_weighting_req == o->row_weighting,
+                    "problem already equilibrated with other Ruiz or row-weighting options; reload it");
             if ((o->exact != 0) != e->exact_tiles) e->retile(o->exact != 0);
             return;
         }
@@ -683,6 +685,7 @@ int mcp_setup(mcp_engine* e, const mcp_setup_opts* o) {
         mcp::launch_fill(d.col_scale, 1.0, e->n, s);
         if (o->ruiz && e->nnz > 0)
             for (int k = 0; k < o->ruiz_iterations; ++k) mcp::launch_ruiz_pass(d, 1e-3, s);
+        if (o->row_weighting > 0 && e->m > 0) mcp::launch_row_weights(d, o->row_weighting, s);
         mcp::launch_setup_data(d, s);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(e->ev[1], s));
@@ -690,6 +693,7 @@ int mcp_setup(mcp_engine* e, const mcp_setup_opts* o) {
         e->setup_ms = ms_between(e->ev[0], e->ev[1]);
         e->ruiz_on_req = o->ruiz;
         e->ruiz_passes_req = o->ruiz_iterations;
+        e->row_weighting_req = o->row_weighting;
         if (o->autotune && !o->exact) {
             uint64_t h = 1469598103934665603ULL;
             for (size_t k = 0; k < e->rp.size(); k += 1 + e->rp.size() / 4096) h = (h ^ (uint64_t)e->rp[k]) * 1099511628211ULL;

@@ -974,13 +974,15 @@ __global__ void k_ruiz_deltas(Dev d) {
 
 // A <- D_r A D_c on both stored orientations (value *= dr[i] * dc[j]).
 // A <- D_r A D_c on both stored orientations (value *= dr[i] * dc[j]).
-// Tiles of A^T first, then tiles of A; one wave per tile.
+// Tiles of A^T first, then tiles of A; one wave per tile.  ALL: every
+// instance, also those whose Ruiz passes have ended (the row weights).
+template <bool ALL>
 __global__ __launch_bounds__(kBlock) void k_ruiz_apply(Dev d) {
     const int g = blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (g >= d.ntT + d.ntA) return;
     const bool cols = g < d.ntT;
     const Tile tl = cols ? d.tT[g] : d.tA[g - d.ntT];
-    if (d.st[tl.inst].ruiz_done) return;
+    if (!ALL && d.st[tl.inst].ruiz_done) return;
     const Lanes L = lanes_of(tl);
     for (int base = tl.r0; base < tl.r1; base += L.step) {
         const int r = base + (L.lane >> L.lg);
@@ -992,6 +994,29 @@ __global__ __launch_bounds__(kBlock) void k_ruiz_apply(Dev d) {
             const double dr = d.rtmp[r];
             for (int p = d.rp[r] + L.gl; p < d.rp[r + 1]; p += L.G) d.av64[p] *= dr * d.ctmp[d.ci[p]];
         }
+    }
+}
+
+// Row weights after the Ruiz passes: w_i = (1 + beta_i)^(-k/8) (row_weight),
+// beta_i the largest magnitude of row i's finite original bounds (0 for a
+// free row).  rtmp <- w, ctmp <- 1, row_scale *= w; k_ruiz_apply<true> then
+// scales the rows of both stored orientations, and k_setup_rows scales the
+// original bounds by the final row_scale.  The relative primal residual
+// divides row i's violation by 1 + |a_i x|, about 1 + beta_i at an active
+// bound, so rows with small bounds must be met more tightly; with the L1-norm
+// step sizes, weighting row i by w_i multiplies its dual step by w_i.
+__global__ void k_row_weights(Dev d, int k8) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < d.m) {
+        const double lo = d.orow_lo[k], hi = d.orow_hi[k];
+        double beta = 0.0;
+        if (!isinf(lo)) beta = maxv(beta, fabs(lo));
+        if (!isinf(hi)) beta = maxv(beta, fabs(hi));
+        const double w = row_weight(beta, k8);
+        d.rtmp[k] = w;
+        d.row_scale[k] *= w;
+    } else if (k < d.m + d.n) {
+        d.ctmp[k - d.m] = 1.0;
     }
 }
 
@@ -1445,7 +1470,14 @@ void launch_ruiz_pass(const Dev& d, double tol, hipStream_t s) {
     hipLaunchKernelGGL(k_ruiz_decide, dim3(elem_grid(d.B)), b, 0, s, d, tol);
     hipLaunchKernelGGL(k_ruiz_deltas, dim3(elem_grid((int64_t)d.m + d.n)), b, 0, s, d);
     const int nt = d.ntT + d.ntA;
-    if (nt) hipLaunchKernelGGL(k_ruiz_apply, dim3((nt + kWaves - 1) / kWaves), b, 0, s, d);
+    if (nt) hipLaunchKernelGGL(k_ruiz_apply<false>, dim3((nt + kWaves - 1) / kWaves), b, 0, s, d);
+}
+
+void launch_row_weights(const Dev& d, int k8, hipStream_t s) {
+    const dim3 b(kBlock);
+    hipLaunchKernelGGL(k_row_weights, dim3(elem_grid((int64_t)d.m + d.n)), b, 0, s, d, k8);
+    const int nt = d.ntT + d.ntA;
+    if (nt) hipLaunchKernelGGL(k_ruiz_apply<true>, dim3((nt + kWaves - 1) / kWaves), b, 0, s, d);
 }
 
 void launch_setup_data(const Dev& d, hipStream_t s) {

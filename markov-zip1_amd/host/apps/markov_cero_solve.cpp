@@ -3,6 +3,7 @@
 //   markov-cero-solve MODEL.(mps|mclp) [--engine pdlp] [--backend cpu|gpu]
 //                     [--tolerance T] [--iteration-limit N] [--output f.json]
 //                     [--precision fp64|mixed|packed] [--order fast|reference] [--autotune]
+//                     [--restart-scheme reference|kkt|halpern] [--row-weighting 0..8]
 //
 // Mirrors the reference's pdlp branch (apps/markov_cero_solve.cpp:136-189):
 // --backend/--tolerance/--iteration-limit map into PdlpOptions (an iteration
@@ -63,7 +64,7 @@ int usage() {
     std::cerr << "usage: markov-cero-solve MODEL.mps|MODEL.mclp [--engine pdlp] [--backend cpu|gpu]\n"
                  "       [--tolerance T] [--iteration-limit N] [--output FILE]\n"
                  "       [--precision fp64|mixed|packed] [--order fast|reference] [--autotune]\n"
-                 "       [--restart-scheme reference|kkt|halpern]\n";
+                 "       [--restart-scheme reference|kkt|halpern] [--row-weighting 0..8]\n";
     return 4;
 }
 
@@ -76,6 +77,7 @@ int main(int argc, char** argv) {
     double tol = 1e-4;
     std::size_t iteration_limit = 10000;
     bool autotune = false;
+    int row_weighting = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -94,6 +96,7 @@ int main(int argc, char** argv) {
         else if (a == "--order") order = next();
         else if (a == "--autotune") autotune = true;
         else if (a == "--restart-scheme") scheme = next();
+        else if (a == "--row-weighting") row_weighting = std::atoi(next().c_str());
         else if (!a.empty() && a[0] == '-') return usage();
         else path = a;
     }
@@ -108,6 +111,10 @@ int main(int argc, char** argv) {
     }
     if (scheme != "reference" && scheme != "kkt" && scheme != "halpern") {
         std::cerr << "invalid restart scheme: " << scheme << "\n";
+        return 4;
+    }
+    if (row_weighting < 0 || row_weighting > 8) {
+        std::cerr << "row weighting must be in 0..8: " << row_weighting << "\n";
         return 4;
     }
     if (!(tol > 0.0)) {
@@ -135,6 +142,7 @@ int main(int argc, char** argv) {
                                                   : fo::Precision::fp64;
         o.reduction_order = order == "reference" ? fo::ReductionOrder::reference : fo::ReductionOrder::fast;
         o.autotune = autotune;
+        o.row_weighting = row_weighting;
         o.restart_scheme = scheme == "kkt"       ? fo::RestartScheme::kkt
                            : scheme == "halpern" ? fo::RestartScheme::halpern
                                                  : fo::RestartScheme::reference;

@@ -67,6 +67,7 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
     p.fp32_stall_checks = o.fp32_stall_checks;
     p.detect_infeasibility = o.detect_infeasibility != 0;
     p.infeasibility_tolerance = o.infeasibility_tolerance > 0.0 ? o.infeasibility_tolerance : 1e-8;
+    p.row_weighting = o.row_weighting;
     p.graph_chunk = o.graph_chunk;
     p.device = o.device;
     p.require_gpu = o.require_gpu != 0;
@@ -151,6 +152,7 @@ int mc_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* opt, mc_pdlp_result* r
         check_lp(lp);
         if (!opt || !res) throw std::invalid_argument("null argument");
         const auto o = to_options(*opt, lp->cols, lp->rows);
+        fo::detail::validate_options(o);
         fo::PdlpResult r;
         if (o.backend == fo::Backend::cpu) {
             r = fo::detail::solve_cpu(fo::detail::model_of(*lp), o, "cpu");
@@ -177,6 +179,7 @@ int mc_solve_pdlp_batch(const mc_lp* lps, int32_t count, const mc_pdlp_options* 
             totm += lp.rows;
         }
         const auto o = to_options(*opt, totn, totm);
+        fo::detail::validate_options(o);
         std::vector<fo::PdlpResult> out;
         auto host = [&](const char* label) {
             out.clear();

@@ -55,6 +55,12 @@ struct PdlpOptions {
     // never reports that status).
     bool detect_infeasibility{false};
     double infeasibility_tolerance{1e-8};
+    // k in 0..8: after the Ruiz passes, scale each row and its bounds by
+    // (1 + beta_i)^(-k/8), beta_i the largest magnitude of its finite bounds
+    // (0: the reference's scaling).  The relative primal residual divides a
+    // row's violation by 1 + |a_i x|, so rows with small bounds must be met
+    // more tightly; the weight raises their dual step.
+    int row_weighting{0};
     int graph_chunk{0};             // iterations per captured HIP graph (0 => 64, <0 => eager launches)
     int device{0};
     bool require_gpu{false};        // true: fail instead of the cpu_fallback path

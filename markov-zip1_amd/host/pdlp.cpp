@@ -42,7 +42,12 @@ model::Model detail::model_of(const mc_lp& lp) {
     return md;
 }
 
+void detail::validate_options(const PdlpOptions& opt) {
+    if (opt.row_weighting < 0 || opt.row_weighting > 8) throw std::invalid_argument("row_weighting must be in 0..8");
+}
+
 PdlpResult solve_pdlp(const model::Model& md, const PdlpOptions& opt) {
+    detail::validate_options(opt);
     if (opt.backend == Backend::cpu) return detail::solve_cpu(md, opt, "cpu");
     try {
         return detail::solve_gpu({detail::view_of(md)}, opt)[0];
@@ -53,6 +58,7 @@ PdlpResult solve_pdlp(const model::Model& md, const PdlpOptions& opt) {
 }
 
 std::vector<PdlpResult> solve_pdlp_batch(const std::vector<const model::Model*>& models, const PdlpOptions& opt) {
+    detail::validate_options(opt);
     std::vector<PdlpResult> out;
     auto host_all = [&](const char* label) {
         out.clear();

@@ -19,6 +19,48 @@ constexpr double kOpen = 1e300;  // bound sentinel of the reference (pdlp.cpp:47
 double lower_of(const model::Bound& b) { return b.kind == model::BoundKind::negative_infinity ? -kOpen : b.value; }
 double upper_of(const model::Bound& b) { return b.kind == model::BoundKind::positive_infinity ? kOpen : b.value; }
 
+// (1 + beta)^(-k/8) from the square roots of 1 + beta that the bits of k
+// select, multiplied in the order 8, 4, 2, 1 (each step correctly rounded,
+// so the HIP engine forms the same bits).
+double row_weight(double beta, int k) {
+    const double s0 = 1.0 + beta;
+    const double s1 = std::sqrt(s0), s2 = std::sqrt(s1), s3 = std::sqrt(s2);
+    double p = 1.0;
+    if (k & 8) p *= s0;
+    if (k & 4) p *= s1;
+    if (k & 2) p *= s2;
+    if (k & 1) p *= s3;
+    return 1.0 / p;
+}
+
+// PdlpOptions::row_weighting after the Ruiz passes: row i of the scaled model
+// s and its row scale are multiplied by row_weight(beta_i, k), beta_i the
+// largest magnitude of row i's finite bounds in the original model, and the
+// scaled row bounds become the original ones times the final row scale.
+void apply_row_weights(const model::Model& orig, model::Model& s, scale::RuizScalers& sc, int k) {
+    const std::size_t m = s.matrix.row_count;
+    if (sc.row_scale.size() != m) {  // no Ruiz passes: unit scalers
+        sc.row_scale.assign(m, 1.0);
+        sc.inv_row_scale.assign(m, 1.0);
+        sc.col_scale.assign(s.matrix.column_count, 1.0);
+        sc.inv_col_scale.assign(s.matrix.column_count, 1.0);
+    }
+    std::vector<double> w(m);
+    for (std::size_t i = 0; i < m; ++i) {
+        double beta = 0.0;
+        const auto& lo = orig.row_lower[i];
+        const auto& hi = orig.row_upper[i];
+        if (lo.kind == model::BoundKind::finite) beta = std::max(beta, std::abs(lo.value));
+        if (hi.kind == model::BoundKind::finite) beta = std::max(beta, std::abs(hi.value));
+        w[i] = row_weight(beta, k);
+        sc.row_scale[i] *= w[i];
+        sc.inv_row_scale[i] = 1.0 / sc.row_scale[i];
+        if (lo.kind == model::BoundKind::finite) s.row_lower[i].value = lo.value * sc.row_scale[i];
+        if (hi.kind == model::BoundKind::finite) s.row_upper[i].value = hi.value * sc.row_scale[i];
+    }
+    for (std::size_t k2 = 0; k2 < s.matrix.value.size(); ++k2) s.matrix.value[k2] *= w[s.matrix.row_index[k2]];
+}
+
 // Scaled problem data + iterate of one solve.
 struct Pdhg {
     const model::SparseMatrixCSC& A;
@@ -214,6 +256,8 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     model::Model s = orig;
     scale::RuizScalers sc;
     if (opt.ruiz_scaling) sc = scale::equilibrate_model(s, {opt.ruiz_iterations});
+    const bool scaled = opt.ruiz_scaling || opt.row_weighting > 0;
+    if (opt.row_weighting > 0) apply_row_weights(orig, s, sc, opt.row_weighting);
     Pdhg p(s);
     const double sense = s.objective_sense == model::ObjectiveSense::maximize ? -1.0 : 1.0;
     p.c.resize(n);
@@ -251,14 +295,14 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     p.eta = std::clamp(opt.step_size_reduction, 0.1, 0.99);
     p.x.resize(n);
     for (std::size_t j = 0; j < n; ++j) {
-        const double start = opt.warm_primal.size() == n && opt.ruiz_scaling ? opt.warm_primal[j] / sc.col_scale[j]
+        const double start = opt.warm_primal.size() == n && scaled ? opt.warm_primal[j] / sc.col_scale[j]
                              : opt.warm_primal.size() == n                 ? opt.warm_primal[j]
                                                                            : 0.0;
         p.x[j] = std::clamp(start, p.lo[j], p.hi[j]);
     }
     p.y.assign(m, 0.0);
     if (opt.warm_dual.size() == m)
-        for (std::size_t i = 0; i < m; ++i) p.y[i] = opt.ruiz_scaling ? opt.warm_dual[i] / sc.row_scale[i] : opt.warm_dual[i];
+        for (std::size_t i = 0; i < m; ++i) p.y[i] = scaled ? opt.warm_dual[i] / sc.row_scale[i] : opt.warm_dual[i];
     p.xbar = p.x;
     p.xavg = p.x;
     p.yavg = p.y;
@@ -282,12 +326,12 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
     auto evaluate_candidates = [&] {
         p.a_times(p.xavg, p.ax);
         p.at_times(p.yavg, p.aty);
-        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, opt.ruiz_scaling, xu, yu);
+        cur = evaluate(orig, p.xavg, p.yavg, p.ax, p.aty, sc, scaled, xu, yu);
         use_cur = false;
         if (!kkt) return;
         p.a_times(p.x, ax2);
         p.at_times(p.y, aty2);
-        const Residuals alt = evaluate(orig, p.x, p.y, ax2, aty2, sc, opt.ruiz_scaling, xu2, yu2);
+        const Residuals alt = evaluate(orig, p.x, p.y, ax2, aty2, sc, scaled, xu2, yu2);
         if (alt.score < cur.score) {
             use_cur = true;
             cur = alt;
@@ -359,7 +403,7 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
             std::vector<double> rdx(n), rdy(m);
             for (std::size_t j = 0; j < n; ++j) rdx[j] = xc[j] - xlast[j];
             for (std::size_t i = 0; i < m; ++i) rdy[i] = yc[i] - ylast[i];
-            if (ray_certificate(orig, p, rdx, rdy, sc, opt.ruiz_scaling, opt.infeasibility_tolerance)) {
+            if (ray_certificate(orig, p, rdx, rdy, sc, scaled, opt.infeasibility_tolerance)) {
                 infeasible = true;
                 break;
             }

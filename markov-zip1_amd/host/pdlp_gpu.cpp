@@ -329,6 +329,7 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     so.ruiz_iterations = (int32_t)opt.ruiz_iterations;
     so.exact = opt.reduction_order == ReductionOrder::reference ? 1 : 0;
     so.autotune = opt.autotune ? 1 : 0;
+    so.row_weighting = opt.row_weighting;
     check(mcp_setup(eng, &so), "mcp_setup");
 
     // warm start: vectors concatenated over all input LPs in input order

@@ -21,6 +21,8 @@ struct DeviceUnavailable : std::runtime_error {
 // size_t equals int64_t in width on the LP64 targets this builds for, so a
 // Model's arrays can be viewed as an mc_lp without copying.
 mc_lp view_of(const model::Model& md);
+// Rejects option values outside their documented ranges (std::invalid_argument).
+void validate_options(const PdlpOptions& opt);
 model::Model model_of(const mc_lp& lp);
 
 PdlpResult solve_cpu(const model::Model& md, const PdlpOptions& opt, const char* backend_label);

@@ -67,7 +67,7 @@ class Options(C.Structure):
         ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("restart_scheme", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
         ("fp32_stall_checks", C.c_int32), ("detect_infeasibility", C.c_int32),
-        ("infeasibility_tolerance", C.c_double),
+        ("infeasibility_tolerance", C.c_double), ("row_weighting", C.c_int32), ("pad3_", C.c_int32),
     ]
 
 
@@ -118,7 +118,8 @@ class EngProblem(C.Structure):
 
 
 class EngSetup(C.Structure):
-    _fields_ = [("ruiz", C.c_int32), ("ruiz_iterations", C.c_int32), ("exact", C.c_int32), ("autotune", C.c_int32)]
+    _fields_ = [("ruiz", C.c_int32), ("ruiz_iterations", C.c_int32), ("exact", C.c_int32), ("autotune", C.c_int32),
+                ("row_weighting", C.c_int32), ("pad_", C.c_int32)]
 
 
 class EngSolve(C.Structure):
@@ -609,8 +610,8 @@ class Engine:
         self.inst_row, self.inst_col = arrays["inst_row"], arrays["inst_col"]
         _check_eng(self.lib.mcp_load(self.h, C.byref(p)))
 
-    def setup(self, ruiz=True, ruiz_iterations=10, exact=False, autotune=False):
-        s = EngSetup(int(ruiz), int(ruiz_iterations), int(exact), int(autotune))
+    def setup(self, ruiz=True, ruiz_iterations=10, exact=False, autotune=False, row_weighting=0):
+        s = EngSetup(int(ruiz), int(ruiz_iterations), int(exact), int(autotune), int(row_weighting), 0)
         _check_eng(self.lib.mcp_setup(self.h, C.byref(s)))
 
     def spmv(self, v, transpose=False, fp32=False, exact=False, reps=1, packed=False):

@@ -385,6 +385,48 @@ static int or_movement_certifies(const mc_lp* lp, const double* xc, const double
  * unchanged.  `res->primal`/`res->dual` must hold
  * n / m doubles (or be NULL).  Timings are wall-clock of this restatement.
  * Optional `trace` records the per-check scalar sequence. */
+/* Row weight (1 + beta)^(-k/8), k in 0..8: the square roots s1 = (1+beta)^(1/2),
+ * s2 = s1^(1/2), s3 = s2^(1/2) that the bits of k select (8: 1 + beta itself),
+ * multiplied in the order 8, 4, 2, 1, then inverted.  Every step is correctly
+ * rounded, so the HIP engine reproduces it bit for bit. */
+double or_row_weight(double beta, int32_t k) {
+    const double s0 = 1.0 + beta;
+    const double s1 = sqrt(s0), s2 = sqrt(s1), s3 = sqrt(s2);
+    double p = 1.0;
+    if (k & 8) p *= s0;
+    if (k & 4) p *= s1;
+    if (k & 2) p *= s2;
+    if (k & 1) p *= s3;
+    return 1.0 / p;
+}
+
+/* Row weighting after the Ruiz passes (extension; no reference counterpart):
+ * beta_i is the largest magnitude of row i's finite ORIGINAL bounds (0 for a
+ * free row), w_i = or_row_weight(beta_i, k); row_scale[i] *= w_i, every
+ * nonzero of row i *= w_i, and the scaled row bounds become the original
+ * bounds times the final row_scale[i] (the order in which the engine forms
+ * them).  The reported residuals divide row i's violation by 1 + |a_i x|, about
+ * 1 + beta_i at an active bound: rows with small bounds must be met more
+ * tightly, and with L1-norm step sizes the weight multiplies row i's dual
+ * step by w_i. */
+void or_apply_row_weights(const mc_lp* lp, int32_t k, double* val, mc_bound* rl, mc_bound* ru,
+                          double* row_scale) {
+    const int64_t m = lp->rows, n = lp->cols;
+    double* w = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    for (int64_t i = 0; i < m; ++i) {
+        double beta = 0.0;
+        if (lp->row_lower[i].kind == MC_BOUND_FINITE) beta = or_max(beta, fabs(lp->row_lower[i].value));
+        if (lp->row_upper[i].kind == MC_BOUND_FINITE) beta = or_max(beta, fabs(lp->row_upper[i].value));
+        w[i] = or_row_weight(beta, k);
+        row_scale[i] *= w[i];
+        if (rl[i].kind == MC_BOUND_FINITE) rl[i].value = lp->row_lower[i].value * row_scale[i];
+        if (ru[i].kind == MC_BOUND_FINITE) ru[i].value = lp->row_upper[i].value * row_scale[i];
+    }
+    for (int64_t col = 0; col < n; ++col)
+        for (int64_t p = lp->column_start[col]; p < lp->column_start[col + 1]; ++p) val[p] *= w[lp->row_index[p]];
+    free(w);
+}
+
 int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res,
                   or_trace* trace) {
     const int64_t m = lp->rows, n = lp->cols;
@@ -426,6 +468,11 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
     if (o->ruiz_scaling)
         or_equilibrate_model(m, n, lp->column_start, lp->row_index, val, obj, rl, ru, vl, vu,
                              (int64_t)o->ruiz_iterations, 1e-3, row_scale, col_scale, &conv);
+    /* Row weights (extension, mc_pdlp_options::row_weighting = k): after the
+     * Ruiz passes, row i and its bounds are scaled by (1 + beta_i)^(-k/8). */
+    const int32_t scaled = o->ruiz_scaling || o->row_weighting > 0;
+    if (o->row_weighting > 0)
+        or_apply_row_weights(lp, o->row_weighting, val, rl, ru, row_scale);
     const int64_t* cs = lp->column_start;
     const int64_t* ri = lp->row_index;
     const double obj_sign = (lp->sense == MC_MAXIMIZE) ? -1.0 : 1.0;
@@ -508,13 +555,13 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
     do {                                                                                          \
         or_spmv(m, n, cs, ri, val, x_avg, Ax);                                                    \
         or_spmv_t(m, n, cs, ri, val, y_avg, At_y);                                                \
-        or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, o->ruiz_scaling, xo, yo,   \
+        or_residuals(lp, x_avg, y_avg, Ax, At_y, row_scale, col_scale, scaled, xo, yo,            \
                      &cur);                                                                       \
         use_cur = 0;                                                                              \
         if (kkt) {                                                                                \
             or_spmv(m, n, cs, ri, val, x, Ax2);                                                   \
             or_spmv_t(m, n, cs, ri, val, y, At_y2);                                               \
-            or_residuals(lp, x, y, Ax2, At_y2, row_scale, col_scale, o->ruiz_scaling, xo2, yo2,   \
+            or_residuals(lp, x, y, Ax2, At_y2, row_scale, col_scale, scaled, xo2, yo2,            \
                          &cur2);                                                                  \
             if (cur2.score < cur.score) {                                                         \
                 double* t_;                                                                       \
@@ -587,7 +634,7 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                 optimal = 1;
             } else if (o->detect_infeasibility && or_movement_certifies(lp, use_cur ? x : x_avg, use_cur ? y : y_avg,
                                                                         x_last, y_last, col_scale, row_scale,
-                                                                        o->ruiz_scaling, o->infeasibility_tolerance)) {
+                                                                        scaled, o->infeasibility_tolerance)) {
                 infeasible = 1;
                 break;
             } else if (!kkt) {

@@ -57,6 +57,10 @@ def oracle_lib():
         lib.or_ray_certificate.argtypes = [C.POINTER(mp.LP), P_f64, P_f64, C.c_double]
         lib.or_halpern_weight.argtypes = [C.c_double] * 5
         lib.or_halpern_weight.restype = C.c_double
+        lib.or_row_weight.argtypes = [C.c_double, C.c_int32]
+        lib.or_row_weight.restype = C.c_double
+        lib.or_apply_row_weights.argtypes = [C.POINTER(mp.LP), C.c_int32, P_f64, C.POINTER(mp.Bound),
+                                             C.POINTER(mp.Bound), P_f64]
         _or = lib
     return _or
 
@@ -195,6 +199,23 @@ def step_cap(lp: mp.LPData, iters: int) -> float:
     rn[rn < 1e-12] = 1.0
     return oracle_lib().or_step_cap(lp.rows, lp.cols, _p(lp.column_start, C.c_int64), _p(lp.row_index, C.c_int64),
                                     _p(lp.value, C.c_double), _p(rn, C.c_double), _p(cn, C.c_double), iters)
+
+
+def row_weight(beta: float, k: int) -> float:
+    """(1 + beta)^(-k/8) as the oracle forms it (nested square roots)."""
+    return oracle_lib().or_row_weight(float(beta), int(k))
+
+
+def weighted_scaling(lp: mp.LPData, k: int, ruiz=True) -> dict:
+    """equilibrate() (or unit scalers without Ruiz) followed by the oracle's
+    row weighting of row_weighting = k: scaled copy of lp and its scalers."""
+    eq = equilibrate(lp) if ruiz else {"lp": lp.copy(), "row_scale": np.ones(lp.rows), "col_scale": np.ones(lp.cols)}
+    q = eq["lp"]
+    rs = np.ascontiguousarray(eq["row_scale"], dtype=np.float64).copy()
+    s = lp.struct()
+    oracle_lib().or_apply_row_weights(C.byref(s), int(k), _p(q.value, C.c_double), _p(q.row_lower, mp.Bound),
+                                      _p(q.row_upper, mp.Bound), _p(rs, C.c_double))
+    return {"lp": q, "row_scale": rs, "col_scale": eq["col_scale"]}
 
 
 def halpern_weight(omega, dx, dy, primal, dual) -> float:

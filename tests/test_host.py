@@ -67,6 +67,10 @@ CASES = [
      dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, max_iterations=20000)),
     (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
      dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, max_iterations=3000)),
+    (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, row_weighting=3, max_iterations=3000)),
+    (lambda: mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, row_weighting=5, ruiz_scaling=0, max_iterations=3000)),
 ]
 
 
@@ -348,3 +352,23 @@ def test_rows_are_local_matches_a_dense_span():
                        value=np.ones(len(ri)), objective=np.zeros(8), var_lower=np.zeros(8),
                        var_upper=np.ones(8), row_lower=np.zeros(4), row_upper=np.ones(4))
         assert mp.rows_are_local(lp) == want  # mean span (3 + s) / 32
+
+
+def test_row_weighting_out_of_range_is_an_error():
+    lp = tiny()
+    for k in (-1, 9):
+        with pytest.raises(RuntimeError, match="row_weighting must be in 0..8"):
+            mp.solve(lp, tolerance=1e-6, row_weighting=k)
+    p = subprocess.run([mp.CLI, os.path.join(GOLDEN, "tiny_simplex.mps"), "--row-weighting", "9"],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 4 and "row weighting must be in 0..8" in p.stderr
+
+
+def test_cli_row_weighting_flag():
+    """--row-weighting 3 with the Halpern scheme solves Netlib adlittle (optimum
+    225494.96316238, scripts/run_netlib.py) to 1e-6, verified."""
+    mps = os.path.join(GOLDEN, "netlib", "adlittle.mps")
+    code, js, err = run_cli([mps, "--engine", "pdlp", "--tolerance", "1e-6", "--restart-scheme", "halpern",
+                             "--row-weighting", "3"])
+    assert code == 0 and js["status"] == "Optimal" and js["verified"]
+    assert abs(js["objective"] - 225494.96316238) <= 1e-6 * 225494.96316238
