@@ -73,6 +73,11 @@ DTYPES = {"fp64": "fp64", "mixed": "fp32 then packed fp32+8-bit values, fp64 sum
           "packed": "packed fp32+8-bit values, fp64 sums"}
 
 
+def max_drop(args, default):
+    """--primal-weight-max-drop, or the workload's default."""
+    return default if args.primal_weight_max_drop is None else args.primal_weight_max_drop
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -104,6 +109,9 @@ def main():
     ap.add_argument("--row-weighting", type=int, default=3, choices=range(9), metavar="0..8",
                     help="rows scaled by (1 + |bound|)^(-k/8) after Ruiz in the measured solves (0: the "
                          "reference's scaling; the plain FP64 comparison always uses 0)")
+    ap.add_argument("--primal-weight-max-drop", type=int, default=None, metavar="K",
+                    help="Halpern scheme: a restart lowers the primal weight by at most K (a power of two; 0: no "
+                         "limit). Default 2 for the batch of small LPs, 0 for the single LPs and the warm family")
     ap.add_argument("--plain-scheme", default="kkt", choices=sorted(SCHEMES),
                     help="restart scheme of the plain FP64 comparison (classic PDLP restarts to the average)")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -149,7 +157,7 @@ def main():
     prec = PRECISIONS[args.precision]
     common = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
                   graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1,
-                  row_weighting=args.row_weighting)
+                  row_weighting=args.row_weighting, primal_weight_max_drop=max_drop(args, 0))
     opts = mp.options(precision=prec, max_iterations=args.max_iterations, **common)
 
     def run(o):
@@ -248,7 +256,8 @@ def main():
         "dtype": DTYPES[args.precision], "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
                    "precision": args.precision, "restart_scheme": args.scheme,
-                   "row_weighting": args.row_weighting, "parallelism": "single GPU"},
+                   "row_weighting": args.row_weighting, "primal_weight_max_drop": max_drop(args, 0),
+                   "parallelism": "single GPU"},
         "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup_iterations": args.warmup_iterations, "budget_s": args.budget_s,
         "walls_ms": [round(w, 1) for w in walls],
@@ -468,7 +477,7 @@ def bench_batch(args):
     t0 = time.perf_counter()
     prec = PRECISIONS[args.precision]
     res = mp.solve_batch(lps, precision=prec, restart_scheme=SCHEMES[args.scheme], row_weighting=args.row_weighting,
-                         **kw)
+                         primal_weight_max_drop=max_drop(args, 2), **kw)
     wall = (time.perf_counter() - t0) * 1e3
     its = np.array([r["iterations"] for r in res])
     ok = sum(r["status"] == "optimal" for r in res)
@@ -540,7 +549,7 @@ def bench_batch(args):
     _line(wall, WORKLOADS["batch"]["name"],
           {"instances": len(lps), "sum_rows": int(sum(l.rows for l in lps)), "sum_nnz": int(sum(l.nnz for l in lps)),
            "tolerance": tol, "precision": args.precision, "restart_scheme": args.scheme,
-           "row_weighting": args.row_weighting,
+           "row_weighting": args.row_weighting, "primal_weight_max_drop": max_drop(args, 2),
            "plain_restart_scheme": args.plain_scheme},
           {"optimal": int(ok), "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
            "kkt_fp64_rescored_worst_of_sample": float(f"{worst:.4e}"),
@@ -572,7 +581,7 @@ def bench_warm(args):
         f"({base.nnz} nnz) in {time.time() - t:.1f}s")
     kw = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
               max_iterations=args.max_iterations, precision=mp.PRECISION_MIXED, graph_chunk=args.graph_chunk,
-              row_weighting=args.row_weighting)
+              row_weighting=args.row_weighting, primal_weight_max_drop=max_drop(args, 0))
     t0 = time.perf_counter()
     sols = []
     for k in range(0, ntr, 128):
@@ -610,7 +619,7 @@ def bench_warm(args):
     _line(warm_ms, WORKLOADS["warm"]["name"].format(rows=base.rows, cols=base.cols),
           {"train": ntr, "test": nte, "m": base.rows, "n": base.cols, "nnz": base.nnz, "tolerance": tol,
            "mlp": "256-256-256-256-64 BF16 MFMA", "precision": "mixed", "restart_scheme": args.scheme,
-           "row_weighting": args.row_weighting},
+           "row_weighting": args.row_weighting, "primal_weight_max_drop": max_drop(args, 0)},
           {"cold_ms": round(cold_ms, 1), "warm_ms": round(warm_ms, 1),
            "iterations_cold_mean": float(ci.mean()), "iterations_warm_mean": float(wi.mean()),
            "iterations_cold_max": int(ci.max()), "iterations_warm_max": int(wi.max()),

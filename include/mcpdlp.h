@@ -152,7 +152,8 @@ typedef struct mc_pdlp_options {
                                          (1 + beta_i)^(-k/8), beta_i the largest magnitude of its finite original
                                          bounds; the reported residuals and vectors stay those of the original
                                          model (0: the reference's scaling) */
-    int32_t pad3_;
+    int32_t primal_weight_max_drop;   /* Halpern scheme: a restart lowers the primal weight by at most this factor
+                                         (0: no limit; else a power of two in 2..2^20) */
 } mc_pdlp_options;
 
 /* PdlpStatus (reference: pdlp.hpp:43). */

@@ -114,6 +114,8 @@ typedef struct mcp_solve_opts {
                                          estimate L = ||D_r^-1/2 A D_c^-1/2|| per instance, and the
                                          step factor eta is capped at 0.95 / L instead of 0.99
                                          (0 => 0.99; reference option pdlp.hpp:25, unread there) */
+    int32_t primal_weight_max_drop;   /* mc_pdlp_options::primal_weight_max_drop */
+    int32_t pad3_;
 } mcp_solve_opts;
 
 typedef struct mcp_solve_stats {

@@ -91,7 +91,7 @@ struct IterOpts {
                                  // kernel sequence, so a change must drop the cached graphs)
     int32_t packed_next;         // the FP32-SpMV phase hands over to the packed stream (else FP64)
     int32_t packed_floor;        // the check also scores the average with the packed values
-    int32_t pad_;
+    int32_t omega_max_drop;      // Halpern scheme: a restart lowers omega by at most this factor (0: none)
 };
 
 // Map (workgroup, wave) to a tile so that each XCD works on a contiguous

@@ -466,6 +466,7 @@ void make_opts(const mcp_solve_opts* o, mcp::IterOpts& io) {
     io.stall_checks = o->fp32_stall_checks > 0 ? o->fp32_stall_checks : 50;
     io.detect = o->detect_infeasibility != 0 ? 1 : 0;
     io.adaptive_step = o->adaptive_step_size != 0 ? 1 : 0;
+    io.omega_max_drop = o->primal_weight_max_drop;
     io.infeas_tol = o->infeasibility_tolerance > 0.0 ? o->infeasibility_tolerance : 1e-8;
     const double tol = std::max({o->primal_tolerance, o->dual_tolerance, o->gap_tolerance});
     // 0 => no score threshold: FP32 runs until it converges or stalls
@@ -735,6 +736,10 @@ int mcp_solve(mcp_engine* e, const mcp_solve_opts* o, mcp_solve_stats* stats) {
         hipStream_t s = e->stream;
         require(o->restart_scheme >= MC_SCHEME_REFERENCE && o->restart_scheme <= MC_SCHEME_HALPERN,
                 "unknown restart scheme");
+        require(o->primal_weight_max_drop == 0 ||
+                    (o->primal_weight_max_drop >= 2 && o->primal_weight_max_drop <= (1 << 20) &&
+                     (o->primal_weight_max_drop & (o->primal_weight_max_drop - 1)) == 0),
+                "primal_weight_max_drop must be 0 or a power of two in 2..2^20");
         mcp::IterOpts io{};
         make_opts(o, io);
         io.packed_next = (o->precision == MC_PRECISION_MIXED && d.cix) ? 1 : 0;

@@ -649,6 +649,8 @@ __global__ __launch_bounds__(kFinal) void k_check_finalize(Dev d, IterOpts o, in
                         if (r.primal > 100.0 * r.dual && w < s.omega) w = s.omega;
                         if (r.dual > 100.0 * r.primal && w > s.omega) w = s.omega;
                     }
+                    // at most a fall by omega_max_drop (a power of two: exact)
+                    if (o.omega_max_drop > 0) w = maxv(w, s.omega / (double)o.omega_max_drop);
                 }
                 if (upd) s.omega = clampv(w, 1e-6, 1e6);
             }

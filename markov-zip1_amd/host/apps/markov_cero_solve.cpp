@@ -4,6 +4,7 @@
 //                     [--tolerance T] [--iteration-limit N] [--output f.json]
 //                     [--precision fp64|mixed|packed] [--order fast|reference] [--autotune]
 //                     [--restart-scheme reference|kkt|halpern] [--row-weighting 0..8]
+//                     [--primal-weight-max-drop K]
 //
 // Mirrors the reference's pdlp branch (apps/markov_cero_solve.cpp:136-189):
 // --backend/--tolerance/--iteration-limit map into PdlpOptions (an iteration
@@ -64,7 +65,8 @@ int usage() {
     std::cerr << "usage: markov-cero-solve MODEL.mps|MODEL.mclp [--engine pdlp] [--backend cpu|gpu]\n"
                  "       [--tolerance T] [--iteration-limit N] [--output FILE]\n"
                  "       [--precision fp64|mixed|packed] [--order fast|reference] [--autotune]\n"
-                 "       [--restart-scheme reference|kkt|halpern] [--row-weighting 0..8]\n";
+                 "       [--restart-scheme reference|kkt|halpern] [--row-weighting 0..8]\n"
+                 "       [--primal-weight-max-drop K]\n";
     return 4;
 }
 
@@ -78,6 +80,7 @@ int main(int argc, char** argv) {
     std::size_t iteration_limit = 10000;
     bool autotune = false;
     int row_weighting = 0;
+    int max_drop = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -97,6 +100,7 @@ int main(int argc, char** argv) {
         else if (a == "--autotune") autotune = true;
         else if (a == "--restart-scheme") scheme = next();
         else if (a == "--row-weighting") row_weighting = std::atoi(next().c_str());
+        else if (a == "--primal-weight-max-drop") max_drop = std::atoi(next().c_str());
         else if (!a.empty() && a[0] == '-') return usage();
         else path = a;
     }
@@ -115,6 +119,10 @@ int main(int argc, char** argv) {
     }
     if (row_weighting < 0 || row_weighting > 8) {
         std::cerr << "row weighting must be in 0..8: " << row_weighting << "\n";
+        return 4;
+    }
+    if (max_drop != 0 && (max_drop < 2 || max_drop > (1 << 20) || (max_drop & (max_drop - 1)) != 0)) {
+        std::cerr << "primal weight max drop must be 0 or a power of two in 2..2^20: " << max_drop << "\n";
         return 4;
     }
     if (!(tol > 0.0)) {
@@ -143,6 +151,7 @@ int main(int argc, char** argv) {
         o.reduction_order = order == "reference" ? fo::ReductionOrder::reference : fo::ReductionOrder::fast;
         o.autotune = autotune;
         o.row_weighting = row_weighting;
+        o.primal_weight_max_drop = max_drop;
         o.restart_scheme = scheme == "kkt"       ? fo::RestartScheme::kkt
                            : scheme == "halpern" ? fo::RestartScheme::halpern
                                                  : fo::RestartScheme::reference;

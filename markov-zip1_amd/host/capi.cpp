@@ -68,6 +68,7 @@ fo::PdlpOptions to_options(const mc_pdlp_options& o, int64_t n, int64_t m) {
     p.detect_infeasibility = o.detect_infeasibility != 0;
     p.infeasibility_tolerance = o.infeasibility_tolerance > 0.0 ? o.infeasibility_tolerance : 1e-8;
     p.row_weighting = o.row_weighting;
+    p.primal_weight_max_drop = o.primal_weight_max_drop;
     p.graph_chunk = o.graph_chunk;
     p.device = o.device;
     p.require_gpu = o.require_gpu != 0;

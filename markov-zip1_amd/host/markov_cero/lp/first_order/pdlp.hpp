@@ -61,6 +61,9 @@ struct PdlpOptions {
     // row's violation by 1 + |a_i x|, so rows with small bounds must be met
     // more tightly; the weight raises their dual step.
     int row_weighting{0};
+    // Halpern scheme: a restart lowers the primal weight by at most this
+    // factor (0: no limit; else a power of two in 2..2^20).
+    int primal_weight_max_drop{0};
     int graph_chunk{0};             // iterations per captured HIP graph (0 => 64, <0 => eager launches)
     int device{0};
     bool require_gpu{false};        // true: fail instead of the cpu_fallback path

@@ -44,6 +44,9 @@ model::Model detail::model_of(const mc_lp& lp) {
 
 void detail::validate_options(const PdlpOptions& opt) {
     if (opt.row_weighting < 0 || opt.row_weighting > 8) throw std::invalid_argument("row_weighting must be in 0..8");
+    const int d = opt.primal_weight_max_drop;
+    if (d != 0 && (d < 2 || d > (1 << 20) || (d & (d - 1)) != 0))
+        throw std::invalid_argument("primal_weight_max_drop must be 0 or a power of two in 2..2^20");
 }
 
 PdlpResult solve_pdlp(const model::Model& md, const PdlpOptions& opt) {

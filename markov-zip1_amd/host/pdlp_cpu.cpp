@@ -461,6 +461,8 @@ PdlpResult solve_cpu(const model::Model& orig, const PdlpOptions& opt, const cha
                         if (cur.primal > 100.0 * cur.dual && w < p.omega) w = p.omega;
                         if (cur.dual > 100.0 * cur.primal && w > p.omega) w = p.omega;
                     }
+                    // at most a fall by primal_weight_max_drop (a power of two: exact)
+                    if (opt.primal_weight_max_drop > 0) w = std::max(w, p.omega / opt.primal_weight_max_drop);
                 }
                 if (upd) p.omega = std::clamp(w, 1e-6, 1e6);
             }

@@ -369,6 +369,7 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     o.detect_infeasibility = opt.detect_infeasibility ? 1 : 0;
     o.infeasibility_tolerance = opt.infeasibility_tolerance;
     o.power_iterations = opt.power_iterations;
+    o.primal_weight_max_drop = opt.primal_weight_max_drop;
     o.reduction_order = so.exact ? MC_ORDER_REFERENCE : MC_ORDER_FAST;
     o.graph_chunk = opt.graph_chunk;
     o.restart_scheme = opt.restart_scheme == RestartScheme::kkt       ? MC_SCHEME_KKT

@@ -67,7 +67,8 @@ class Options(C.Structure):
         ("graph_chunk", C.c_int32), ("device", C.c_int32), ("require_gpu", C.c_int32), ("restart_scheme", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
         ("fp32_stall_checks", C.c_int32), ("detect_infeasibility", C.c_int32),
-        ("infeasibility_tolerance", C.c_double), ("row_weighting", C.c_int32), ("pad3_", C.c_int32),
+        ("infeasibility_tolerance", C.c_double), ("row_weighting", C.c_int32),
+        ("primal_weight_max_drop", C.c_int32),
     ]
 
 
@@ -136,7 +137,7 @@ class EngSolve(C.Structure):
         ("restart_scheme", C.c_int32), ("fp32_stall_checks", C.c_int32),
         ("warm_primal", P_f64), ("warm_dual", P_f64),
         ("detect_infeasibility", C.c_int32), ("pad2_", C.c_int32), ("infeasibility_tolerance", C.c_double),
-        ("power_iterations", C.c_uint64),
+        ("power_iterations", C.c_uint64), ("primal_weight_max_drop", C.c_int32), ("pad3_", C.c_int32),
     ]
 
 

@@ -345,7 +345,7 @@ int or_ray_certificate(const mc_lp* lp, const double* dx, const double* dy, doub
  * move against a residual imbalance of 100x or more, clamped to [1e-6, 1e6].
  * Every power is a chain of correctly rounded square roots, so a device
  * evaluation in the same order is bit-identical. */
-double or_halpern_weight(double omega, double dX, double dY, double primal, double dual) {
+double or_halpern_weight(double omega, double dX, double dY, double primal, double dual, int32_t max_drop) {
     double wmove = omega;
     if (dX > 1e-10 && dY > 1e-10) {
         const double ratio = dY / dX;
@@ -360,6 +360,8 @@ double or_halpern_weight(double omega, double dX, double dY, double primal, doub
         if (primal > 100.0 * dual && w < omega) w = omega;
         if (dual > 100.0 * primal && w > omega) w = omega;
     }
+    /* extension: at most a fall by max_drop (0: none; a power of two, so exact) */
+    if (max_drop > 0) w = or_max(w, omega / (double)max_drop);
     return or_clamp(w, 1e-6, 1e6);
 }
 
@@ -680,7 +682,7 @@ int or_solve_pdlp(const mc_lp* lp, const mc_pdlp_options* o, mc_pdlp_result* res
                         for (int64_t i = 0; i < m; ++i) dys += (yc[i] - y_last[i]) * (yc[i] - y_last[i]);
                         const double dX = sqrt(dxs), dY = sqrt(dys);
                         if (halpern) {
-                            omega = or_halpern_weight(omega, dX, dY, cur.primal, cur.dual);
+                            omega = or_halpern_weight(omega, dX, dY, cur.primal, cur.dual, o->primal_weight_max_drop);
                             OR_UPDATE_STEPS();
                         } else if (dX > 1e-10 && dY > 1e-10) {
                             const double th = o->primal_weight_smoothing;

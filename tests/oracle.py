@@ -55,7 +55,7 @@ def oracle_lib():
         lib.or_step_cap.argtypes = [C.c_int64, C.c_int64, P_i64, P_i64, P_f64, P_f64, P_f64, C.c_int64]
         lib.or_step_cap.restype = C.c_double
         lib.or_ray_certificate.argtypes = [C.POINTER(mp.LP), P_f64, P_f64, C.c_double]
-        lib.or_halpern_weight.argtypes = [C.c_double] * 5
+        lib.or_halpern_weight.argtypes = [C.c_double] * 5 + [C.c_int32]
         lib.or_halpern_weight.restype = C.c_double
         lib.or_row_weight.argtypes = [C.c_double, C.c_int32]
         lib.or_row_weight.restype = C.c_double
@@ -218,9 +218,9 @@ def weighted_scaling(lp: mp.LPData, k: int, ruiz=True) -> dict:
     return {"lp": q, "row_scale": rs, "col_scale": eq["col_scale"]}
 
 
-def halpern_weight(omega, dx, dy, primal, dual) -> float:
+def halpern_weight(omega, dx, dy, primal, dual, max_drop=0) -> float:
     """The Halpern scheme's primal weight after a restart (oracle)."""
-    return oracle_lib().or_halpern_weight(omega, dx, dy, primal, dual)
+    return oracle_lib().or_halpern_weight(omega, dx, dy, primal, dual, int(max_drop))
 
 
 def ray_certificate(lp: mp.LPData, dx, dy, tol=1e-8) -> int:

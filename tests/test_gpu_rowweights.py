@@ -1,4 +1,6 @@
-"""Row weighting (extension, mc_pdlp_options::row_weighting = k): after the
+"""Row weighting and the primal-weight drop limit (extensions).
+
+Row weighting (mc_pdlp_options::row_weighting = k): after the
 Ruiz passes (reference src/scale/ruiz_scaling.cpp:152-247) row i of A and its
 bounds are scaled by (1 + beta_i)^(-k/8), beta_i the largest magnitude of its
 finite original bounds.  The solve (reference pdlp.cpp:139-276) then runs on
@@ -14,6 +16,10 @@ Bars:
   uniform LP, whose objective cancels terms 1e4 times larger); the pair
   re-scored by the oracle's FP64 residuals is within 1.5x of it, the
   objective within 1e-6 x (1 + |p*|) of the planted optimum.
+
+The drop limit (primal_weight_max_drop = K, Halpern scheme): a restart lowers
+the primal weight by at most K.  Bars: reference order bit-identical to the
+oracle, single and batched; fast order reaches the planted optimum.
 """
 import numpy as np
 import pytest
@@ -101,3 +107,34 @@ def test_row_weighted_mixed_solve_reaches_the_planted_optimum(maker, tol):
     assert r["backend_actually_used"] == "hip" and r["status"] == "optimal"
     assert orc.kkt(lp, r["primal"], r["dual"])["score"] <= 1.5 * tol
     assert abs(r["objective"] - lp.planted_objective) <= 1e-6 * (1.0 + abs(lp.planted_objective))
+
+
+@pytest.mark.parametrize("maker", [uniform, zipf_mixed, ranged])
+def test_max_drop_reference_order_bit_exact(maker):
+    lp = maker()
+    kw = dict(tolerance=1e-6, max_iterations=5000, restart_scheme=mp.SCHEME_HALPERN, row_weighting=3,
+              primal_weight_max_drop=2)
+    print("max drop solve", maker.__name__, flush=True)
+    want = orc.solve(lp, **kw)
+    got = mp.solve(lp, backend=mp.BACKEND_GPU, require_gpu=1, reduction_order=mp.ORDER_REFERENCE, **kw)
+    assert got["status"] == want["status"] and got["iterations"] == want["iterations"]
+    assert got["objective"] == want["objective"]
+    assert np.array_equal(got["primal"], want["primal"]) and np.array_equal(got["dual"], want["dual"])
+
+
+def test_max_drop_batch_reference_order_and_fast_solve():
+    lps = [uniform(seed=300 + k, m=150 + 41 * k, n=380 + 95 * k) for k in range(5)] + [zipf_mixed(seed=78, m=140, n=300)]
+    kw = dict(tolerance=1e-6, max_iterations=5000, restart_scheme=mp.SCHEME_HALPERN, row_weighting=3,
+              primal_weight_max_drop=2)
+    got = mp.solve_batch(lps, backend=mp.BACKEND_GPU, require_gpu=1, reduction_order=mp.ORDER_REFERENCE, **kw)
+    for lp, g in zip(lps, got):
+        w = orc.solve(lp, **kw)
+        assert g["iterations"] == w["iterations"] and g["objective"] == w["objective"]
+        assert np.array_equal(g["primal"], w["primal"]) and np.array_equal(g["dual"], w["dual"])
+    tol = 1e-8  # the uniform LPs' objectives cancel terms 1e4 times larger
+    fast = mp.solve_batch(lps, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_MIXED,
+                          **dict(kw, tolerance=tol, max_iterations=400000))
+    for lp, r in zip(lps, fast):
+        assert r["status"] == "optimal"
+        assert orc.kkt(lp, r["primal"], r["dual"])["score"] <= 1.5 * tol
+        assert abs(r["objective"] - lp.planted_objective) <= 1e-6 * (1.0 + abs(lp.planted_objective))

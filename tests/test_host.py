@@ -71,6 +71,9 @@ CASES = [
      dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, row_weighting=3, max_iterations=3000)),
     (lambda: mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11),
      dict(tolerance=1e-6, restart_scheme=mp.SCHEME_KKT, row_weighting=5, ruiz_scaling=0, max_iterations=3000)),
+    (lambda: mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=12),
+     dict(tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, row_weighting=3, primal_weight_max_drop=2,
+          max_iterations=20000)),
 ]
 
 
@@ -372,3 +375,14 @@ def test_cli_row_weighting_flag():
                              "--row-weighting", "3"])
     assert code == 0 and js["status"] == "Optimal" and js["verified"]
     assert abs(js["objective"] - 225494.96316238) <= 1e-6 * 225494.96316238
+
+
+def test_primal_weight_max_drop_must_be_a_power_of_two():
+    lp = tiny()
+    for k in (1, 3, -2, 1 << 21):
+        with pytest.raises(RuntimeError, match="primal_weight_max_drop must be 0 or a power of two"):
+            mp.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, primal_weight_max_drop=k)
+    assert mp.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, primal_weight_max_drop=4)["status"] == "optimal"
+    p = subprocess.run([mp.CLI, os.path.join(GOLDEN, "tiny_simplex.mps"), "--primal-weight-max-drop", "3"],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 4 and "power of two" in p.stderr

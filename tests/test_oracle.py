@@ -490,3 +490,32 @@ def test_row_weighted_solve_reaches_planted_optimum(scheme, ruiz):
         np.testing.assert_allclose([r["primal_infeasibility"], r["dual_infeasibility"], r["duality_gap"]],
                                    [k["primal"], k["dual"], k["gap"]], rtol=1e-4, atol=1e-12)
         assert abs(r["objective"] - lp.planted_objective) <= 1e-5 * (1 + abs(lp.planted_objective))
+
+
+# ---- primal-weight drop limit (extension: primal_weight_max_drop) ----------
+def test_halpern_weight_max_drop():
+    """primal_weight_max_drop = K: a restart lowers the weight by at most K
+    (to exactly omega / K, K a power of two); a smaller fall and any rise are
+    what the rule gives without the limit."""
+    omega = 0.37
+    # movement ratio 1e-6 omega with balanced residuals: the rule falls ~3e4-fold
+    free = orc.halpern_weight(omega, 1.0, 1e-6 * omega, 1e-5, 1e-5)
+    assert free < omega / 1024
+    assert orc.halpern_weight(omega, 1.0, 1e-6 * omega, 1e-5, 1e-5, 2) == omega / 2
+    assert orc.halpern_weight(omega, 1.0, 1e-6 * omega, 1e-5, 1e-5, 1024) == omega / 1024
+    for dy in (0.9 * omega, 10.0 * omega):
+        assert orc.halpern_weight(omega, 1.0, dy, 1e-5, 1e-5, 2) == orc.halpern_weight(omega, 1.0, dy, 1e-5, 1e-5)
+    # the guard still holds the weight while the primal residual is 100x the dual
+    assert orc.halpern_weight(omega, 1.0, 1e-6 * omega, 1e-3, 1e-6, 2) == omega
+
+
+def test_max_drop_solve_reaches_planted_optimum():
+    """Halpern solves with the drop limit (and row weighting) reach the planted
+    optimum of a uniform and a Zipf LP; re-scored pairs are within tolerance."""
+    for lp in (mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11),
+               mp.generate(mp.FAMILY_ZIPF_MIXED, rows=300, cols=700, nnz_max=40, seed=31)):
+        r = orc.solve(lp, tolerance=1e-6, restart_scheme=mp.SCHEME_HALPERN, row_weighting=3,
+                      primal_weight_max_drop=2, max_iterations=200000)
+        assert r["status"] == "optimal"
+        assert orc.kkt(lp, r["primal"], r["dual"])["score"] <= 1e-6 * 1.01
+        assert abs(r["objective"] - lp.planted_objective) <= 1e-5 * (1 + abs(lp.planted_objective))
