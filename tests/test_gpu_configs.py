@@ -8,19 +8,23 @@ src/scale/ruiz_scaling.cpp:152-247).
   and columns relabelled as the solve relabels them): reference order
   bit-identical, fast order within 1e-13, the packed stream within 2^-30 and
   FP32 within 2e-6 of the row's absolute sum; 120 iterations in reference order bit-identical to the oracle
-  (reference and Halpern schemes); and the benchmark's own solve (mixed
-  precision, Halpern scheme, autotune on) to 1e-6, re-scored by the oracle's
-  FP64 residuals and held to 1e-6 x (1 + |p*|) of the planted optimum;
+  (reference and Halpern schemes, and Halpern with the benchmark's row
+  weighting); and the benchmark's own solve (mixed precision, Halpern scheme,
+  row weighting k = 3, autotune on) to 1e-6, re-scored by the oracle's FP64
+  residuals and held to 1e-6 x (1 + |p*|) of the planted optimum;
 * 256-LP uneven batch (m log-uniform in [500, 20000], n = 2.5 m, 21.1M
   nonzeros): 200 iterations in reference order, every instance bit-identical
-  to its own oracle solve; then the batched fast-order solve to 1e-6 with
-  every instance re-scored and held to its planted optimum;
+  to its own oracle solve (default options, and the benchmark's Halpern
+  scheme with the primal-weight drop limit); then the
+  batched fast-order solve with the benchmark's options to 1e-6 with every
+  instance re-scored and held to its planted optimum;
 * learned warm-start family on the 20,000 x 50,000 structure (6 nonzeros per
   column): a small train/test split; every warm-started instance converges,
   is re-scored by the oracle and matches its planted optimum, and the warm
   start does not cost iterations on average;
 * 48-period block-angular LP (2.12M x 4.8M, 48M nonzeros): device Ruiz and
-  SpMV parity as above, 80 iterations in reference order bit-identical.
+  SpMV parity as above, 80 iterations in reference order bit-identical
+  (reference scheme; Halpern with row weighting).
 """
 import numpy as np
 import pytest
@@ -82,8 +86,14 @@ def check_ruiz_and_spmv(lp):
         e.close()
 
 
-def check_bit_exact(lp, its, scheme):
-    kw = dict(tolerance=1e-6, max_iterations=its, restart_scheme=scheme)
+# the benchmark's options beyond the scheme: row weighting k = 3 on the single
+# LPs, the primal-weight drop limit 2 (and no row weighting) on the batch
+BENCH_SINGLE = dict(row_weighting=3)
+BENCH_BATCH = dict(primal_weight_max_drop=2)
+
+
+def check_bit_exact(lp, its, scheme, **extra):
+    kw = dict(tolerance=1e-6, max_iterations=its, restart_scheme=scheme, **extra)
     want = orc.solve(lp, **kw)
     got = mp.solve(lp, backend=mp.BACKEND_GPU, require_gpu=1, reduction_order=mp.ORDER_REFERENCE, **kw)
     assert got["backend_actually_used"] == "hip"
@@ -117,18 +127,20 @@ def test_large_ruiz_and_spmv(large):
     check_ruiz_and_spmv(large)
 
 
-@pytest.mark.parametrize("scheme", [mp.SCHEME_REFERENCE, mp.SCHEME_HALPERN], ids=["reference", "halpern"])
-def test_large_reference_order_120_iterations(large, scheme):
-    print("large reference order, scheme", scheme, flush=True)
-    check_bit_exact(large, 120, scheme)
+@pytest.mark.parametrize("scheme,extra", [(mp.SCHEME_REFERENCE, {}), (mp.SCHEME_HALPERN, {}),
+                                          (mp.SCHEME_HALPERN, BENCH_SINGLE)], ids=["reference", "halpern", "bench"])
+def test_large_reference_order_120_iterations(large, scheme, extra):
+    print("large reference order, scheme", scheme, extra, flush=True)
+    check_bit_exact(large, 120, scheme, **extra)
 
 
 @pytest.mark.timeout(600)
 def test_large_benchmark_solve_to_1e6(large):
-    """The benchmark's solve: mixed precision, Halpern scheme, autotuned tiles."""
-    print("large LP full solve (about 100 s)", flush=True)
+    """The benchmark's solve: mixed precision, Halpern scheme, row weighting,
+    autotuned tiles."""
+    print("large LP full solve (about 50 s)", flush=True)
     r = mp.solve(large, tolerance=1e-6, backend=mp.BACKEND_GPU, require_gpu=1, precision=mp.PRECISION_MIXED,
-                 restart_scheme=mp.SCHEME_HALPERN, autotune=1, max_iterations=2_000_000)
+                 restart_scheme=mp.SCHEME_HALPERN, autotune=1, max_iterations=2_000_000, **BENCH_SINGLE)
     assert r["backend_actually_used"] == "hip"
     # 11.1M nonzeros: an FP32 phase, then the packed stream
     assert r["fp32_iterations"] > 0 and r["packed_iterations"] > 0
@@ -159,8 +171,10 @@ def batch():
 
 
 @pytest.mark.timeout(600)
-def test_batch_reference_order_200_iterations(batch):
-    kw = dict(tolerance=1e-6, max_iterations=200)
+@pytest.mark.parametrize("extra", [{}, dict(restart_scheme=mp.SCHEME_HALPERN, **BENCH_BATCH)], ids=["reference", "bench"])
+def test_batch_reference_order_200_iterations(batch, extra):
+    kw = dict(tolerance=1e-6, max_iterations=200, **extra)
+    print("batch reference order", extra, flush=True)
     got = mp.solve_batch(batch, backend=mp.BACKEND_GPU, require_gpu=1, reduction_order=mp.ORDER_REFERENCE, **kw)
     for k, (lp, g) in enumerate(zip(batch, got)):
         w = orc.solve(lp, **kw)
@@ -173,7 +187,8 @@ def test_batch_reference_order_200_iterations(batch):
 @pytest.mark.timeout(600)
 def test_batch_solve_to_1e6(batch):
     got = mp.solve_batch(batch, tolerance=1e-6, backend=mp.BACKEND_GPU, require_gpu=1,
-                         precision=mp.PRECISION_MIXED, restart_scheme=mp.SCHEME_HALPERN, max_iterations=2_000_000)
+                         precision=mp.PRECISION_MIXED, restart_scheme=mp.SCHEME_HALPERN, max_iterations=2_000_000,
+                         **BENCH_BATCH)
     worst = [0.0, 0.0]
     for lp, r in zip(batch, got):
         score, err = check_solution(lp, r)
@@ -228,5 +243,8 @@ def test_scale_ruiz_and_spmv(scale):
 
 
 @pytest.mark.timeout(600)
-def test_scale_reference_order_80_iterations(scale):
-    check_bit_exact(scale, 80, mp.SCHEME_REFERENCE)
+@pytest.mark.parametrize("scheme,extra", [(mp.SCHEME_REFERENCE, {}), (mp.SCHEME_HALPERN, BENCH_SINGLE)],
+                         ids=["reference", "bench"])
+def test_scale_reference_order_80_iterations(scale, scheme, extra):
+    print("scale reference order", scheme, extra, flush=True)
+    check_bit_exact(scale, 80, scheme, **extra)
