@@ -2,13 +2,17 @@
 // HIP engine (include/mcpdlp_engine.h), runs it and maps the outcome back to
 // PdlpResult (reference pdlp.hpp:45-63 semantics, backend "hip").
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -225,83 +229,120 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     const bool relabel = opt.reduction_order != ReductionOrder::reference;
     std::vector<int32_t> row_perm(m), col_perm(n);
     const double inf = std::numeric_limits<double>::infinity();
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    {
-        int64_t r0 = 0, c0 = 0, p0 = 0;
-        for (int b = 0; b < B; ++b) {
-            const mc_lp& lp = lps[live[b]];
-            inst_row[b] = (int32_t)r0;
-            inst_col[b] = (int32_t)c0;
-            const int64_t lm = lp.rows, ln = lp.cols, lnz = lp.column_start[ln];
-            const int th = B == 1 ? hw : 1;
-            std::vector<int32_t> cperm(ln), rinv(lm);
-            if (relabel && !rows_are_local(lp)) {
-                std::vector<int64_t> clen(ln), rlen(lm, 0);
-                for (int64_t j = 0; j < ln; ++j) clen[j] = lp.column_start[j + 1] - lp.column_start[j];
-                for (int64_t k = 0; k < lnz; ++k) ++rlen[lp.row_index[k]];
-                cperm = bucket_order(clen);
-                const auto rperm = bucket_order(rlen);
-                for (int64_t i = 0; i < lm; ++i) {
-                    rinv[rperm[i]] = (int32_t)i;
-                    row_perm[r0 + i] = (int32_t)(r0 + rperm[i]);
-                }
-            } else {
-                for (int64_t j = 0; j < ln; ++j) cperm[j] = (int32_t)j;
-                for (int64_t i = 0; i < lm; ++i) {
-                    rinv[i] = (int32_t)i;
-                    row_perm[r0 + i] = (int32_t)(r0 + i);
-                }
-            }
-            // local CSC in the new labels (entries of a column sorted by new row)
-            std::vector<int32_t> lcs(ln + 1), lri(lnz);
-            std::vector<double> lcv(lnz);
-            lcs[0] = 0;
-            for (int64_t j = 0; j < ln; ++j) {
-                const int64_t oj = cperm[j];
-                lcs[j + 1] = lcs[j] + (int32_t)(lp.column_start[oj + 1] - lp.column_start[oj]);
-            }
-            parallel_for(ln, th, [&](int64_t j0, int64_t j1) {
-                std::vector<std::pair<int32_t, double>> tmp;
-                for (int64_t j = j0; j < j1; ++j) {
-                    const int64_t oj = cperm[j];
-                    const int64_t a = lp.column_start[oj], e = lp.column_start[oj + 1];
-                    tmp.clear();
-                    for (int64_t k = a; k < e; ++k) tmp.emplace_back(rinv[lp.row_index[k]], lp.value[k]);
-                    if (relabel) std::sort(tmp.begin(), tmp.end(), [](auto& u, auto& v) { return u.first < v.first; });
-                    for (int64_t k = 0; k < e - a; ++k) {
-                        lri[lcs[j] + k] = tmp[k].first;
-                        lcv[lcs[j] + k] = tmp[k].second;
-                    }
-                }
-            });
-            for (int64_t j = 0; j <= ln; ++j) cp[c0 + j] = (int32_t)(p0 + lcs[j]);
-            for (int64_t k = 0; k < lnz; ++k) ri[p0 + k] = (int32_t)(r0 + lri[k]);
-            std::memcpy(cv.data() + p0, lcv.data(), sizeof(double) * lnz);
-            csc_to_csr<int32_t>(lm, ln, lcs.data(), lri.data(), lcv.data(), rp.data() + r0, ci.data() + p0,
-                                rv.data() + p0, th);
-            for (int64_t i = 0; i <= lm; ++i) rp[r0 + i] += (int32_t)p0;
-            for (int64_t k = 0; k < lnz; ++k) ci[p0 + k] += (int32_t)c0;
-            for (int64_t i = 0; i < lm; ++i) {
-                const int64_t oi = row_perm[r0 + i] - r0;
-                rlo[r0 + i] = bound_value(lp.row_lower[oi], -inf);
-                rhi[r0 + i] = bound_value(lp.row_upper[oi], inf);
-            }
-            for (int64_t j = 0; j < ln; ++j) {
-                const int64_t oj = cperm[j];
-                col_perm[c0 + j] = (int32_t)(c0 + oj);
-                vlo[c0 + j] = bound_value(lp.var_lower[oj], -inf);
-                vhi[c0 + j] = bound_value(lp.var_upper[oj], inf);
-                obj[c0 + j] = lp.objective[oj];
-            }
-            sense[b] = lp.sense == MC_MAXIMIZE ? -1.0 : 1.0;
-            offset[b] = lp.objective_offset;
-            r0 += lm;
-            c0 += ln;
-            p0 += lnz;
-        }
-        inst_row[B] = (int32_t)r0;
-        inst_col[B] = (int32_t)c0;
+    // host threads for packing: at most 16, like the verifier's and the name audit's pools
+    const int hw = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    // Instance b occupies rows [r0s[b], r0s[b+1]), columns [c0s[b], c0s[b+1])
+    // and nonzeros [p0s[b], p0s[b+1]) of the packed arrays; instances write
+    // disjoint ranges, so a batch packs its instances on parallel threads
+    // (largest first) and a single LP threads inside its own packing.
+    std::vector<int64_t> r0s(B + 1, 0), c0s(B + 1, 0), p0s(B + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        const mc_lp& lp = lps[live[b]];
+        r0s[b + 1] = r0s[b] + lp.rows;
+        c0s[b + 1] = c0s[b] + lp.cols;
+        p0s[b + 1] = p0s[b] + lp.column_start[lp.cols];
     }
+    auto pack_one = [&](int b, int th) {
+        const mc_lp& lp = lps[live[b]];
+        const int64_t r0 = r0s[b], c0 = c0s[b], p0 = p0s[b];
+        inst_row[b] = (int32_t)r0;
+        inst_col[b] = (int32_t)c0;
+        const int64_t lm = lp.rows, ln = lp.cols, lnz = lp.column_start[ln];
+        std::vector<int32_t> cperm(ln), rinv(lm);
+        if (relabel && !rows_are_local(lp)) {
+            std::vector<int64_t> clen(ln), rlen(lm, 0);
+            for (int64_t j = 0; j < ln; ++j) clen[j] = lp.column_start[j + 1] - lp.column_start[j];
+            for (int64_t k = 0; k < lnz; ++k) ++rlen[lp.row_index[k]];
+            cperm = bucket_order(clen);
+            const auto rperm = bucket_order(rlen);
+            for (int64_t i = 0; i < lm; ++i) {
+                rinv[rperm[i]] = (int32_t)i;
+                row_perm[r0 + i] = (int32_t)(r0 + rperm[i]);
+            }
+        } else {
+            for (int64_t j = 0; j < ln; ++j) cperm[j] = (int32_t)j;
+            for (int64_t i = 0; i < lm; ++i) {
+                rinv[i] = (int32_t)i;
+                row_perm[r0 + i] = (int32_t)(r0 + i);
+            }
+        }
+        // local CSC in the new labels (entries of a column sorted by new row)
+        std::vector<int32_t> lcs(ln + 1), lri(lnz);
+        std::vector<double> lcv(lnz);
+        lcs[0] = 0;
+        for (int64_t j = 0; j < ln; ++j) {
+            const int64_t oj = cperm[j];
+            lcs[j + 1] = lcs[j] + (int32_t)(lp.column_start[oj + 1] - lp.column_start[oj]);
+        }
+        parallel_for(ln, th, [&](int64_t j0, int64_t j1) {
+            std::vector<std::pair<int32_t, double>> tmp;
+            for (int64_t j = j0; j < j1; ++j) {
+                const int64_t oj = cperm[j];
+                const int64_t a = lp.column_start[oj], e = lp.column_start[oj + 1];
+                tmp.clear();
+                for (int64_t k = a; k < e; ++k) tmp.emplace_back(rinv[lp.row_index[k]], lp.value[k]);
+                if (relabel) std::sort(tmp.begin(), tmp.end(), [](auto& u, auto& v) { return u.first < v.first; });
+                for (int64_t k = 0; k < e - a; ++k) {
+                    lri[lcs[j] + k] = tmp[k].first;
+                    lcv[lcs[j] + k] = tmp[k].second;
+                }
+            }
+        });
+        // entries [r0, r0 + lm) of rp and [c0, c0 + ln) of cp only: the next
+        // instance owns the boundary entry (rp[m] and cp[n] are set after all)
+        for (int64_t j = 0; j < ln; ++j) cp[c0 + j] = (int32_t)(p0 + lcs[j]);
+        for (int64_t k = 0; k < lnz; ++k) ri[p0 + k] = (int32_t)(r0 + lri[k]);
+        std::memcpy(cv.data() + p0, lcv.data(), sizeof(double) * lnz);
+        std::vector<int32_t> lrp(lm + 1);
+        csc_to_csr<int32_t>(lm, ln, lcs.data(), lri.data(), lcv.data(), lrp.data(), ci.data() + p0, rv.data() + p0,
+                            th);
+        for (int64_t i = 0; i < lm; ++i) rp[r0 + i] = lrp[i] + (int32_t)p0;
+        for (int64_t k = 0; k < lnz; ++k) ci[p0 + k] += (int32_t)c0;
+        for (int64_t i = 0; i < lm; ++i) {
+            const int64_t oi = row_perm[r0 + i] - r0;
+            rlo[r0 + i] = bound_value(lp.row_lower[oi], -inf);
+            rhi[r0 + i] = bound_value(lp.row_upper[oi], inf);
+        }
+        for (int64_t j = 0; j < ln; ++j) {
+            const int64_t oj = cperm[j];
+            col_perm[c0 + j] = (int32_t)(c0 + oj);
+            vlo[c0 + j] = bound_value(lp.var_lower[oj], -inf);
+            vhi[c0 + j] = bound_value(lp.var_upper[oj], inf);
+            obj[c0 + j] = lp.objective[oj];
+        }
+        sense[b] = lp.sense == MC_MAXIMIZE ? -1.0 : 1.0;
+        offset[b] = lp.objective_offset;
+    };
+    if (B == 1) {
+        pack_one(0, hw);
+    } else {
+        std::vector<int> order(B);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int u, int v) { return p0s[u + 1] - p0s[u] > p0s[v + 1] - p0s[v]; });
+        std::atomic<int> next{0};
+        std::exception_ptr failure;
+        std::mutex failure_mu;
+        auto worker = [&] {
+            for (int q = next.fetch_add(1); q < B; q = next.fetch_add(1)) {
+                try {
+                    pack_one(order[q], 1);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lock(failure_mu);
+                    if (!failure) failure = std::current_exception();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < std::min(hw, B); ++t) pool.emplace_back(worker);
+        worker();
+        for (auto& t : pool) t.join();
+        if (failure) std::rethrow_exception(failure);
+    }
+    inst_row[B] = (int32_t)r0s[B];
+    inst_col[B] = (int32_t)c0s[B];
+    rp[m] = (int32_t)nnz;
+    cp[n] = (int32_t)nnz;
     mcp_problem pb{};
     pb.instances = B;
     pb.rows = (int32_t)m;
