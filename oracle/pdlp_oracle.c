@@ -25,6 +25,8 @@
  *                               by planted and Netlib optima, and the Halpern
  *                               update by three iterations worked by hand
  *   or_verify_primal         -- hand cases + compiled reference
+ *   or_halpern_weight        -- closed forms (long-double powers), both sides
+ *                               of the guard, the clamp, the drop limit
  *   or_row_weight,           -- closed forms of (1 + beta)^(-k/8); the weighted
  *   or_apply_row_weights        matrix, scales and bounds entry for entry;
  *                               weighted solves reach planted optima
