@@ -52,11 +52,11 @@ def left():
 METRIC = "time to 1e-6 rel KKT + SpMV HBM GB/s; mixed/batched/warm vs plain PDLP, 1 GPU"
 
 WORKLOADS = {
-    "large": dict(family=2, spec={}, tol=1e-6,
+    "large": dict(family=2, spec={}, tol=1e-6, row_weighting=2,
                   name="single LP m=200000 n=500000, Zipf(1.2) nnz/col in [1,200], 70% '=' rows"),
-    "cpu-lp": dict(family=1, spec={}, tol=1e-6,
+    "cpu-lp": dict(family=1, spec={}, tol=1e-6, row_weighting=3,
                    name="single LP m=2000 n=5000, 5 nnz/col, '=' rows, 0<=x<=10"),
-    "scale": dict(family=5, spec={}, tol=1e-6,
+    "scale": dict(family=5, spec={}, tol=1e-6, row_weighting=3,
                   name="block-angular LP, 48 periods x (40000 rows x 100000 cols, 8 nnz/col) + 200000 "
                        "linking rows (2 nnz/col), |a| log-uniform [1e-2,1e2]"),
     "batch": dict(tol=1e-6, name="256 planted LPs, m log-uniform [500,20000], n=2.5m, 4-8 nnz/col, "
@@ -113,10 +113,11 @@ def main():
                     help="restart scheme of the measured solves")
     ap.add_argument("--row-weighting", type=int, default=None, choices=range(9), metavar="0..8",
                     help="rows scaled by (1 + |bound|)^(-k/8) after Ruiz in the measured solves (0: the "
-                         "reference's scaling; the plain FP64 comparison always uses 0). Default 3 for the "
-                         "single LPs, 0 for the batch and the warm family (uniform LPs whose objectives cancel "
-                         "larger terms: the weights' looser absolute residuals on rows with large bounds move "
-                         "their objectives by more than 1e-6)")
+                         "reference's scaling; the plain FP64 comparison always uses 0). Default: 2 for the "
+                         "Zipf LP (large), 3 for the uniform and block-angular LPs (cpu-lp, scale), the values "
+                         "measured best on each; 0 for the batch and the warm family (uniform LPs whose "
+                         "objectives cancel larger terms: the weights' looser absolute residuals on rows with "
+                         "large bounds move their objectives by more than 1e-6)")
     ap.add_argument("--primal-weight-max-drop", type=int, default=None, metavar="K",
                     help="Halpern scheme: a restart lowers the primal weight by at most K (a power of two; 0: no "
                          "limit). Default 2 for the batch of small LPs, 0 for the single LPs and the warm family")
@@ -165,7 +166,7 @@ def main():
     prec = PRECISIONS[args.precision]
     common = dict(tolerance=tol, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=SCHEMES[args.scheme],
                   graph_chunk=args.graph_chunk, autotune=0 if args.no_autotune else 1,
-                  row_weighting=row_weighting(args, 3), primal_weight_max_drop=max_drop(args, 0))
+                  row_weighting=row_weighting(args, wl["row_weighting"]), primal_weight_max_drop=max_drop(args, 0))
     opts = mp.options(precision=prec, max_iterations=args.max_iterations, **common)
 
     def run(o):
@@ -264,7 +265,7 @@ def main():
         "dtype": DTYPES[args.precision], "data": "synthetic (planted-feasible, seeded)",
         "config": {"workload": wl["name"], "m": lp.rows, "n": lp.cols, "nnz": lp.nnz, "tolerance": tol,
                    "precision": args.precision, "restart_scheme": args.scheme,
-                   "row_weighting": row_weighting(args, 3), "primal_weight_max_drop": max_drop(args, 0),
+                   "row_weighting": row_weighting(args, wl["row_weighting"]), "primal_weight_max_drop": max_drop(args, 0),
                    "parallelism": "single GPU"},
         "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup_iterations": args.warmup_iterations, "budget_s": args.budget_s,
