@@ -132,7 +132,9 @@ typedef struct mc_pdlp_options {
     int32_t ruiz_scaling;             /* 1 */
     int32_t autotune;                 /* extension: time SpMV tile shapes on the device first (0) */
     uint64_t ruiz_iterations;         /* 10 */
-    /* --- extensions (HIP engine only) --- */
+    /* --- extensions: no counterpart in the reference's PdlpOptions (pdlp.hpp:19-34); precision,
+       reduction_order, polish_threshold, graph_chunk, device, require_gpu and fp32_stall_checks act on
+       the HIP engine only, the others on both backends --- */
     int32_t precision;                /* MC_PRECISION_FP64 */
     int32_t reduction_order;          /* MC_ORDER_FAST */
     double polish_threshold;          /* mixed: switch to FP64 once the FP64-checked score <= this (0 => only on stall) */
@@ -151,9 +153,9 @@ typedef struct mc_pdlp_options {
     int32_t row_weighting;            /* k in 0..8: after the Ruiz passes, row i of A and its bounds are scaled by
                                          (1 + beta_i)^(-k/8), beta_i the largest magnitude of its finite original
                                          bounds; the reported residuals and vectors stay those of the original
-                                         model (0: the reference's scaling) */
+                                         model (0: the reference's scaling); both backends */
     int32_t primal_weight_max_drop;   /* Halpern scheme: a restart lowers the primal weight by at most this factor
-                                         (0: no limit; else a power of two in 2..2^20) */
+                                         (0: no limit; else a power of two in 2..2^20); both backends */
 } mc_pdlp_options;
 
 /* PdlpStatus (reference: pdlp.hpp:43). */
