@@ -186,64 +186,79 @@ double bound_value(const mc_bound& b, double open) { return b.kind == MC_BOUND_F
 
 }  // namespace
 
-std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptions& opt) {
-    const auto t0 = std::chrono::steady_clock::now();
-    std::vector<PdlpResult> out(lps.size());
-    std::vector<int> live;  // instances that go to the device (m > 0 and n > 0)
-    for (std::size_t k = 0; k < lps.size(); ++k) {
-        auto& r = out[k];
-        r.tolerance = opt.primal_tolerance;
-        if (lps[k].rows == 0 || lps[k].cols == 0) {
-            r.status = PdlpStatus::optimal;
-            r.message = "trivial";
-            r.backend_actually_used = "cpu_fallback";  // nothing ran on the device
-        } else {
-            live.push_back((int)k);
-        }
-    }
-    if (live.empty()) return out;
-    mcp_engine* eng = engine_for(opt.device);
-    if (opt.reduction_order == ReductionOrder::reference && opt.precision != Precision::fp64)
-        throw std::invalid_argument("reference reduction order requires FP64 precision");
-
-    // ---- pack (block-diagonal) ----
-    const int B = (int)live.size();
+// Block-diagonal device layout of a batch of LPs (every one with m, n > 0):
+// CSR(A) and CSR(A^T) with int32 indices, bounds with IEEE infinities, each
+// instance's rows and columns contiguous.  In the fast reduction order
+// (relabel) rows and columns are relabelled into length buckets
+// (row_perm[new] = old, col_perm[new] = old, both global), so each SpMV tile
+// holds rows of similar length -- unless the instance's rows already have
+// column locality (rows_are_local): then a tile of consecutive rows gathers
+// from a narrow window of x_bar, and the bucket order would interleave rows
+// from all over the matrix.  The reference order keeps the model's labels:
+// its sums must run in the reference's index order.
+PackedBatch pack_batch(const std::vector<mc_lp>& lps, bool relabel, int threads) {
+    const int B = (int)lps.size();
     int64_t m = 0, n = 0, nnz = 0;
-    for (int k : live) {
-        m += lps[k].rows;
-        n += lps[k].cols;
-        nnz += lps[k].column_start[lps[k].cols];
+    for (const auto& lp : lps) {
+        m += lp.rows;
+        n += lp.cols;
+        nnz += lp.column_start[lp.cols];
     }
     constexpr int64_t kMax = std::numeric_limits<int32_t>::max();
     if (m >= kMax || n >= kMax || nnz >= kMax) throw std::invalid_argument("problem exceeds int32 indexing");
-    std::vector<int32_t> inst_row(B + 1), inst_col(B + 1), rp(m + 1), ci(nnz), cp(n + 1), ri(nnz);
-    std::vector<double> rv(nnz), cv(nnz), rlo(m), rhi(m), vlo(n), vhi(n), obj(n), sense(B), offset(B);
-    // In the fast reduction order rows and columns are relabelled into length
-    // buckets (row_perm[new] = old, col_perm[new] = old, per instance), so each
-    // SpMV tile holds rows of similar length -- unless the instance's rows
-    // already have column locality (rows_are_local): then a tile of
-    // consecutive rows gathers from a narrow window of x_bar, and the bucket
-    // order would interleave rows from all over the matrix.  The reference
-    // order keeps the model's labels: its sums must run in the reference's
-    // index order.
-    const bool relabel = opt.reduction_order != ReductionOrder::reference;
-    std::vector<int32_t> row_perm(m), col_perm(n);
+    PackedBatch pk;
+    pk.m = m;
+    pk.n = n;
+    pk.nnz = nnz;
+    auto& inst_row = pk.inst_row;
+    auto& inst_col = pk.inst_col;
+    auto& rp = pk.rp;
+    auto& ci = pk.ci;
+    auto& cp = pk.cp;
+    auto& ri = pk.ri;
+    auto& rv = pk.rv;
+    auto& cv = pk.cv;
+    auto& rlo = pk.rlo;
+    auto& rhi = pk.rhi;
+    auto& vlo = pk.vlo;
+    auto& vhi = pk.vhi;
+    auto& obj = pk.obj;
+    auto& sense = pk.sense;
+    auto& offset = pk.offset;
+    auto& row_perm = pk.row_perm;
+    auto& col_perm = pk.col_perm;
+    inst_row.resize(B + 1);
+    inst_col.resize(B + 1);
+    rp.resize(m + 1);
+    ci.resize(nnz);
+    cp.resize(n + 1);
+    ri.resize(nnz);
+    rv.resize(nnz);
+    cv.resize(nnz);
+    rlo.resize(m);
+    rhi.resize(m);
+    vlo.resize(n);
+    vhi.resize(n);
+    obj.resize(n);
+    sense.resize(B);
+    offset.resize(B);
+    row_perm.resize(m);
+    col_perm.resize(n);
     const double inf = std::numeric_limits<double>::infinity();
-    // host threads for packing: at most 16, like the verifier's and the name audit's pools
-    const int hw = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    const int hw = std::max(1, threads);
     // Instance b occupies rows [r0s[b], r0s[b+1]), columns [c0s[b], c0s[b+1])
     // and nonzeros [p0s[b], p0s[b+1]) of the packed arrays; instances write
     // disjoint ranges, so a batch packs its instances on parallel threads
     // (largest first) and a single LP threads inside its own packing.
     std::vector<int64_t> r0s(B + 1, 0), c0s(B + 1, 0), p0s(B + 1, 0);
     for (int b = 0; b < B; ++b) {
-        const mc_lp& lp = lps[live[b]];
+        const mc_lp& lp = lps[b];
         r0s[b + 1] = r0s[b] + lp.rows;
         c0s[b + 1] = c0s[b] + lp.cols;
         p0s[b + 1] = p0s[b] + lp.column_start[lp.cols];
     }
     auto pack_one = [&](int b, int th) {
-        const mc_lp& lp = lps[live[b]];
+        const mc_lp& lp = lps[b];
         const int64_t r0 = r0s[b], c0 = c0s[b], p0 = p0s[b];
         inst_row[b] = (int32_t)r0;
         inst_col[b] = (int32_t)c0;
@@ -343,6 +358,55 @@ std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptio
     inst_col[B] = (int32_t)c0s[B];
     rp[m] = (int32_t)nnz;
     cp[n] = (int32_t)nnz;
+    return pk;
+}
+
+std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptions& opt) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<PdlpResult> out(lps.size());
+    std::vector<int> live;  // instances that go to the device (m > 0 and n > 0)
+    for (std::size_t k = 0; k < lps.size(); ++k) {
+        auto& r = out[k];
+        r.tolerance = opt.primal_tolerance;
+        if (lps[k].rows == 0 || lps[k].cols == 0) {
+            r.status = PdlpStatus::optimal;
+            r.message = "trivial";
+            r.backend_actually_used = "cpu_fallback";  // nothing ran on the device
+        } else {
+            live.push_back((int)k);
+        }
+    }
+    if (live.empty()) return out;
+    mcp_engine* eng = engine_for(opt.device);
+    if (opt.reduction_order == ReductionOrder::reference && opt.precision != Precision::fp64)
+        throw std::invalid_argument("reference reduction order requires FP64 precision");
+
+    // ---- pack (block-diagonal) ----
+    const int B = (int)live.size();
+    std::vector<mc_lp> live_lps;
+    live_lps.reserve(B);
+    for (int k : live) live_lps.push_back(lps[k]);
+    // host threads for packing: at most 16, like the verifier's and the name audit's pools
+    const int hw = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    PackedBatch pk = pack_batch(live_lps, opt.reduction_order != ReductionOrder::reference, hw);
+    const int64_t m = pk.m, n = pk.n, nnz = pk.nnz;
+    auto& inst_row = pk.inst_row;
+    auto& inst_col = pk.inst_col;
+    auto& row_perm = pk.row_perm;
+    auto& col_perm = pk.col_perm;
+    auto& rp = pk.rp;
+    auto& ci = pk.ci;
+    auto& rv = pk.rv;
+    auto& cp = pk.cp;
+    auto& ri = pk.ri;
+    auto& cv = pk.cv;
+    auto& rlo = pk.rlo;
+    auto& rhi = pk.rhi;
+    auto& vlo = pk.vlo;
+    auto& vhi = pk.vhi;
+    auto& obj = pk.obj;
+    auto& sense = pk.sense;
+    auto& offset = pk.offset;
     mcp_problem pb{};
     pb.instances = B;
     pb.rows = (int32_t)m;

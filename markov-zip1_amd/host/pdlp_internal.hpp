@@ -26,6 +26,20 @@ void validate_options(const PdlpOptions& opt);
 model::Model model_of(const mc_lp& lp);
 
 PdlpResult solve_cpu(const model::Model& md, const PdlpOptions& opt, const char* backend_label);
+
+// Block-diagonal layout of a batch for mcp_load: instance b holds rows
+// [inst_row[b], inst_row[b+1]) and columns [inst_col[b], inst_col[b+1]);
+// rp/ci/rv is CSR(A), cp/ri/cv CSR(A^T), row_perm[new] / col_perm[new] the
+// original (global) index of each packed row / column.
+struct PackedBatch {
+    int64_t m = 0, n = 0, nnz = 0;
+    std::vector<int32_t> inst_row, inst_col, rp, ci, cp, ri, row_perm, col_perm;
+    std::vector<double> rv, cv, rlo, rhi, vlo, vhi, obj, sense, offset;
+};
+// Packs LPs with m, n > 0 on up to `threads` host threads (one instance per
+// thread in a batch; a single LP threads inside its own packing), relabelled
+// into row/column length buckets when `relabel` (the fast reduction order).
+PackedBatch pack_batch(const std::vector<mc_lp>& lps, bool relabel, int threads);
 std::vector<PdlpResult> solve_gpu(const std::vector<mc_lp>& lps, const PdlpOptions& opt);
 
 // CSC -> CSR (int32), entries of a row in increasing column order;
