@@ -138,3 +138,19 @@ def test_max_drop_batch_reference_order_and_fast_solve():
         assert r["status"] == "optimal"
         assert orc.kkt(lp, r["primal"], r["dual"])["score"] <= 1.5 * tol
         assert abs(r["objective"] - lp.planted_objective) <= 1e-6 * (1.0 + abs(lp.planted_objective))
+
+
+@pytest.mark.parametrize("k", [0, 3])
+def test_warm_start_from_the_solution_stops_at_the_first_check(k):
+    """On the device, a warm start at a solution found to 1e-9 is optimal at
+    the first check (40 iterations), with and without row weighting: warm_x
+    and warm_y enter the scaled space through the weighted row scales."""
+    lp = uniform()
+    kw = dict(tolerance=1e-9, backend=mp.BACKEND_GPU, require_gpu=1, restart_scheme=mp.SCHEME_HALPERN,
+              row_weighting=k, max_iterations=200000)
+    ref = mp.solve(lp, **kw)
+    assert ref["status"] == "optimal"
+    r = mp.solve(lp, **dict(kw, tolerance=1e-6), warm_primal=ref["primal"], warm_dual=ref["dual"],
+                 initial_primal_weight=ref["primal_weight"])
+    print("warm iterations", r["iterations"], flush=True)
+    assert r["status"] == "optimal" and r["iterations"] == 40, r["iterations"]

@@ -386,3 +386,19 @@ def test_primal_weight_max_drop_must_be_a_power_of_two():
     p = subprocess.run([mp.CLI, os.path.join(GOLDEN, "tiny_simplex.mps"), "--primal-weight-max-drop", "3"],
                        capture_output=True, text=True, timeout=60)
     assert p.returncode == 4 and "power of two" in p.stderr
+
+
+@pytest.mark.parametrize("k", [0, 3])
+def test_warm_start_from_the_solution_stops_at_the_first_check(k):
+    """A warm start (warm_primal / warm_dual, original space) at a solution
+    found to 1e-9 is optimal at the first check (40 iterations) on the host
+    path, with and without row weighting: the start is mapped into the scaled
+    space through the same row scales (Ruiz times the weights) that map the
+    answer back."""
+    lp = mp.generate(mp.FAMILY_UNIFORM_EQ, rows=200, cols=500, seed=11)
+    kw = dict(tolerance=1e-9, restart_scheme=mp.SCHEME_HALPERN, row_weighting=k, max_iterations=200000)
+    ref = mp.solve(lp, **kw)
+    assert ref["status"] == "optimal"
+    r = mp.solve(lp, **dict(kw, tolerance=1e-6), warm_primal=ref["primal"], warm_dual=ref["dual"],
+                 initial_primal_weight=ref["primal_weight"])
+    assert r["status"] == "optimal" and r["iterations"] == 40, r["iterations"]
